@@ -1,0 +1,402 @@
+"""bfam_amd -- MI355X-native DG right-hand side and LSRK time stepping.
+
+Python front end over the C ABI in include/bfamx.h (libbfamx.so, built
+in-tree by ``python -m bfam_amd.build``).  It mirrors the pieces of bfam a
+caller touches on the hot path: the bfam_subdomain_dgx data (bfam.h:1364-1420),
+bfamo_lsrk_init (bfam.c:8996-9090) and the device layer (bfam.c:8389-9092).
+There is no CPU fallback: every compute call goes to the HIP library, and a
+missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbfamx.so")
+
+ACOUSTIC2D, ELASTIC3D = 0, 1
+FP64, FP64_PLAIN, FP32_RHS, FP32 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp64_plain": FP64_PLAIN, "fp32_rhs": FP32_RHS, "fp32": FP32}
+LSRK_KC54, LSRK_FE, LSRK_HEUN, LSRK_W33, LSRK_NOOP = 0, 1, 2, 3, 4
+OK, EINVAL, EBUDGET, EHIP, ESTATE, EUNSUPPORTED = 0, 1, 2, 3, 4, 5
+
+ELASTIC_FIELDS = ("v1", "v2", "v3", "s11", "s22", "s33", "s12", "s13", "s23")
+ACOUSTIC_FIELDS = ("p", "u", "v")
+
+_lib = None
+
+
+class BfamxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bfamx error {code}: {msg}")
+        self.code = code
+
+
+class Lsrk(C.Structure):
+    _fields_ = [("n_stages", C.c_int), ("A", C.c_double * 8), ("B", C.c_double * 8),
+                ("C", C.c_double * 9)]
+
+
+class Dgx(C.Structure):
+    _fields_ = [("dim", C.c_int), ("N", C.c_int), ("Np", C.c_int), ("K", C.c_int32),
+                ("r", C.c_void_p), ("w", C.c_void_p), ("wi", C.c_void_p), ("Dr", C.c_void_p),
+                ("vmapM", C.c_void_p), ("vmapP", C.c_void_p)]
+
+
+class PhysDesc(C.Structure):
+    _fields_ = [("physics", C.c_int), ("precision", C.c_int),
+                ("elem_h", C.c_void_p), ("elem_x0", C.c_void_p),
+                ("rho", C.c_void_p), ("cp", C.c_void_p), ("cs", C.c_void_p)]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+
+# exported symbol -> (restype, argtypes); also the list the ABI tests check
+SIGNATURES = {
+    "bfamx_last_error": (C.c_char_p, []),
+    "bfamx_set_abort_on_error": (None, [C.c_int]),
+    "bfamx_lsrk_init": (C.c_int, [C.POINTER(Lsrk), C.c_int]),
+    "bfamx_lgl": (C.c_int, [C.c_int, _dp, _dp]),
+    "bfamx_diff_matrix": (C.c_int, [C.c_int, _dp, _dp]),
+    "bfamx_lagrange_eval": (C.c_int, [C.c_int, _dp, C.c_double, _dp]),
+    "bfamx_mortar_ops": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]),
+    "bfamx_face_mask": (C.c_int, [C.c_int, C.c_int, _ip]),
+    "bfamx_build_vmaps": (C.c_int, [C.c_int, C.c_int, C.c_int32, _ip, _i8p, _ip, _ip]),
+    "bfamx_face_connectivity": (C.c_int, [C.c_int, C.c_int, C.c_int32, _ip, _ip, _ip, _u8p]),
+    "bfamx_pcg32_seed": (None, [_vp, C.c_uint64, C.c_uint64]),
+    "bfamx_pcg32_next": (C.c_uint32, [_vp]),
+    "bfamx_pcg32_bounded": (C.c_uint32, [_vp, C.c_uint32]),
+    "bfamx_pcg32_uniform": (C.c_double, [_vp]),
+    "bfamx_brick": (C.c_int, [C.c_int, _ip, _ip, _dp, C.c_int, _ip, _i8p, _ip, _dp, _dp]),
+    "bfamx_device_create": (C.c_int, [C.c_int, C.c_size_t, _PP]),
+    "bfamx_device_destroy": (C.c_int, [_vp]),
+    "bfamx_device_memory": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "bfamx_device_synchronize": (C.c_int, [_vp]),
+    "bfamx_device_stream": (C.c_void_p, [_vp]),
+    "bfamx_device_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bfamx_subdomain_upload": (C.c_int, [_vp, C.POINTER(Dgx), C.POINTER(PhysDesc),
+                                         C.POINTER(Lsrk), _PP]),
+    "bfamx_subdomain_free": (C.c_int, [_vp]),
+    "bfamx_subdomain_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+    "bfamx_subdomain_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int32, C.c_int, C.c_int]),
+    "bfamx_set_source": (C.c_int, [_vp, C.c_int32, _dp, _dp, C.c_double, C.c_double]),
+    "bfamx_set_forcing_mms": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
+    "bfamx_set_fields": (C.c_int, [_vp, _PP, C.c_double, C.c_double]),
+    "bfamx_download_fields": (C.c_int, [_vp, _PP]),
+    "bfamx_get_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "bfamx_rhs": (C.c_int, [_vp, C.c_double, _PP]),
+    "bfamx_stage": (C.c_int, [_vp]),
+    "bfamx_step": (C.c_int, [_vp, C.c_int]),
+    "bfamx_reduce_energy": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "bfamx_reduce_l2_diff": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bfamx_reduce_mms_error": (C.c_int, [_vp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bfamx_batch_create": (C.c_int, [_vp, _PP, C.c_int, _PP]),
+    "bfamx_batch_free": (C.c_int, [_vp]),
+    "bfamx_batch_step": (C.c_int, [_vp, C.c_int]),
+    "bfamx_batch_capture": (C.c_int, [_vp, C.c_int]),
+    "bfamx_batch_launch_graph": (C.c_int, [_vp]),
+    "bfamx_timer_start": (C.c_int, [_vp]),
+    "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
+}
+
+
+def lib():
+    """The loaded libbfamx.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m bfam_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        msg = lib().bfamx_last_error()
+        raise BfamxError(rc, msg.decode() if msg else "")
+
+
+def _ptrs(arrays):
+    return (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+# ---------------------------------------------------------------- host setup
+
+def lsrk_init(method=LSRK_KC54):
+    t = Lsrk()
+    _check(lib().bfamx_lsrk_init(C.byref(t), method))
+    n = t.n_stages
+    return np.array(t.A[:n]), np.array(t.B[:n]), np.array(t.C[:n + 1])
+
+
+def lgl(N):
+    r, w = np.zeros(N + 1), np.zeros(N + 1)
+    _check(lib().bfamx_lgl(N, r, w))
+    return r, w
+
+
+def diff_matrix(N, r):
+    D = np.zeros((N + 1) ** 2)
+    _check(lib().bfamx_diff_matrix(N, np.ascontiguousarray(r, dtype=np.float64), D))
+    return D.reshape(N + 1, N + 1).T.copy()  # D[i, j] = l_j'(r_i)
+
+
+def lagrange_eval(N, r, x):
+    out = np.zeros(N + 1)
+    _check(lib().bfamx_lagrange_eval(N, np.ascontiguousarray(r, dtype=np.float64), float(x), out))
+    return out
+
+
+def mortar_ops(Na, Nb):
+    na, nb = Na + 1, Nb + 1
+    P, MP, WMP = np.zeros(5 * na * nb), np.zeros(5 * na * nb), np.zeros(5 * na * nb)
+    ident = C.c_int(0)
+    _check(lib().bfamx_mortar_ops(Na, Nb, P, MP, WMP, C.byref(ident)))
+
+    def blocks(a):
+        return [a[k * na * nb:(k + 1) * na * nb].reshape(na, nb).T.copy() for k in range(5)]
+
+    return {"prj": blocks(P), "mass_prj": blocks(MP), "wi_mass_prj": blocks(WMP),
+            "prj0_identity": bool(ident.value)}
+
+
+def face_mask(dim, N):
+    nfp = (N + 1) ** (dim - 1)
+    g = np.zeros(2 * dim * nfp, dtype=np.int32)
+    _check(lib().bfamx_face_mask(dim, N, g))
+    return g.reshape(2 * dim, nfp)
+
+
+def build_vmaps(dim, N, EToE, EToF):
+    EToE = np.ascontiguousarray(EToE, dtype=np.int32).ravel()
+    EToF = np.ascontiguousarray(EToF, dtype=np.int8).ravel()
+    K = EToE.size // (2 * dim)
+    nfp = (N + 1) ** (dim - 1)
+    vM = np.zeros(K * 2 * dim * nfp, dtype=np.int32)
+    vP = np.zeros_like(vM)
+    _check(lib().bfamx_build_vmaps(dim, N, K, EToE, EToF, vM, vP))
+    return vM, vP
+
+
+def face_connectivity(dim, N, vmapM, vmapP):
+    vmapM = np.ascontiguousarray(vmapM, dtype=np.int32)
+    vmapP = np.ascontiguousarray(vmapP, dtype=np.int32)
+    nfp = (N + 1) ** (dim - 1)
+    K = vmapM.size // (2 * dim * nfp)
+    ne = np.zeros(K * 2 * dim, dtype=np.int32)
+    nc = np.zeros(K * 2 * dim, dtype=np.uint8)
+    _check(lib().bfamx_face_connectivity(dim, N, K, vmapM, vmapP, ne, nc))
+    return ne, nc
+
+
+def brick(dim, n, periodic, L=None, morton=True):
+    """Uniform brick: returns dict(EToE, EToF, lex, x0, h, K)."""
+    n3 = np.ascontiguousarray(list(n) + [1] * (3 - len(n)), dtype=np.int32)
+    per = np.ascontiguousarray(list(periodic) + [0] * (3 - len(periodic)), dtype=np.int32)
+    L = np.ascontiguousarray(list(L if L is not None else [1.0] * dim) + [1.0] * (3 - dim),
+                             dtype=np.float64)
+    K = int(np.prod(n3[:dim]))
+    EToE = np.zeros(K * 2 * dim, dtype=np.int32)
+    EToF = np.zeros(K * 2 * dim, dtype=np.int8)
+    lex = np.zeros(K, dtype=np.int32)
+    x0 = np.zeros(K * dim)
+    h = np.zeros(K * dim)
+    _check(lib().bfamx_brick(dim, n3, per, L, 1 if morton else 0, EToE, EToF, lex, x0, h))
+    return {"EToE": EToE, "EToF": EToF, "lex": lex, "x0": x0.reshape(K, dim),
+            "h": h.reshape(K, dim), "K": K, "n": list(n[:dim]), "L": list(L[:dim])}
+
+
+class PCG32:
+    """bfam_pcg32 (O'Neill's PCG32) through the library."""
+
+    def __init__(self, initstate, initseq):
+        self._st = (C.c_uint64 * 2)()
+        lib().bfamx_pcg32_seed(self._st, initstate, initseq)
+
+    def next(self):
+        return lib().bfamx_pcg32_next(self._st)
+
+    def bounded(self, bound):
+        return lib().bfamx_pcg32_bounded(self._st, bound)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * lib().bfamx_pcg32_uniform(self._st)
+
+
+# -------------------------------------------------------------------- device
+
+class Device:
+    def __init__(self, hip_device=0, budget_bytes=0):
+        h = C.c_void_p()
+        _check(lib().bfamx_device_create(hip_device, budget_bytes, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().bfamx_device_destroy(self.h)
+            self.h = None
+
+    def memory(self):
+        u, b = C.c_size_t(), C.c_size_t()
+        _check(lib().bfamx_device_memory(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def info(self):
+        cu, xcd = C.c_int(), C.c_int()
+        _check(lib().bfamx_device_info(self.h, C.byref(cu), C.byref(xcd)))
+        return cu.value, xcd.value
+
+    def sync(self):
+        _check(lib().bfamx_device_synchronize(self.h))
+
+    def timer_start(self):
+        _check(lib().bfamx_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _check(lib().bfamx_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def set_tuning(self, xcd_remap=1):
+        _check(lib().bfamx_set_tuning(self.h, int(xcd_remap)))
+
+
+def subdomain_bytes(dim, N, K, physics, precision):
+    return lib().bfamx_subdomain_bytes(dim, N, K, physics, precision)
+
+
+class Subdomain:
+    """A volume subdomain on the device (bfamx_subdomain_upload)."""
+
+    def __init__(self, dev, physics, precision, N, r, w, Dr, vmapM, vmapP, h, rho, cp,
+                 cs=None, x0=None, lsrk_method=LSRK_KC54):
+        dim = 2 if physics == ACOUSTIC2D else 3
+        self.dev, self.physics, self.precision, self.N, self.dim = dev, physics, precision, N, dim
+        self.nf = 3 if physics == ACOUSTIC2D else 9
+        self.Np = (N + 1) ** dim
+        keep = {}
+
+        def arr(name, a, dt=np.float64):
+            keep[name] = np.ascontiguousarray(a, dtype=dt).ravel()
+            return keep[name].ctypes.data
+
+        wi = 1.0 / np.asarray(w, dtype=np.float64)
+        D = np.asarray(Dr, dtype=np.float64)
+        vM = np.ascontiguousarray(vmapM, dtype=np.int32)
+        self.K = vM.size // (2 * dim * (N + 1) ** (dim - 1))
+        g = Dgx(dim, N, self.Np, self.K, arr("r", r), arr("w", w), arr("wi", wi),
+                arr("Dr", D.T), arr("vM", vM, np.int32), arr("vP", vmapP, np.int32))
+        p = PhysDesc(physics, precision, arr("h", h), arr("x0", x0) if x0 is not None else None,
+                     arr("rho", rho), arr("cp", cp), arr("cs", cs) if cs is not None else None)
+        t = Lsrk()
+        _check(lib().bfamx_lsrk_init(C.byref(t), lsrk_method))
+        hh = C.c_void_p()
+        _check(lib().bfamx_subdomain_upload(dev.h, C.byref(g), C.byref(p), C.byref(t),
+                                            C.byref(hh)))
+        self.h = hh
+        self.nstages = t.n_stages
+
+    def close(self):
+        if self.h:
+            lib().bfamx_subdomain_free(self.h)
+            self.h = None
+
+    def info(self):
+        nf, b = C.c_int(), C.c_size_t()
+        _check(lib().bfamx_subdomain_info(self.h, C.byref(nf), C.byref(b)))
+        return nf.value, b.value
+
+    def set_source(self, elem, weights, amp, f0, t0):
+        a = np.zeros(9)
+        a[:len(amp)] = amp
+        _check(lib().bfamx_set_source(self.h, int(elem),
+                                      np.ascontiguousarray(weights, dtype=np.float64), a,
+                                      float(f0), float(t0)))
+
+    def set_forcing_mms(self, k, omega, F):
+        _check(lib().bfamx_set_forcing_mms(self.h, float(k), float(omega),
+                                           np.ascontiguousarray(F, dtype=np.float64)))
+
+    def set_fields(self, q, t0=0.0, dt=0.0):
+        """q: (nfields, K*Np) in bfam's field layout."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.shape == (self.nf, self.K * self.Np), q.shape
+        rows = [q[f] for f in range(self.nf)]
+        _check(lib().bfamx_set_fields(self.h, _ptrs(rows), float(t0), float(dt)))
+
+    def download(self):
+        q = np.zeros((self.nf, self.K * self.Np))
+        rows = [q[f] for f in range(self.nf)]
+        _check(lib().bfamx_download_fields(self.h, _ptrs(rows)))
+        return q
+
+    def rhs(self, t=0.0):
+        out = np.zeros((self.nf, self.K * self.Np))
+        rows = [out[f] for f in range(self.nf)]
+        _check(lib().bfamx_rhs(self.h, float(t), _ptrs(rows)))
+        return out
+
+    def stage(self):
+        _check(lib().bfamx_stage(self.h))
+
+    def step(self, n=1):
+        _check(lib().bfamx_step(self.h, int(n)))
+
+    def time(self):
+        t, s = C.c_double(), C.c_int64()
+        _check(lib().bfamx_get_time(self.h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def energy(self):
+        e = C.c_double()
+        _check(lib().bfamx_reduce_energy(self.h, C.byref(e)))
+        return e.value
+
+    def l2_diff(self, other):
+        """(sum Jw|self - other|^2, sum Jw|other|^2)"""
+        d, n = C.c_double(), C.c_double()
+        _check(lib().bfamx_reduce_l2_diff(self.h, other.h, C.byref(d), C.byref(n)))
+        return d.value, n.value
+
+    def mms_error(self, A):
+        d, n = C.c_double(), C.c_double()
+        _check(lib().bfamx_reduce_mms_error(self.h, np.ascontiguousarray(A, dtype=np.float64),
+                                            C.byref(d), C.byref(n)))
+        return d.value, n.value
+
+
+class Batch:
+    """Many subdomains advanced by one launch per stage (bfamx_batch_*)."""
+
+    def __init__(self, dev, subs):
+        self.dev, self.subs = dev, list(subs)
+        arr = (C.c_void_p * len(subs))(*[s.h for s in subs])
+        h = C.c_void_p()
+        _check(lib().bfamx_batch_create(dev.h, arr, len(subs), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().bfamx_batch_free(self.h)
+            self.h = None
+
+    def step(self, n=1):
+        _check(lib().bfamx_batch_step(self.h, int(n)))
+
+    def capture(self, nsteps):
+        _check(lib().bfamx_batch_capture(self.h, int(nsteps)))
+
+    def launch_graph(self):
+        _check(lib().bfamx_batch_launch_graph(self.h))

@@ -1,0 +1,274 @@
+/*
+ * Device-side data layout shared by the stage, plain and reduction kernels.
+ *
+ * HBM layout of one volume subdomain (K elements, order N, Np nodes, NF
+ * fields): element-major, field-major inside the element,
+ *     q[e][f][n]   (state, two ping-pong copies for the fused stage)
+ *     dq[e][f][n]  (LSRK rate register)
+ * so a workgroup streams one contiguous NF*Np block per element and every
+ * field row is a fully coalesced run of Np values.  Per-element geometry and
+ * material live in small AoS tables (GEO_STRIDE / MAT_STRIDE entries), the
+ * face connectivity in nbr_elem/nbr_code (one entry per element face).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bfamx
+{
+
+enum
+{
+  GEO_GX = 0, /* 2/h_x, 2/h_y, 2/h_z : reference -> physical derivative */
+  GEO_GY = 1,
+  GEO_GZ = 2,
+  GEO_LX = 3, /* lift factors (sJ/J)/w_end = (2/h_d)/w_0 */
+  GEO_LY = 4,
+  GEO_LZ = 5,
+  GEO_X0 = 6, /* element lower corner */
+  GEO_Y0 = 7,
+  GEO_Z0 = 8,
+  GEO_HX = 9,
+  GEO_HY = 10,
+  GEO_HZ = 11,
+  GEO_STRIDE = 12
+};
+
+enum
+{
+  MAT_RHOI = 0, /* 1/rho */
+  MAT_LAM = 1,  /* lambda (acoustic: bulk modulus) */
+  MAT_MU = 2,
+  MAT_ZP = 3, /* rho cp */
+  MAT_ZS = 4, /* rho cs */
+  MAT_STRIDE = 8
+};
+
+enum
+{
+  MODE_STAGE = 0, /* RHS + LSRK update, clock advanced by the last block */
+  MODE_RHS = 1    /* write L(q, t) as FP64 into rhs, no update */
+};
+
+/* device-resident clock of one subdomain; all dynamic state lives here so a
+ * captured graph can be replayed */
+struct Clock
+{
+  double t;            /* time at the start of the current step */
+  double dt;           /* step size */
+  long long stage;     /* stages completed since set_fields */
+  unsigned int arrive; /* blocks of the current launch that finished */
+  unsigned int pad;
+  double g;            /* ricker(t_stage) of the stage about to run */
+  double sw, cw;       /* sin, cos(omega t_stage) for the manufactured forcing */
+};
+
+/* everything a stage needs about one subdomain (passed by value or read
+ * from a device table in batched launches) */
+struct SubDev
+{
+  int physics;
+  int N;
+  int K;
+  int nf;
+  int nstages;
+  int src_elem; /* -1 none */
+  int mms;      /* 1: manufactured forcing */
+  int pad0;
+  void *q[2];   /* state ping-pong (double or float) */
+  void *dq;     /* rate register (double or float) */
+  double *rhs;  /* FP64 RHS (plain path and MODE_RHS) */
+  const int32_t *nbr_elem;
+  const uint8_t *nbr_code;
+  const double *geo;
+  const float *geo_f;
+  const double *mat;
+  const float *mat_f;
+  const double *Dr; /* (N+1)^2 column-major */
+  const float *Dr_f;
+  const double *r; /* LGL nodes */
+  const double *w; /* LGL weights */
+  const double *src_wt;
+  const float *src_wt_f;
+  const double *mms_tab; /* K x 3 axes x {sin, cos} x (N+1): sin/cos(k x) */
+  Clock *clock;
+  double A[8], B[8], C[9];
+  double src_amp[9];
+  double src_f0, src_t0;
+  double mms_k, mms_omega;
+  double mms_F[9];
+};
+
+/* one work item of a stage launch: elements [e0, e0+ne) of subdomain sub */
+struct Task
+{
+  int sub;
+  int e0;
+  int ne;
+  int N;
+};
+
+struct LaunchArgs
+{
+  const SubDev *subs;       /* device table (batched) */
+  const Task *tasks;        /* device table */
+  const int *sub_ntasks;    /* tasks per subdomain in this launch */
+  int ntasks;
+  int mode;
+  int xcd_remap;
+  int nxcd;
+  double t_rhs;  /* time for MODE_RHS */
+  double g_rhs;  /* ricker(t_rhs) */
+  double sw_rhs; /* sin(omega t_rhs) */
+  double cw_rhs; /* cos(omega t_rhs) */
+};
+
+/* element-major offset of (element, field, node) */
+__device__ __forceinline__ size_t qidx(int e, int f, int n, int NF, int Np)
+{
+  return ((size_t)e * NF + f) * Np + n;
+}
+
+/*
+ * Order in which a block walks the task list.  Hardware dispatches
+ * consecutive workgroups round-robin over the XCDs; with remapping, XCD x
+ * instead owns the contiguous slice [x*C, (x+1)*C) of the (Morton-ordered)
+ * task list, so face neighbours of a task are mostly resident in the same
+ * XCD's L2.
+ */
+__device__ __forceinline__ int first_task(const LaunchArgs &a, int *stride,
+                                          int *end)
+{
+  const int b = blockIdx.x, G = gridDim.x;
+  if (!a.xcd_remap || G % a.nxcd)
+  {
+    *stride = G;
+    *end = a.ntasks;
+    return b;
+  }
+  const int x = b % a.nxcd, s = b / a.nxcd, per = G / a.nxcd;
+  const int C = (a.ntasks + a.nxcd - 1) / a.nxcd;
+  const int lo = x * C;
+  int hi = lo + C;
+  if (hi > a.ntasks)
+    hi = a.ntasks;
+  *stride = per;
+  *end = hi;
+  return lo + s;
+}
+
+/* Ricker wavelet (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2 */
+__device__ __forceinline__ double ricker(double f0, double t0, double t)
+{
+  const double x = 3.14159265358979323846 * f0 * (t - t0);
+  const double a = x * x;
+  return (1 - 2 * a) * exp(-a);
+}
+
+/* time-dependent source/forcing scalars of the stage that starts next.  Kept
+ * out of line: only the clock-advancing thread runs it, and inlining the FP64
+ * exp/sincos would raise the register budget of the whole stage kernel. */
+__device__ __noinline__ void clock_scalars(const SubDev &sd, Clock *c)
+{
+  const long long st = c->stage;
+  const double ts = c->t + sd.C[(int)(st % sd.nstages)] * c->dt;
+  if (sd.src_elem >= 0)
+    c->g = ricker(sd.src_f0, sd.src_t0, ts);
+  if (sd.mms)
+  {
+    double sv, cv;
+    sincos(sd.mms_omega * ts, &sv, &cv);
+    c->sw = sv;
+    c->cw = cv;
+  }
+}
+
+/* called by one thread after a task's results are stored: the last task of
+ * a subdomain in this launch advances its clock */
+__device__ __forceinline__ void task_done(const SubDev &sd, int ntasks_sub)
+{
+  __threadfence();
+  const unsigned prev = atomicAdd(&sd.clock->arrive, 1u);
+  if (prev == (unsigned)ntasks_sub - 1u)
+  {
+    __threadfence();
+    Clock *c = sd.clock;
+    const long long st = c->stage + 1;
+    c->arrive = 0u;
+    c->stage = st;
+    if (st % sd.nstages == 0)
+      c->t = c->t + c->dt;
+    clock_scalars(sd, c);
+    __threadfence();
+  }
+}
+
+/* stage parameters of the subdomain read at the start of a task */
+struct StageCtx
+{
+  int parity;
+  double a, b, dt, t;
+  double g, sw, cw;
+};
+
+__device__ __forceinline__ StageCtx stage_ctx(const SubDev &sd,
+                                              const LaunchArgs &a)
+{
+  StageCtx c;
+  const int mode = a.mode;
+  const double t_rhs = a.t_rhs;
+  if (mode == MODE_RHS)
+  {
+    c.parity = (int)(sd.clock->stage & 1);
+    c.a = 0;
+    c.b = 0;
+    c.dt = 0;
+    c.t = t_rhs;
+    c.g = a.g_rhs;
+    c.sw = a.sw_rhs;
+    c.cw = a.cw_rhs;
+    return c;
+  }
+  c.g = sd.clock->g;
+  c.sw = sd.clock->sw;
+  c.cw = sd.clock->cw;
+  const long long st = sd.clock->stage;
+  const int s = (int)(st % sd.nstages);
+  c.parity = (int)(st & 1);
+  c.a = sd.A[s];
+  c.b = sd.B[s];
+  c.dt = sd.clock->dt;
+  c.t = sd.clock->t + sd.C[s] * c.dt;
+  return c;
+}
+
+
+/* face permutation (swap / reverse bits), same as bfamx_perm_apply */
+__device__ __forceinline__ int perm_apply(int code, int n1, int a, int b)
+{
+  int x = (code & 4) ? b : a;
+  int y = (code & 4) ? a : b;
+  if (code & 1)
+    x = n1 - 1 - x;
+  if (code & 2)
+    y = n1 - 1 - y;
+  return x + y * n1;
+}
+
+/* volume node of face f, in-face indices (a, b) (bfam.c:6599-6781 order) */
+template <int DIM>
+__device__ __forceinline__ int face_node(int f, int n1, int a, int b)
+{
+  const int fixed = (f & 1) ? n1 - 1 : 0;
+  if (DIM == 2)
+    return (f < 2) ? fixed + n1 * a : a + n1 * fixed;
+  switch (f >> 1)
+  {
+  case 0: return fixed + n1 * (a + n1 * b);
+  case 1: return a + n1 * (fixed + n1 * b);
+  default: return a + n1 * (b + n1 * fixed);
+  }
+}
+
+} // namespace bfamx

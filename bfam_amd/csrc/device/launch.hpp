@@ -1,0 +1,60 @@
+/* Host-side entry points of the kernel translation units (C++ linkage, used
+ * only by runtime.hip). */
+#pragma once
+
+#include "common.hpp"
+
+namespace bfamx
+{
+
+/* elements per workgroup for order N (threads = epb * Np, at most 512) */
+inline int elastic_epb(int N)
+{
+  const int np = (N + 1) * (N + 1) * (N + 1);
+  const int e = 512 / np;
+  return e < 1 ? 1 : e;
+}
+inline int acoustic_epb(int N)
+{
+  const int np = (N + 1) * (N + 1);
+  const int e = 256 / np;
+  return e < 1 ? 1 : e;
+}
+
+/* precision codes as in bfamx_precision_t */
+enum
+{
+  PREC_FP64 = 0,
+  PREC_FP64_PLAIN = 1,
+  PREC_FP32_RHS = 2,
+  PREC_FP32 = 3
+};
+
+/* fused stage or RHS-only launch over a task table.  N < 0: mixed orders
+ * (one kernel handles every N in 3..7).  grid: number of workgroups. */
+hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
+                                hipStream_t s);
+hipError_t launch_acoustic_stage(int prec, int N, const LaunchArgs &a,
+                                 int grid, hipStream_t s);
+
+/* unfused FP64 path: volume kernel (rhs = volume + sources), surface kernel
+ * (rhs += lifted fluctuations), then the element-wise LSRK update */
+hipError_t launch_elastic_plain(int N, const LaunchArgs &a, int grid,
+                                hipStream_t s);
+hipError_t launch_acoustic_plain(int N, const LaunchArgs &a, int grid,
+                                 hipStream_t s);
+hipError_t launch_plain_update(const SubDev *sd_dev, size_t ndof,
+                               hipStream_t s, int nblocks);
+
+/* reductions: per-element partial sums (FP64) of the energy density (K), or
+ * interleaved pairs (2K) of |qa - qb|^2, |qb|^2 and of the error against the
+ * manufactured solution with time factors cos/sin(omega t) */
+hipError_t launch_energy(const SubDev &sd, int state_is_fp32, int parity,
+                         double *partial, hipStream_t s);
+hipError_t launch_l2diff(const SubDev &a, int a_fp32, int pa, const SubDev &b,
+                         int b_fp32, int pb, double *partial, hipStream_t s);
+hipError_t launch_mms_error(const SubDev &sd, int state_is_fp32, int parity,
+                            const double *A, double cwt, double swt,
+                            double *partial, hipStream_t s);
+
+} // namespace bfamx

@@ -1,0 +1,150 @@
+/* Physics helpers shared by the fused stage, unfused (plain) and reduction
+ * kernels: precision traits, per-element table access, the elastic and
+ * acoustic upwind fluctuations and lifts, manufactured-solution patterns. */
+#pragma once
+
+#include "launch.hpp"
+
+namespace bfamx
+{
+
+template <class S, class R, class Cc> struct Prec
+{
+  using TS = S; /* state */
+  using TR = R; /* rate register */
+  using TC = Cc; /* arithmetic */
+};
+using P64 = Prec<double, double, double>;
+using P32R = Prec<double, float, float>;
+using P32 = Prec<float, float, float>;
+
+template <class T> __device__ __forceinline__ const T *geo_tab(const SubDev &s);
+template <> __device__ __forceinline__ const double *geo_tab(const SubDev &s)
+{
+  return s.geo;
+}
+template <> __device__ __forceinline__ const float *geo_tab(const SubDev &s)
+{
+  return s.geo_f;
+}
+template <class T> __device__ __forceinline__ const T *mat_tab(const SubDev &s);
+template <> __device__ __forceinline__ const double *mat_tab(const SubDev &s)
+{
+  return s.mat;
+}
+template <> __device__ __forceinline__ const float *mat_tab(const SubDev &s)
+{
+  return s.mat_f;
+}
+template <class T> __device__ __forceinline__ const T *dr_tab(const SubDev &s);
+template <> __device__ __forceinline__ const double *dr_tab(const SubDev &s)
+{
+  return s.Dr;
+}
+template <> __device__ __forceinline__ const float *dr_tab(const SubDev &s)
+{
+  return s.Dr_f;
+}
+template <class T> __device__ __forceinline__ const T *src_tab(const SubDev &s);
+template <> __device__ __forceinline__ const double *src_tab(const SubDev &s)
+{
+  return s.src_wt;
+}
+template <> __device__ __forceinline__ const float *src_tab(const SubDev &s)
+{
+  return s.src_wt_f;
+}
+
+/* which part of the stage a task body runs */
+enum
+{
+  PART_FUSED = 0,   /* volume + faces + update (or RHS-only output) */
+  PART_VOLUME = 1,  /* rhs = volume terms + sources (unfused path) */
+  PART_SURFACE = 2  /* rhs += lifted face fluctuations (unfused path) */
+};
+
+template <int N, int EPB, class T> struct ElasticSmem
+{
+  static constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
+  static constexpr int QW = EPB * 9 * Np;  /* staged fields */
+  static constexpr int FW = EPB * 36 * Nfp; /* face fluctuations (aliased) */
+  static constexpr int W = QW > FW ? QW : FW;
+  /* the double-precision tables start 16-byte aligned after fields + D */
+  static constexpr size_t moff = (sizeof(T) * (W + n1 * n1) + 15) / 16 * 16;
+  static constexpr size_t bytes = moff + sizeof(double) * (EPB * 6 * n1 + 8);
+};
+
+/*
+ * Upwind fluctuations on a face with normal s*e_axis: from the minus and plus
+ * velocities and stresses (stress order s11 s22 s33 s12 s13 s23) returns
+ * dv = v* - v-  and  dT = T* - T-  with T = sigma n.
+ */
+template <class T>
+__device__ __forceinline__ void elastic_riemann(int axis, T s, const T *vm,
+                                                const T *sm, T Zpm, T Zsm,
+                                                const T *vp, const T *sp,
+                                                T Zpp, T Zsp, T *dv, T *dT)
+{
+  /* column `axis` of the symmetric stress: sigma[c][axis] */
+  const int col0 = axis == 0 ? 0 : (axis == 1 ? 3 : 4);
+  const int col1 = axis == 0 ? 3 : (axis == 1 ? 1 : 5);
+  const int col2 = axis == 0 ? 4 : (axis == 1 ? 5 : 2);
+  const T Tm[3] = {s * sm[col0], s * sm[col1], s * sm[col2]};
+  const T Tp[3] = {s * sp[col0], s * sp[col1], s * sp[col2]};
+  const T vnm = s * vm[axis], vnp = s * vp[axis];
+  const T Tnm = s * Tm[axis], Tnp = s * Tp[axis];
+  const T dvn = (Zpp * (vnp - vnm) + (Tnp - Tnm)) / (Zpm + Zpp);
+  const T zs = Zsm + Zsp;
+  const T izs = zs > T(0) ? T(1) / zs : T(0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    const T nc = (c == axis) ? s : T(0);
+    const T vtm = vm[c] - vnm * nc, vtp = vp[c] - vnp * nc;
+    const T Ttm = Tm[c] - Tnm * nc, Ttp = Tp[c] - Tnp * nc;
+    const T dvt = (Zsp * (vtp - vtm) + (Ttp - Ttm)) * izs;
+    dv[c] = dvn * nc + dvt;
+    dT[c] = Zpm * dvn * nc + Zsm * dvt;
+  }
+}
+
+/* rate += lift * (dT/rho, lambda dvn I + mu (dv n^T + n dv^T)) */
+template <class T>
+__device__ __forceinline__ void elastic_lift(int axis, T s, T lift, T rhoi,
+                                             T lam, T mu, const T *fl, T *rate)
+{
+  const T dv0 = fl[0], dv1 = fl[1], dv2 = fl[2];
+  const T dvn = s * (axis == 0 ? dv0 : (axis == 1 ? dv1 : dv2));
+  rate[0] += lift * fl[3] * rhoi;
+  rate[1] += lift * fl[4] * rhoi;
+  rate[2] += lift * fl[5] * rhoi;
+  const T ld = lam * dvn;
+  const T m2 = T(2) * mu * s;
+  rate[3] += lift * (ld + (axis == 0 ? m2 * dv0 : T(0)));
+  rate[4] += lift * (ld + (axis == 1 ? m2 * dv1 : T(0)));
+  rate[5] += lift * (ld + (axis == 2 ? m2 * dv2 : T(0)));
+  const T ms = mu * s;
+  rate[6] += lift * (axis == 0 ? ms * dv1 : (axis == 1 ? ms * dv0 : T(0)));
+  rate[7] += lift * (axis == 0 ? ms * dv2 : (axis == 2 ? ms * dv0 : T(0)));
+  rate[8] += lift * (axis == 1 ? ms * dv2 : (axis == 2 ? ms * dv1 : T(0)));
+}
+
+__device__ __forceinline__ double mms_pattern(int f, const double *Sv,
+                                              const double *Cv)
+{
+  switch (f)
+  {
+  case 0: return Sv[0] * Cv[1] * Cv[2];
+  case 1: return Cv[0] * Sv[1] * Cv[2];
+  case 2: return Cv[0] * Cv[1] * Sv[2];
+  case 3:
+  case 4:
+  case 5: return Cv[0] * Cv[1] * Cv[2];
+  case 6: return Sv[0] * Sv[1] * Cv[2];
+  case 7: return Sv[0] * Cv[1] * Sv[2];
+  default: return Cv[0] * Sv[1] * Sv[2];
+  }
+}
+
+
+} // namespace bfamx

@@ -1,0 +1,268 @@
+/*
+ * Element-wise LSRK update of the unfused FP64 path, and the FP64 reductions
+ * (energy, L2 difference, error against the manufactured solution).
+ *
+ * Reductions produce one FP64 partial sum per element (LGL-quadrature weighted
+ * J w_i w_j (w_k) sums over the element's nodes); the host adds the K
+ * partials with Neumaier's compensated summation, so the result does not
+ * depend on launch geometry.
+ */
+#include "physics.hpp"
+
+namespace bfamx
+{
+
+/* dq = A dq + dt rhs;  q = q + B dq  over all DOF, then advance the clock */
+__global__ __launch_bounds__(256) void plain_update_kernel(const SubDev *sdp,
+                                                          size_t ndof)
+{
+  const SubDev &sd = *sdp;
+  const long long st = sd.clock->stage;
+  const int s = (int)(st % sd.nstages);
+  const double a = sd.A[s], b = sd.B[s], dt = sd.clock->dt;
+  double *q = static_cast<double *>(sd.q[0]);
+  double *dq = static_cast<double *>(sd.dq);
+  const double *rhs = sd.rhs;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ndof;
+       i += (size_t)gridDim.x * blockDim.x)
+  {
+    const double d = a * dq[i] + dt * rhs[i];
+    dq[i] = d;
+    q[i] = q[i] + b * d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    task_done(sd, gridDim.x);
+}
+
+hipError_t launch_plain_update(const SubDev *sd_dev, size_t ndof,
+                               hipStream_t s, int nblocks)
+{
+  hipLaunchKernelGGL(plain_update_kernel, dim3(nblocks), dim3(256), 0, s,
+                     sd_dev, ndof);
+  return hipGetLastError();
+}
+
+/* ---------------------------------------------------------------------- */
+
+template <class T>
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1)
+  {
+    if ((int)threadIdx.x < w)
+      red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+struct ElemQuad
+{
+  int dim, n1, Np, nf;
+};
+
+__device__ __forceinline__ double quad_weight(const SubDev &sd, int dim,
+                                              int n1, int e, int n)
+{
+  const double *G = sd.geo + (size_t)e * GEO_STRIDE;
+  const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
+  double wq = sd.w[i] * sd.w[j] * 0.5 * G[GEO_HX] * 0.5 * G[GEO_HY];
+  if (dim == 3)
+    wq *= sd.w[k] * 0.5 * G[GEO_HZ];
+  return wq;
+}
+
+template <class T>
+__device__ __forceinline__ double ld(const void *p, size_t i)
+{
+  return (double)static_cast<const T *>(p)[i];
+}
+
+template <class TS>
+__global__ __launch_bounds__(256) void energy_kernel(SubDev sd, int parity,
+                                                     double *partial)
+{
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  const int dim = sd.physics == 0 ? 2 : 3;
+  const int n1 = sd.N + 1;
+  const int Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  const int NF = sd.nf;
+  const double *M = sd.mat + (size_t)e * MAT_STRIDE;
+  const double rho = 1.0 / M[MAT_RHOI], lam = M[MAT_LAM], mu = M[MAT_MU];
+  const void *q = sd.q[parity];
+  double acc = 0;
+  for (int n = threadIdx.x; n < Np; n += blockDim.x)
+  {
+    const size_t b = (size_t)e * NF * Np + n;
+    double dens;
+    if (dim == 2)
+    {
+      const double p = ld<TS>(q, b), u = ld<TS>(q, b + Np),
+                   v = ld<TS>(q, b + 2 * Np);
+      dens = rho * (u * u + v * v) + p * p / lam;
+    }
+    else
+    {
+      double f[9];
+      for (int c = 0; c < 9; ++c)
+        f[c] = ld<TS>(q, b + (size_t)c * Np);
+      const double tr = f[3] + f[4] + f[5];
+      const double ss = f[3] * f[3] + f[4] * f[4] + f[5] * f[5] +
+                        2 * (f[6] * f[6] + f[7] * f[7] + f[8] * f[8]);
+      dens = rho * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) +
+             (ss - lam / (3 * lam + 2 * mu) * tr * tr) / (2 * mu);
+    }
+    acc += 0.5 * quad_weight(sd, dim, n1, e, n) * dens;
+  }
+  const double r = block_sum<double>(acc, red);
+  if (threadIdx.x == 0)
+    partial[e] = r;
+}
+
+hipError_t launch_energy(const SubDev &sd, int state_is_fp32, int parity,
+                         double *partial, hipStream_t s)
+{
+  if (state_is_fp32)
+    hipLaunchKernelGGL(energy_kernel<float>, dim3(sd.K), dim3(256), 0, s, sd,
+                       parity, partial);
+  else
+    hipLaunchKernelGGL(energy_kernel<double>, dim3(sd.K), dim3(256), 0, s, sd,
+                       parity, partial);
+  return hipGetLastError();
+}
+
+template <class TA, class TB>
+__global__ __launch_bounds__(256) void l2diff_kernel(SubDev a, int pa,
+                                                     SubDev b, int pb,
+                                                     double *partial)
+{
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  const int dim = a.physics == 0 ? 2 : 3;
+  const int n1 = a.N + 1;
+  const int Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  const int NF = a.nf;
+  double d2 = 0, n2 = 0;
+  for (int n = threadIdx.x; n < Np; n += blockDim.x)
+  {
+    const double wq = quad_weight(a, dim, n1, e, n);
+    double sd2 = 0, sn2 = 0;
+    for (int f = 0; f < NF; ++f)
+    {
+      const size_t id = ((size_t)e * NF + f) * Np + n;
+      const double x = ld<TA>(a.q[pa], id), y = ld<TB>(b.q[pb], id);
+      sd2 += (x - y) * (x - y);
+      sn2 += y * y;
+    }
+    d2 += wq * sd2;
+    n2 += wq * sn2;
+  }
+  const double r1 = block_sum<double>(d2, red);
+  const double r2 = block_sum<double>(n2, red);
+  if (threadIdx.x == 0)
+  {
+    partial[2 * (size_t)e] = r1;
+    partial[2 * (size_t)e + 1] = r2;
+  }
+}
+
+hipError_t launch_l2diff(const SubDev &a, int a_fp32, int pa, const SubDev &b,
+                         int b_fp32, int pb, double *partial, hipStream_t s)
+{
+  if (a_fp32 && b_fp32)
+    hipLaunchKernelGGL((l2diff_kernel<float, float>), dim3(a.K), dim3(256), 0,
+                       s, a, pa, b, pb, partial);
+  else if (a_fp32)
+    hipLaunchKernelGGL((l2diff_kernel<float, double>), dim3(a.K), dim3(256),
+                       0, s, a, pa, b, pb, partial);
+  else if (b_fp32)
+    hipLaunchKernelGGL((l2diff_kernel<double, float>), dim3(a.K), dim3(256),
+                       0, s, a, pa, b, pb, partial);
+  else
+    hipLaunchKernelGGL((l2diff_kernel<double, double>), dim3(a.K), dim3(256),
+                       0, s, a, pa, b, pb, partial);
+  return hipGetLastError();
+}
+
+struct MmsAmp
+{
+  double A[9];
+};
+
+__device__ __forceinline__ double mms_pat(int f, const double *Sv,
+                                          const double *Cv)
+{
+  switch (f)
+  {
+  case 0: return Sv[0] * Cv[1] * Cv[2];
+  case 1: return Cv[0] * Sv[1] * Cv[2];
+  case 2: return Cv[0] * Cv[1] * Sv[2];
+  case 3:
+  case 4:
+  case 5: return Cv[0] * Cv[1] * Cv[2];
+  case 6: return Sv[0] * Sv[1] * Cv[2];
+  case 7: return Sv[0] * Cv[1] * Sv[2];
+  default: return Cv[0] * Sv[1] * Sv[2];
+  }
+}
+
+template <class TS>
+__global__ __launch_bounds__(256) void mms_error_kernel(SubDev sd, int parity,
+                                                        MmsAmp amp, double cwt,
+                                                        double swt,
+                                                        double *partial)
+{
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  const int n1 = sd.N + 1, Np = n1 * n1 * n1;
+  const double *T = sd.mms_tab + (size_t)e * 6 * n1;
+  double d2 = 0, n2 = 0;
+  for (int n = threadIdx.x; n < Np; n += blockDim.x)
+  {
+    const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
+    const double Sv[3] = {T[0 * n1 + i], T[2 * n1 + j], T[4 * n1 + k]};
+    const double Cv[3] = {T[1 * n1 + i], T[3 * n1 + j], T[5 * n1 + k]};
+    const double wq = quad_weight(sd, 3, n1, e, n);
+    double sd2 = 0, sn2 = 0;
+    for (int f = 0; f < 9; ++f)
+    {
+      const double ex = amp.A[f] * mms_pat(f, Sv, Cv) * (f < 3 ? cwt : swt);
+      const double x = ld<TS>(sd.q[parity], ((size_t)e * 9 + f) * Np + n);
+      sd2 += (x - ex) * (x - ex);
+      sn2 += ex * ex;
+    }
+    d2 += wq * sd2;
+    n2 += wq * sn2;
+  }
+  const double r1 = block_sum<double>(d2, red);
+  const double r2 = block_sum<double>(n2, red);
+  if (threadIdx.x == 0)
+  {
+    partial[2 * (size_t)e] = r1;
+    partial[2 * (size_t)e + 1] = r2;
+  }
+}
+
+hipError_t launch_mms_error(const SubDev &sd, int state_is_fp32, int parity,
+                            const double *A, double cwt, double swt,
+                            double *partial, hipStream_t s)
+{
+  MmsAmp amp;
+  for (int f = 0; f < 9; ++f)
+    amp.A[f] = A[f];
+  if (state_is_fp32)
+    hipLaunchKernelGGL(mms_error_kernel<float>, dim3(sd.K), dim3(256), 0, s, sd,
+                       parity, amp, cwt, swt, partial);
+  else
+    hipLaunchKernelGGL(mms_error_kernel<double>, dim3(sd.K), dim3(256), 0, s,
+                       sd, parity, amp, cwt, swt, partial);
+  return hipGetLastError();
+}
+
+} // namespace bfamx

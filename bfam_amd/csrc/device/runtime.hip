@@ -1,0 +1,1107 @@
+/*
+ * bfamx device runtime: the C ABI of include/bfamx.h on top of HIP.
+ *
+ * Replaces bfam's OCCA device layer (bfamo_*, bfam.c:8389-9092): device
+ * creation with an HBM budget (bfamo_device_malloc's mem_fudge check,
+ * bfam.c:8561-8589, returning BFAMX_EBUDGET instead of aborting), upload of
+ * the per-N operators and the integer maps (bfamo_add_operator_*,
+ * bfam.c:8712-8828), and the per-stage launches that the reference's
+ * applications issued through BFAMO_KERNEL_RUN (bfam.h:1673-1719).  On one
+ * GPU the glue trace exchange (bfamo_communicator_post_* and _wait,
+ * bfam.c:8900-8929) is a gather inside the stage kernel.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../bfamx_internal.h"
+#include "launch.hpp"
+
+using namespace bfamx;
+
+struct bfamx_device
+{
+  int hip_dev;
+  hipStream_t stream;
+  size_t budget;
+  size_t used;
+  int num_cu;
+  int num_xcd;
+  int xcd_remap;
+  hipEvent_t ev0, ev1;
+};
+
+struct bfamx_subdomain
+{
+  bfamx_device_t *dev;
+  int physics, prec, dim, N, Np, nf;
+  int32_t K;
+  int state_fp32, rate_fp32;
+  SubDev h;        /* host image of the descriptor */
+  SubDev *d_sub;   /* device copy */
+  Task *d_tasks;
+  int *d_ntasks;
+  int ntasks;
+  Clock *d_clock;
+  double *d_partial; /* 2K reduction partials */
+  double *d_rhs;     /* lazily allocated for bfamx_rhs on fused precisions */
+  std::vector<void *> allocs;
+  size_t bytes;
+  int fields_set;
+  bfamx_lsrk_t lsrk;
+};
+
+struct bfamx_batch
+{
+  bfamx_device_t *dev;
+  std::vector<bfamx_subdomain_t *> subs;
+  SubDev *d_subs;
+  Task *d_tasks;
+  int *d_ntasks;
+  int ntasks;
+  int physics, prec, N; /* N = -1: mixed orders */
+  size_t bytes;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int graph_steps;
+};
+
+#define HIPCHK(call)                                                           \
+  do                                                                           \
+  {                                                                            \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return bfamx_fail(BFAMX_EHIP, "%s failed: %s", #call,                   \
+                        hipGetErrorString(e_));                                \
+  } while (0)
+
+/* ---------------------------------------------------------------------- */
+/* device                                                                  */
+/* ---------------------------------------------------------------------- */
+
+extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
+                                   bfamx_device_t **out)
+{
+  if (!out)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_device_create: NULL out");
+  *out = NULL;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (hip_device < 0 || hip_device >= ndev)
+    return bfamx_fail(BFAMX_EINVAL, "no HIP device %d (%d present)",
+                      hip_device, ndev);
+  HIPCHK(hipSetDevice(hip_device));
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  bfamx_device_t *d = new bfamx_device_t();
+  d->hip_dev = hip_device;
+  d->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.9 * total_b);
+  d->used = 0;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, hip_device));
+  d->num_cu = prop.multiProcessorCount;
+  int nx = 1;
+  if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, hip_device) !=
+          hipSuccess ||
+      nx < 1)
+    nx = 1;
+  d->num_xcd = nx;
+  d->xcd_remap = 1;
+  /* default priority only: streams of different priorities have hung on
+   * this hardware */
+  HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&d->ev0));
+  HIPCHK(hipEventCreate(&d->ev1));
+  *out = d;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_device_destroy(bfamx_device_t *d)
+{
+  if (!d)
+    return BFAMX_OK;
+  (void)hipSetDevice(d->hip_dev);
+  (void)hipStreamSynchronize(d->stream);
+  (void)hipEventDestroy(d->ev0);
+  (void)hipEventDestroy(d->ev1);
+  (void)hipStreamDestroy(d->stream);
+  delete d;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_device_memory(const bfamx_device_t *d, size_t *used,
+                                   size_t *budget)
+{
+  if (!d)
+    return bfamx_fail(BFAMX_EINVAL, "NULL device");
+  if (used)
+    *used = d->used;
+  if (budget)
+    *budget = d->budget;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_device_synchronize(bfamx_device_t *d)
+{
+  if (!d)
+    return bfamx_fail(BFAMX_EINVAL, "NULL device");
+  HIPCHK(hipStreamSynchronize(d->stream));
+  return BFAMX_OK;
+}
+
+extern "C" void *bfamx_device_stream(bfamx_device_t *d)
+{
+  return d ? (void *)d->stream : NULL;
+}
+
+extern "C" int bfamx_device_info(const bfamx_device_t *d, int *num_cu,
+                                 int *num_xcd)
+{
+  if (!d)
+    return bfamx_fail(BFAMX_EINVAL, "NULL device");
+  if (num_cu)
+    *num_cu = d->num_cu;
+  if (num_xcd)
+    *num_xcd = d->num_xcd;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_set_tuning(bfamx_device_t *d, int xcd_remap)
+{
+  if (!d)
+    return bfamx_fail(BFAMX_EINVAL, "NULL device");
+  d->xcd_remap = xcd_remap ? 1 : 0;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_timer_start(bfamx_device_t *d)
+{
+  if (!d)
+    return bfamx_fail(BFAMX_EINVAL, "NULL device");
+  HIPCHK(hipEventRecord(d->ev0, d->stream));
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_timer_stop(bfamx_device_t *d, float *ms)
+{
+  if (!d || !ms)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_timer_stop: bad arguments");
+  HIPCHK(hipEventRecord(d->ev1, d->stream));
+  HIPCHK(hipEventSynchronize(d->ev1));
+  HIPCHK(hipEventElapsedTime(ms, d->ev0, d->ev1));
+  return BFAMX_OK;
+}
+
+/* budgeted allocation (bfamo_device_malloc, bfam.c:8561-8589) */
+static int dev_alloc(bfamx_device_t *d, std::vector<void *> &list,
+                     size_t *acc, size_t bytes, void **p)
+{
+  *p = NULL;
+  if (bytes == 0)
+    bytes = 16;
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (d->used + bytes > d->budget)
+    return bfamx_fail(BFAMX_EBUDGET,
+                      "HBM budget exceeded: %zu used + %zu requested > %zu",
+                      d->used, bytes, d->budget);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "hipMalloc(%zu) failed: %s", bytes,
+                      hipGetErrorString(e));
+  d->used += bytes;
+  *acc += bytes;
+  list.push_back(*p);
+  return BFAMX_OK;
+}
+
+static void dev_free_all(bfamx_device_t *d, std::vector<void *> &list,
+                         size_t bytes)
+{
+  for (void *p : list)
+    (void)hipFree(p);
+  list.clear();
+  d->used -= bytes;
+}
+
+/* ---------------------------------------------------------------------- */
+/* subdomain                                                              */
+/* ---------------------------------------------------------------------- */
+
+static int nfields_of(int physics) { return physics == BFAMX_ACOUSTIC2D ? 3 : 9; }
+
+static int epb_of(int physics, int N)
+{
+  return physics == BFAMX_ACOUSTIC2D ? acoustic_epb(N) : elastic_epb(N);
+}
+
+static int order_supported(int physics, int N)
+{
+  return physics == BFAMX_ACOUSTIC2D ? (N >= 1 && N <= 8) : (N >= 3 && N <= 7);
+}
+
+extern "C" size_t bfamx_subdomain_bytes(int dim, int N, int32_t K,
+                                        bfamx_physics_t ph,
+                                        bfamx_precision_t prec)
+{
+  const size_t n1 = N + 1;
+  const size_t Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  const size_t nf = nfields_of(ph);
+  const size_t ndof = (size_t)K * nf * Np;
+  const size_t rnd = 255;
+  size_t b = 0;
+  auto add = [&](size_t x) { b += ((x ? x : 16) + rnd) & ~rnd; };
+  switch (prec)
+  {
+  case BFAMX_FP64: add(8 * ndof), add(8 * ndof), add(8 * ndof); break;
+  case BFAMX_FP64_PLAIN: add(8 * ndof), add(8 * ndof), add(8 * ndof); break;
+  case BFAMX_FP32_RHS: add(8 * ndof), add(8 * ndof), add(4 * ndof); break;
+  default: add(4 * ndof), add(4 * ndof), add(4 * ndof); break;
+  }
+  const size_t nfaces = 2 * dim;
+  add(4 * K * nfaces), add(K * nfaces);
+  add(8 * K * GEO_STRIDE), add(4 * K * GEO_STRIDE);
+  add(8 * K * MAT_STRIDE), add(4 * K * MAT_STRIDE);
+  add(8 * n1 * n1), add(4 * n1 * n1), add(8 * n1), add(8 * n1);
+  add(sizeof(Clock)), add(sizeof(SubDev));
+  const int epb = dim == 2 ? acoustic_epb(N) : elastic_epb(N);
+  add(sizeof(Task) * ((K + epb - 1) / epb)), add(sizeof(int));
+  add(16 * (size_t)K);
+  return b;
+}
+
+static int push_sub(bfamx_subdomain_t *s)
+{
+  HIPCHK(hipMemcpy(s->d_sub, &s->h, sizeof(SubDev), hipMemcpyHostToDevice));
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
+                                      const bfamx_dgx_t *sub,
+                                      const bfamx_physics_desc_t *phys,
+                                      const bfamx_lsrk_t *lsrk,
+                                      bfamx_subdomain_t **out)
+{
+  if (!dev || !sub || !phys || !out)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_subdomain_upload: NULL argument");
+  *out = NULL;
+  const int dim = sub->dim, N = sub->N;
+  const int physics = phys->physics, prec = phys->precision;
+  if (physics != BFAMX_ACOUSTIC2D && physics != BFAMX_ELASTIC3D)
+    return bfamx_fail(BFAMX_EINVAL, "unknown physics %d", physics);
+  if (dim != (physics == BFAMX_ACOUSTIC2D ? 2 : 3))
+    return bfamx_fail(BFAMX_EINVAL, "dim %d does not match physics", dim);
+  if (!order_supported(physics, N))
+    return bfamx_fail(BFAMX_EUNSUPPORTED, "order N=%d not built for physics %d",
+                      N, physics);
+  if (prec < BFAMX_FP64 || prec > BFAMX_FP32)
+    return bfamx_fail(BFAMX_EINVAL, "unknown precision %d", prec);
+  const int n1 = N + 1;
+  const int Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  if (sub->Np != Np || sub->K <= 0 || !sub->r || !sub->w || !sub->Dr ||
+      !sub->vmapM || !sub->vmapP || !phys->elem_h || !phys->rho || !phys->cp ||
+      (physics == BFAMX_ELASTIC3D && !phys->cs))
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_subdomain_upload: incomplete input");
+  const int32_t K = sub->K;
+  const int nf = nfields_of(physics), nfaces = 2 * dim;
+  if ((double)K * nf * Np > 4.0e9)
+    return bfamx_fail(BFAMX_EINVAL, "subdomain too large for 32-bit tasks");
+
+  /* face connectivity from vmapP */
+  std::vector<int32_t> nbr_e((size_t)K * nfaces);
+  std::vector<uint8_t> nbr_c((size_t)K * nfaces);
+  int rc = bfamx_face_connectivity(dim, N, K, sub->vmapM, sub->vmapP,
+                                   nbr_e.data(), nbr_c.data());
+  if (rc)
+    return rc;
+
+  /* geometry and material tables */
+  std::vector<double> geo((size_t)K * GEO_STRIDE, 0.0), mat((size_t)K * MAT_STRIDE, 0.0);
+  const double wend = sub->w[0];
+  for (int32_t e = 0; e < K; ++e)
+  {
+    double *G = &geo[(size_t)e * GEO_STRIDE];
+    for (int d = 0; d < dim; ++d)
+    {
+      const double h = phys->elem_h[(size_t)e * dim + d];
+      if (!(h > 0))
+        return bfamx_fail(BFAMX_EINVAL, "element %d has edge length %g", e, h);
+      G[GEO_GX + d] = 2.0 / h;
+      G[GEO_LX + d] = (2.0 / h) / wend;
+      G[GEO_HX + d] = h;
+      G[GEO_X0 + d] = phys->elem_x0 ? phys->elem_x0[(size_t)e * dim + d] : 0.0;
+    }
+    double *M = &mat[(size_t)e * MAT_STRIDE];
+    const double rho = phys->rho[e], cp = phys->cp[e];
+    const double cs = physics == BFAMX_ELASTIC3D ? phys->cs[e] : 0.0;
+    if (!(rho > 0) || !(cp > 0) || cs < 0 || (physics == BFAMX_ELASTIC3D && !(cs > 0)))
+      return bfamx_fail(BFAMX_EINVAL, "element %d has invalid material", e);
+    M[MAT_RHOI] = 1.0 / rho;
+    M[MAT_LAM] = rho * (cp * cp - 2 * cs * cs);
+    M[MAT_MU] = rho * cs * cs;
+    M[MAT_ZP] = rho * cp;
+    M[MAT_ZS] = rho * cs;
+  }
+  std::vector<float> geof(geo.begin(), geo.end()), matf(mat.begin(), mat.end());
+  std::vector<float> Drf(sub->Dr, sub->Dr + n1 * n1);
+
+  bfamx_subdomain_t *s = new bfamx_subdomain_t();
+  s->dev = dev;
+  s->physics = physics;
+  s->prec = prec;
+  s->dim = dim;
+  s->N = N;
+  s->Np = Np;
+  s->nf = nf;
+  s->K = K;
+  s->state_fp32 = prec == BFAMX_FP32;
+  s->rate_fp32 = prec == BFAMX_FP32 || prec == BFAMX_FP32_RHS;
+  s->bytes = 0;
+  s->d_rhs = NULL;
+  s->fields_set = 0;
+  if (lsrk)
+    s->lsrk = *lsrk;
+  else
+    bfamx_lsrk_init(&s->lsrk, BFAMX_LSRK_KC54);
+  if (s->lsrk.n_stages < 1 || s->lsrk.n_stages > 8)
+  {
+    delete s;
+    return bfamx_fail(BFAMX_EINVAL, "bad LSRK table");
+  }
+
+  const size_t ndof = (size_t)K * nf * Np;
+  const size_t ssz = s->state_fp32 ? 4 : 8, rsz = s->rate_fp32 ? 4 : 8;
+  SubDev &h = s->h;
+  memset(&h, 0, sizeof(h));
+  h.physics = physics;
+  h.N = N;
+  h.K = K;
+  h.nf = nf;
+  h.nstages = s->lsrk.n_stages;
+  h.src_elem = -1;
+  h.mms = 0;
+  for (int i = 0; i < 8; ++i)
+  {
+    h.A[i] = s->lsrk.A[i];
+    h.B[i] = s->lsrk.B[i];
+  }
+  for (int i = 0; i < 9; ++i)
+    h.C[i] = s->lsrk.C[i];
+
+  const int epb = epb_of(physics, N);
+  s->ntasks = (K + epb - 1) / epb;
+  std::vector<Task> tasks(s->ntasks);
+  for (int t = 0; t < s->ntasks; ++t)
+  {
+    tasks[t].sub = 0;
+    tasks[t].e0 = t * epb;
+    tasks[t].ne = (t * epb + epb <= K) ? epb : K - t * epb;
+    tasks[t].N = N;
+  }
+
+  void *p;
+#define ALLOC(ptr, nb_)                                                        \
+  do                                                                           \
+  {                                                                            \
+    rc = dev_alloc(dev, s->allocs, &s->bytes, (nb_), &p);                      \
+    if (rc)                                                                    \
+      goto fail;                                                               \
+    ptr = (decltype(ptr))p;                                                    \
+  } while (0)
+  ALLOC(h.q[0], ssz * ndof);
+  if (prec == BFAMX_FP64_PLAIN)
+  {
+    h.q[1] = h.q[0];
+    ALLOC(h.rhs, 8 * ndof);
+  }
+  else
+    ALLOC(h.q[1], ssz * ndof);
+  ALLOC(h.dq, rsz * ndof);
+  {
+    int32_t *ne;
+    uint8_t *nc;
+    double *g, *m, *D, *r, *w;
+    float *gf, *mf, *Df;
+    ALLOC(ne, 4 * nbr_e.size());
+    ALLOC(nc, nbr_c.size());
+    ALLOC(g, 8 * geo.size());
+    ALLOC(gf, 4 * geof.size());
+    ALLOC(m, 8 * mat.size());
+    ALLOC(mf, 4 * matf.size());
+    ALLOC(D, 8 * n1 * n1);
+    ALLOC(Df, 4 * n1 * n1);
+    ALLOC(r, 8 * n1);
+    ALLOC(w, 8 * n1);
+    ALLOC(s->d_clock, sizeof(Clock));
+    ALLOC(s->d_sub, sizeof(SubDev));
+    ALLOC(s->d_tasks, sizeof(Task) * tasks.size());
+    ALLOC(s->d_ntasks, sizeof(int));
+    ALLOC(s->d_partial, 16 * (size_t)K);
+    hipError_t he = hipSuccess;
+#define CP(dst, src, bytes)                                                    \
+  if (he == hipSuccess)                                                        \
+  he = hipMemcpy((void *)(dst), (const void *)(src), (bytes), hipMemcpyHostToDevice)
+    CP(ne, nbr_e.data(), 4 * nbr_e.size());
+    CP(nc, nbr_c.data(), nbr_c.size());
+    CP(g, geo.data(), 8 * geo.size());
+    CP(gf, geof.data(), 4 * geof.size());
+    CP(m, mat.data(), 8 * mat.size());
+    CP(mf, matf.data(), 4 * matf.size());
+    CP(D, sub->Dr, 8 * n1 * n1);
+    CP(Df, Drf.data(), 4 * n1 * n1);
+    CP(r, sub->r, 8 * n1);
+    CP(w, sub->w, 8 * n1);
+    CP(s->d_tasks, tasks.data(), sizeof(Task) * tasks.size());
+    CP(s->d_ntasks, &s->ntasks, sizeof(int));
+#undef CP
+    if (he != hipSuccess)
+    {
+      rc = bfamx_fail(BFAMX_EHIP, "upload failed: %s", hipGetErrorString(he));
+      goto fail;
+    }
+    h.nbr_elem = ne;
+    h.nbr_code = nc;
+    h.geo = g;
+    h.geo_f = gf;
+    h.mat = m;
+    h.mat_f = mf;
+    h.Dr = D;
+    h.Dr_f = Df;
+    h.r = r;
+    h.w = w;
+    h.clock = s->d_clock;
+  }
+#undef ALLOC
+  {
+    Clock c;
+    memset(&c, 0, sizeof(c));
+    if (hipMemcpy(s->d_clock, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess)
+    {
+      rc = bfamx_fail(BFAMX_EHIP, "clock init failed");
+      goto fail;
+    }
+  }
+  rc = push_sub(s);
+  if (rc)
+    goto fail;
+  *out = s;
+  return BFAMX_OK;
+fail:
+  dev_free_all(dev, s->allocs, s->bytes);
+  delete s;
+  return rc;
+}
+
+extern "C" int bfamx_subdomain_free(bfamx_subdomain_t *s)
+{
+  if (!s)
+    return BFAMX_OK;
+  (void)hipStreamSynchronize(s->dev->stream);
+  dev_free_all(s->dev, s->allocs, s->bytes);
+  delete s;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_subdomain_info(const bfamx_subdomain_t *s, int *nfields,
+                                    size_t *device_bytes)
+{
+  if (!s)
+    return bfamx_fail(BFAMX_EINVAL, "NULL subdomain");
+  if (nfields)
+    *nfields = s->nf;
+  if (device_bytes)
+    *device_bytes = s->bytes;
+  return BFAMX_OK;
+}
+
+static double host_ricker(double f0, double t0, double t)
+{
+  const double x = 3.14159265358979323846 * f0 * (t - t0);
+  const double a = x * x;
+  return (1 - 2 * a) * exp(-a);
+}
+
+/* recompute the time scalars of the stage about to run (host side) */
+static int refresh_clock(bfamx_subdomain_t *s)
+{
+  Clock c;
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(&c, s->d_clock, sizeof(c), hipMemcpyDeviceToHost));
+  const double ts = c.t + s->h.C[(int)(c.stage % s->h.nstages)] * c.dt;
+  if (s->h.src_elem >= 0)
+    c.g = host_ricker(s->h.src_f0, s->h.src_t0, ts);
+  if (s->h.mms)
+  {
+    c.sw = sin(s->h.mms_omega * ts);
+    c.cw = cos(s->h.mms_omega * ts);
+  }
+  HIPCHK(hipMemcpy(s->d_clock, &c, sizeof(c), hipMemcpyHostToDevice));
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_set_source(bfamx_subdomain_t *s, int32_t elem,
+                                const double *weights, const double *amp,
+                                double f0, double t0)
+{
+  if (!s || !weights || !amp || elem < 0 || elem >= s->K)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_source: bad arguments");
+  int rc;
+  if (!s->h.src_wt)
+  {
+    void *p;
+    rc = dev_alloc(s->dev, s->allocs, &s->bytes, 8 * s->Np, &p);
+    if (rc)
+      return rc;
+    s->h.src_wt = (const double *)p;
+    rc = dev_alloc(s->dev, s->allocs, &s->bytes, 4 * s->Np, &p);
+    if (rc)
+      return rc;
+    s->h.src_wt_f = (const float *)p;
+  }
+  std::vector<float> wf(weights, weights + s->Np);
+  HIPCHK(hipMemcpy((void *)s->h.src_wt, weights, 8 * s->Np, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)s->h.src_wt_f, wf.data(), 4 * s->Np, hipMemcpyHostToDevice));
+  s->h.src_elem = elem;
+  for (int f = 0; f < 9; ++f)
+    s->h.src_amp[f] = f < s->nf ? amp[f] : 0.0;
+  s->h.src_f0 = f0;
+  s->h.src_t0 = t0;
+  rc = push_sub(s);
+  if (rc)
+    return rc;
+  return refresh_clock(s);
+}
+
+extern "C" int bfamx_set_forcing_mms(bfamx_subdomain_t *s, double k,
+                                     double omega, const double *F)
+{
+  if (!s || !F || s->physics != BFAMX_ELASTIC3D)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_forcing_mms: elastic only");
+  const int n1 = s->N + 1;
+  if (!s->h.mms_tab)
+  {
+    /* sin/cos(k x) at the element's node coordinates, per axis (static in
+     * time, so computed once here instead of in every stage) */
+    std::vector<double> geo((size_t)s->K * GEO_STRIDE), r(n1);
+    HIPCHK(hipMemcpy(geo.data(), s->h.geo, 8 * geo.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(r.data(), s->h.r, 8 * n1, hipMemcpyDeviceToHost));
+    std::vector<double> tab((size_t)s->K * 6 * n1);
+    for (int32_t e = 0; e < s->K; ++e)
+      for (int ax = 0; ax < 3; ++ax)
+        for (int i = 0; i < n1; ++i)
+        {
+          const double *G = &geo[(size_t)e * GEO_STRIDE];
+          const double x = G[GEO_X0 + ax] + 0.5 * (r[i] + 1) * G[GEO_HX + ax];
+          tab[((size_t)e * 3 + ax) * 2 * n1 + i] = sin(k * x);
+          tab[((size_t)e * 3 + ax) * 2 * n1 + n1 + i] = cos(k * x);
+        }
+    void *p;
+    int rc = dev_alloc(s->dev, s->allocs, &s->bytes, 8 * tab.size(), &p);
+    if (rc)
+      return rc;
+    HIPCHK(hipMemcpy(p, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice));
+    s->h.mms_tab = (const double *)p;
+  }
+  s->h.mms = 1;
+  s->h.mms_k = k;
+  s->h.mms_omega = omega;
+  for (int f = 0; f < 9; ++f)
+    s->h.mms_F[f] = F[f];
+  int rc = push_sub(s);
+  if (rc)
+    return rc;
+  return refresh_clock(s);
+}
+
+extern "C" int bfamx_set_fields(bfamx_subdomain_t *s,
+                                const double *const *fields, double t0,
+                                double dt)
+{
+  if (!s || !fields)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_fields: NULL argument");
+  for (int f = 0; f < s->nf; ++f)
+    if (!fields[f])
+      return bfamx_fail(BFAMX_EINVAL, "bfamx_set_fields: field %d is NULL", f);
+  const size_t Np = s->Np, nf = s->nf, K = s->K;
+  const size_t ndof = K * nf * Np;
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  if (s->state_fp32)
+  {
+    std::vector<float> buf(ndof);
+    for (size_t e = 0; e < K; ++e)
+      for (size_t f = 0; f < nf; ++f)
+        for (size_t n = 0; n < Np; ++n)
+          buf[(e * nf + f) * Np + n] = (float)fields[f][e * Np + n];
+    HIPCHK(hipMemcpy(s->h.q[0], buf.data(), 4 * ndof, hipMemcpyHostToDevice));
+  }
+  else
+  {
+    std::vector<double> buf(ndof);
+    for (size_t e = 0; e < K; ++e)
+      for (size_t f = 0; f < nf; ++f)
+        for (size_t n = 0; n < Np; ++n)
+          buf[(e * nf + f) * Np + n] = fields[f][e * Np + n];
+    HIPCHK(hipMemcpy(s->h.q[0], buf.data(), 8 * ndof, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemset(s->h.dq, 0, (s->rate_fp32 ? 4 : 8) * ndof));
+  Clock c;
+  memset(&c, 0, sizeof(c));
+  c.t = t0;
+  c.dt = dt;
+  HIPCHK(hipMemcpy(s->d_clock, &c, sizeof(c), hipMemcpyHostToDevice));
+  s->fields_set = 1;
+  return refresh_clock(s);
+}
+
+static int current_parity(bfamx_subdomain_t *s, int *parity)
+{
+  Clock c;
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(&c, s->d_clock, sizeof(c), hipMemcpyDeviceToHost));
+  *parity = s->prec == BFAMX_FP64_PLAIN ? 0 : (int)(c.stage & 1);
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_get_time(bfamx_subdomain_t *s, double *t, int64_t *stages)
+{
+  if (!s)
+    return bfamx_fail(BFAMX_EINVAL, "NULL subdomain");
+  Clock c;
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(&c, s->d_clock, sizeof(c), hipMemcpyDeviceToHost));
+  if (t)
+    *t = c.t;
+  if (stages)
+    *stages = c.stage;
+  return BFAMX_OK;
+}
+
+/* copy a device [K][nf][Np] array of T into bfam's per-field host layout */
+template <class T>
+static int download_layout(const bfamx_subdomain_t *s, const void *src,
+                           double *const *fields)
+{
+  const size_t Np = s->Np, nf = s->nf, K = s->K;
+  std::vector<T> buf(K * nf * Np);
+  HIPCHK(hipMemcpy(buf.data(), src, sizeof(T) * buf.size(), hipMemcpyDeviceToHost));
+  for (size_t e = 0; e < K; ++e)
+    for (size_t f = 0; f < nf; ++f)
+      for (size_t n = 0; n < Np; ++n)
+        fields[f][e * Np + n] = (double)buf[(e * nf + f) * Np + n];
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_download_fields(bfamx_subdomain_t *s,
+                                     double *const *fields)
+{
+  if (!s || !fields)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_download_fields: NULL argument");
+  if (!s->fields_set)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_download_fields: no state set");
+  int parity, rc = current_parity(s, &parity);
+  if (rc)
+    return rc;
+  return s->state_fp32 ? download_layout<float>(s, s->h.q[parity], fields)
+                       : download_layout<double>(s, s->h.q[parity], fields);
+}
+
+static LaunchArgs launch_args(bfamx_subdomain_t *s, int mode)
+{
+  LaunchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.subs = s->d_sub;
+  a.tasks = s->d_tasks;
+  a.sub_ntasks = s->d_ntasks;
+  a.ntasks = s->ntasks;
+  a.mode = mode;
+  a.xcd_remap = s->dev->xcd_remap;
+  a.nxcd = s->dev->num_xcd;
+  return a;
+}
+
+static int grid_for(const bfamx_device_t *d, int ntasks)
+{
+  if (!d->xcd_remap || d->num_xcd <= 1)
+    return ntasks;
+  return d->num_xcd * ((ntasks + d->num_xcd - 1) / d->num_xcd);
+}
+
+static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
+{
+  const int grid = grid_for(s->dev, a.ntasks);
+  hipError_t e = s->physics == BFAMX_ELASTIC3D
+                     ? launch_elastic_stage(s->prec, s->N, a, grid, s->dev->stream)
+                     : launch_acoustic_stage(s->prec, s->N, a, grid, s->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "stage launch failed: %s", hipGetErrorString(e));
+  return BFAMX_OK;
+}
+
+static int run_plain_rhs(bfamx_subdomain_t *s, const LaunchArgs &a)
+{
+  const int grid = grid_for(s->dev, a.ntasks);
+  hipError_t e = s->physics == BFAMX_ELASTIC3D
+                     ? launch_elastic_plain(s->N, a, grid, s->dev->stream)
+                     : launch_acoustic_plain(s->N, a, grid, s->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "plain launch failed: %s", hipGetErrorString(e));
+  return BFAMX_OK;
+}
+
+static int stage_once(bfamx_subdomain_t *s)
+{
+  if (s->prec == BFAMX_FP64_PLAIN)
+  {
+    int rc = run_plain_rhs(s, launch_args(s, MODE_STAGE));
+    if (rc)
+      return rc;
+    const size_t ndof = (size_t)s->K * s->nf * s->Np;
+    const int nb = (int)((ndof + 255) / 256 < (size_t)s->dev->num_cu * 16
+                             ? (ndof + 255) / 256
+                             : (size_t)s->dev->num_cu * 16);
+    hipError_t e = launch_plain_update(s->d_sub, ndof, s->dev->stream, nb);
+    if (e != hipSuccess)
+      return bfamx_fail(BFAMX_EHIP, "update launch failed: %s", hipGetErrorString(e));
+    return BFAMX_OK;
+  }
+  return run_fused(s, launch_args(s, MODE_STAGE));
+}
+
+extern "C" int bfamx_stage(bfamx_subdomain_t *s)
+{
+  if (!s)
+    return bfamx_fail(BFAMX_EINVAL, "NULL subdomain");
+  if (!s->fields_set)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_stage: no state set");
+  return stage_once(s);
+}
+
+extern "C" int bfamx_step(bfamx_subdomain_t *s, int nsteps)
+{
+  if (!s || nsteps < 0)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_step: bad arguments");
+  if (!s->fields_set)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_step: no state set");
+  for (int i = 0; i < nsteps; ++i)
+    for (int st = 0; st < s->h.nstages; ++st)
+    {
+      int rc = stage_once(s);
+      if (rc)
+        return rc;
+    }
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_rhs(bfamx_subdomain_t *s, double t, double *const *rhs)
+{
+  if (!s || !rhs)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_rhs: NULL argument");
+  if (!s->fields_set)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_rhs: no state set");
+  const size_t ndof = (size_t)s->K * s->nf * s->Np;
+  int rc;
+  if (!s->h.rhs)
+  {
+    void *p;
+    rc = dev_alloc(s->dev, s->allocs, &s->bytes, 8 * ndof, &p);
+    if (rc)
+      return rc;
+    s->h.rhs = (double *)p;
+    rc = push_sub(s);
+    if (rc)
+      return rc;
+  }
+  LaunchArgs a = launch_args(s, MODE_RHS);
+  a.t_rhs = t;
+  a.g_rhs = s->h.src_elem >= 0 ? host_ricker(s->h.src_f0, s->h.src_t0, t) : 0.0;
+  a.sw_rhs = s->h.mms ? sin(s->h.mms_omega * t) : 0.0;
+  a.cw_rhs = s->h.mms ? cos(s->h.mms_omega * t) : 0.0;
+  rc = s->prec == BFAMX_FP64_PLAIN ? run_plain_rhs(s, a) : run_fused(s, a);
+  if (rc)
+    return rc;
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  return download_layout<double>(s, s->h.rhs, rhs);
+}
+
+/* Neumaier's compensated sum of n values with stride */
+static double neumaier(const double *x, size_t n, size_t stride)
+{
+  double s = 0, c = 0;
+  for (size_t i = 0; i < n; ++i)
+  {
+    const double v = x[i * stride];
+    const double t = s + v;
+    if (fabs(s) >= fabs(v))
+      c += (s - t) + v;
+    else
+      c += (v - t) + s;
+    s = t;
+  }
+  return s + c;
+}
+
+extern "C" int bfamx_reduce_energy(bfamx_subdomain_t *s, double *energy)
+{
+  if (!s || !energy)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_energy: NULL argument");
+  int parity, rc = current_parity(s, &parity);
+  if (rc)
+    return rc;
+  hipError_t e = launch_energy(s->h, s->state_fp32, parity, s->d_partial,
+                               s->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "energy launch: %s", hipGetErrorString(e));
+  std::vector<double> part(s->K);
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(part.data(), s->d_partial, 8 * part.size(), hipMemcpyDeviceToHost));
+  *energy = neumaier(part.data(), part.size(), 1);
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_reduce_l2_diff(bfamx_subdomain_t *a, bfamx_subdomain_t *b,
+                                    double *diff2, double *norm2)
+{
+  if (!a || !b || !diff2 || !norm2)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_l2_diff: NULL argument");
+  if (a->K != b->K || a->N != b->N || a->physics != b->physics || a->dev != b->dev)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_l2_diff: different shapes");
+  int pa, pb, rc = current_parity(a, &pa);
+  if (rc || (rc = current_parity(b, &pb)))
+    return rc;
+  hipError_t e = launch_l2diff(a->h, a->state_fp32, pa, b->h, b->state_fp32, pb,
+                               a->d_partial, a->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "l2diff launch: %s", hipGetErrorString(e));
+  std::vector<double> part(2 * (size_t)a->K);
+  HIPCHK(hipStreamSynchronize(a->dev->stream));
+  HIPCHK(hipMemcpy(part.data(), a->d_partial, 8 * part.size(), hipMemcpyDeviceToHost));
+  *diff2 = neumaier(part.data(), a->K, 2);
+  *norm2 = neumaier(part.data() + 1, a->K, 2);
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_reduce_mms_error(bfamx_subdomain_t *s, const double *A,
+                                      double *diff2, double *norm2)
+{
+  if (!s || !A || !diff2 || !norm2 || !s->h.mms_tab)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_mms_error: no forcing set");
+  Clock c;
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(&c, s->d_clock, sizeof(c), hipMemcpyDeviceToHost));
+  const int parity = s->prec == BFAMX_FP64_PLAIN ? 0 : (int)(c.stage & 1);
+  /* the state is at a step boundary only when stage % nstages == 0 */
+  hipError_t e = launch_mms_error(s->h, s->state_fp32, parity, A,
+                                  cos(s->h.mms_omega * c.t),
+                                  sin(s->h.mms_omega * c.t), s->d_partial,
+                                  s->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "mms error launch: %s", hipGetErrorString(e));
+  std::vector<double> part(2 * (size_t)s->K);
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(part.data(), s->d_partial, 8 * part.size(), hipMemcpyDeviceToHost));
+  *diff2 = neumaier(part.data(), s->K, 2);
+  *norm2 = neumaier(part.data() + 1, s->K, 2);
+  return BFAMX_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* batches                                                                */
+/* ---------------------------------------------------------------------- */
+
+extern "C" int bfamx_batch_create(bfamx_device_t *dev,
+                                  bfamx_subdomain_t *const *subs, int nsubs,
+                                  bfamx_batch_t **out)
+{
+  if (!dev || !subs || nsubs < 1 || !out)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_create: bad arguments");
+  *out = NULL;
+  const int physics = subs[0]->physics, prec = subs[0]->prec;
+  int N = subs[0]->N;
+  for (int i = 0; i < nsubs; ++i)
+  {
+    const bfamx_subdomain_t *s = subs[i];
+    if (!s || s->dev != dev || s->physics != physics || s->prec != prec)
+      return bfamx_fail(BFAMX_EINVAL,
+                        "batch members must share device, physics, precision");
+    if (prec == BFAMX_FP64_PLAIN)
+      return bfamx_fail(BFAMX_EUNSUPPORTED, "batches run the fused stage only");
+    if (s->N != N)
+      N = -1;
+  }
+  if (N < 0 && physics != BFAMX_ELASTIC3D)
+    return bfamx_fail(BFAMX_EUNSUPPORTED, "mixed orders: elastic only");
+  bfamx_batch_t *b = new bfamx_batch_t();
+  b->dev = dev;
+  b->subs.assign(subs, subs + nsubs);
+  b->physics = physics;
+  b->prec = prec;
+  b->N = N;
+  b->bytes = 0;
+  b->graph = NULL;
+  b->exec = NULL;
+  b->graph_steps = 0;
+  std::vector<SubDev> hs(nsubs);
+  std::vector<Task> tasks;
+  std::vector<int> per(nsubs);
+  for (int i = 0; i < nsubs; ++i)
+  {
+    bfamx_subdomain_t *s = subs[i];
+    hs[i] = s->h;
+    const int epb = N < 0 ? elastic_epb(s->N) : epb_of(physics, s->N);
+    int cnt = 0;
+    for (int32_t e0 = 0; e0 < s->K; e0 += epb, ++cnt)
+    {
+      Task t;
+      t.sub = i;
+      t.e0 = e0;
+      t.ne = e0 + epb <= s->K ? epb : s->K - e0;
+      t.N = s->N;
+      tasks.push_back(t);
+    }
+    per[i] = cnt;
+  }
+  b->ntasks = (int)tasks.size();
+  std::vector<void *> allocs;
+  void *p;
+  int rc = dev_alloc(dev, allocs, &b->bytes, sizeof(SubDev) * nsubs, &p);
+  if (!rc)
+  {
+    b->d_subs = (SubDev *)p;
+    rc = dev_alloc(dev, allocs, &b->bytes, sizeof(Task) * tasks.size(), &p);
+  }
+  if (!rc)
+  {
+    b->d_tasks = (Task *)p;
+    rc = dev_alloc(dev, allocs, &b->bytes, sizeof(int) * nsubs, &p);
+  }
+  if (rc)
+  {
+    dev_free_all(dev, allocs, b->bytes);
+    delete b;
+    return rc;
+  }
+  b->d_ntasks = (int *)p;
+  if (hipMemcpy(b->d_subs, hs.data(), sizeof(SubDev) * nsubs, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_tasks, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_ntasks, per.data(), sizeof(int) * nsubs, hipMemcpyHostToDevice) != hipSuccess)
+  {
+    dev_free_all(dev, allocs, b->bytes);
+    delete b;
+    return bfamx_fail(BFAMX_EHIP, "batch upload failed");
+  }
+  *out = b;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_batch_free(bfamx_batch_t *b)
+{
+  if (!b)
+    return BFAMX_OK;
+  (void)hipStreamSynchronize(b->dev->stream);
+  if (b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if (b->graph)
+    (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->d_subs);
+  (void)hipFree(b->d_tasks);
+  (void)hipFree(b->d_ntasks);
+  b->dev->used -= b->bytes;
+  delete b;
+  return BFAMX_OK;
+}
+
+static int batch_stage(bfamx_batch_t *b)
+{
+  LaunchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.subs = b->d_subs;
+  a.tasks = b->d_tasks;
+  a.sub_ntasks = b->d_ntasks;
+  a.ntasks = b->ntasks;
+  a.mode = MODE_STAGE;
+  a.xcd_remap = b->dev->xcd_remap;
+  a.nxcd = b->dev->num_xcd;
+  const int grid = grid_for(b->dev, b->ntasks);
+  hipError_t e = b->physics == BFAMX_ELASTIC3D
+                     ? launch_elastic_stage(b->prec, b->N, a, grid, b->dev->stream)
+                     : launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "batch stage launch: %s", hipGetErrorString(e));
+  return BFAMX_OK;
+}
+
+static int batch_nstages(const bfamx_batch_t *b)
+{
+  int ns = 0;
+  for (auto *s : b->subs)
+    ns = s->h.nstages > ns ? s->h.nstages : ns;
+  return ns;
+}
+
+extern "C" int bfamx_batch_step(bfamx_batch_t *b, int nsteps)
+{
+  if (!b || nsteps < 0)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_step: bad arguments");
+  for (auto *s : b->subs)
+    if (!s->fields_set)
+      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  const int ns = batch_nstages(b);
+  for (int i = 0; i < nsteps * ns; ++i)
+  {
+    int rc = batch_stage(b);
+    if (rc)
+      return rc;
+  }
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_batch_capture(bfamx_batch_t *b, int nsteps)
+{
+  if (!b || nsteps < 1)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_capture: bad arguments");
+  if (b->exec)
+  {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = NULL;
+  }
+  if (b->graph)
+  {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = NULL;
+  }
+  hipStream_t st = b->dev->stream;
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int ns = batch_nstages(b);
+  int rc = BFAMX_OK;
+  for (int i = 0; i < nsteps * ns && rc == BFAMX_OK; ++i)
+    rc = batch_stage(b);
+  hipGraph_t g = NULL;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc)
+  {
+    if (g)
+      (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "graph capture: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, g, NULL, NULL, 0));
+  b->graph_steps = nsteps;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_batch_launch_graph(bfamx_batch_t *b)
+{
+  if (!b || !b->exec)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_launch_graph: not captured");
+  for (auto *s : b->subs)
+    if (!s->fields_set)
+      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  HIPCHK(hipGraphLaunch(b->exec, b->dev->stream));
+  return BFAMX_OK;
+}

@@ -1,0 +1,191 @@
+"""Synthetic workloads of the benchmark configurations, built with the library's
+own host setup (operators, brick meshes, PCG32 materials, Ricker sources).
+
+Config names:
+  acoustic2d_plane_wave   16x16 periodic quads, N=4, plane wave, 100 KC54 steps
+  elastic3d_ricker        48^3 periodic hexes, N=7, PCG32 material, Ricker
+  elastic3d_mms           32^3 periodic hexes, N=7, manufactured solution
+The time step used everywhere is dt = CFL * h_min / (c_max (N+1)^2); the
+reference has no CFL rule, so this one is stated here and used by the
+oracle and the device runs alike.
+"""
+import math
+
+import numpy as np
+
+import bfam_amd as bx
+
+
+def ops(N):
+    r, w = bx.lgl(N)
+    return r, w, bx.diff_matrix(N, r)
+
+
+def cfl_dt(hmin, cmax, N, cfl=0.5):
+    return cfl * hmin / (cmax * (N + 1) ** 2)
+
+
+class Mesh:
+    """Brick mesh + LGL operators + volume maps for order N."""
+
+    def __init__(self, dim, n, N, L=None, periodic=None, morton=True):
+        self.dim, self.N = dim, N
+        periodic = periodic if periodic is not None else [1] * dim
+        m = bx.brick(dim, n, periodic, L, morton)
+        self.__dict__.update(m)
+        self.r, self.w, self.Dr = ops(N)
+        self.vmapM, self.vmapP = bx.build_vmaps(dim, N, self.EToE, self.EToF)
+        self.Np = (N + 1) ** dim
+
+    def coords(self):
+        """x[d] of shape (K*Np,) for every volume node (bfam node order)."""
+        n1 = self.N + 1
+        idx = np.arange(self.Np)
+        loc = [idx % n1, (idx // n1) % n1, idx // (n1 * n1)]
+        out = []
+        for d in range(self.dim):
+            ref = 0.5 * (self.r[loc[d]] + 1.0)
+            out.append((self.x0[:, d][:, None] + ref[None, :] * self.h[:, d][:, None]).ravel())
+        return out
+
+    def locate(self, x):
+        """storage index of the element containing point x, and its reference
+        coordinates in [-1, 1]^dim"""
+        n = np.array(self.n)
+        L = np.array(self.L)
+        hh = L / n
+        ijk = np.minimum((np.array(x) / hh).astype(int), n - 1)
+        lexi = int(ijk[0] + n[0] * (ijk[1] + (n[1] * ijk[2] if self.dim == 3 else 0)))
+        e = int(np.nonzero(self.lex == lexi)[0][0])
+        rs = 2.0 * (np.array(x) - self.x0[e]) / self.h[e] - 1.0
+        return e, rs
+
+
+def pcg_material(K_lex, seed, rho=(1.5, 3.0), cp=(3.0, 6.0), ratio=(1.6, 2.0)):
+    """Per-element rho ~ U[rho], cp ~ U[cp], cs = cp / U[ratio], drawn in
+    lexicographic element order from PCG32(seed, 54)."""
+    g = bx.PCG32(seed, 54)
+    out = np.zeros((K_lex, 3))
+    for l in range(K_lex):
+        rr = g.uniform(*rho)
+        c = g.uniform(*cp)
+        out[l] = (rr, c, c / g.uniform(*ratio))
+    return out
+
+
+def point_source_weights(mesh, e, rs):
+    """weights[n] = l_i(r) l_j(s) (l_k(t)) / (J w_i w_j (w_k)): the DG projection
+    M^{-1} phi(x_s) of a point source on the LGL (diagonal) mass matrix."""
+    N, n1 = mesh.N, mesh.N + 1
+    lx = [bx.lagrange_eval(N, mesh.r, rs[d]) for d in range(mesh.dim)]
+    J = np.prod(0.5 * mesh.h[e])
+    idx = np.arange(mesh.Np)
+    loc = [idx % n1, (idx // n1) % n1, idx // (n1 * n1)]
+    wgt = np.ones(mesh.Np)
+    for d in range(mesh.dim):
+        wgt *= lx[d][loc[d]] / mesh.w[loc[d]]
+    return wgt / J
+
+
+def elastic3d_ricker(n=48, N=7, seed=2024, cfl=0.5, nsteps=200, L=1.0):
+    """Config: 3-D isotropic elastic, uniform n^3 periodic hex brick, order N,
+    PCG32 per-element material, explosive Ricker point source at a seeded
+    interior point.  Ricker f0 is chosen so the wavelet peaks mid-run."""
+    mesh = Mesh(3, [n, n, n], N, L=[L, L, L])
+    mat_lex = pcg_material(mesh.K, seed)
+    mat = mat_lex[mesh.lex]
+    hmin = float(mesh.h.min())
+    dt = cfl_dt(hmin, float(mat[:, 1].max()), N, cfl)
+    T = nsteps * dt
+    g = bx.PCG32(seed, 7)
+    xs = [L * g.uniform(0.25, 0.75) for _ in range(3)]
+    e, rs = mesh.locate(xs)
+    src = {"elem": e, "weights": point_source_weights(mesh, e, rs),
+           "amp": [0, 0, 0, 1, 1, 1, 0, 0, 0], "t0": 0.5 * T, "f0": 2.4 / T, "x": xs}
+    return {"mesh": mesh, "rho": mat[:, 0].copy(), "cp": mat[:, 1].copy(),
+            "cs": mat[:, 2].copy(), "dt": dt, "nsteps": nsteps, "src": src,
+            "physics": bx.ELASTIC3D}
+
+
+def acoustic2d_plane_wave(n=16, N=4, cfl=0.5, nsteps=100, rho=1.0, c=1.0):
+    """Config: 2-D acoustic on the periodic unit square, p = cos(2 pi (x+y)),
+    velocity of the plane wave travelling along (1,1)/sqrt(2)."""
+    mesh = Mesh(2, [n, n], N, L=[1.0, 1.0])
+    x, y = mesh.coords()
+    p = np.cos(2 * np.pi * (x + y))
+    u = p / (rho * c * math.sqrt(2.0))
+    q = np.stack([p, u, u])
+    dt = cfl_dt(1.0 / n, c, N, cfl)
+    K = mesh.K
+    return {"mesh": mesh, "rho": np.full(K, rho), "cp": np.full(K, c), "cs": None,
+            "dt": dt, "nsteps": nsteps, "q0": q, "physics": bx.ACOUSTIC2D,
+            "omega": 2 * np.pi * math.sqrt(2.0) * c}
+
+
+def acoustic2d_exact(cfg, t):
+    x, y = cfg["mesh"].coords()
+    c = float(cfg["cp"][0])
+    rho = float(cfg["rho"][0])
+    p = np.cos(2 * np.pi * (x + y) - cfg["omega"] * t)
+    u = p / (rho * c * math.sqrt(2.0))
+    return np.stack([p, u, u])
+
+
+MMS_A = np.array([1.0, 0.7, -0.4, 0.5, -0.3, 0.8, 0.35, -0.25, 0.6])
+
+
+def mms_forcing_coef(A, k, omega, rho, lam, mu):
+    """F = dq/dt - L q for the manufactured solution
+    v_f = A_f P_f(x) cos(w t), sigma_f = A_f P_f(x) sin(w t)."""
+    sv = A[0] + A[1] + A[2]
+    F = np.zeros(9)
+    F[0] = -omega * A[0] - k * (-A[3] + A[6] + A[7]) / rho
+    F[1] = -omega * A[1] - k * (A[6] - A[4] + A[8]) / rho
+    F[2] = -omega * A[2] - k * (A[7] + A[8] - A[5]) / rho
+    F[3] = omega * A[3] - k * (lam * sv + 2 * mu * A[0])
+    F[4] = omega * A[4] - k * (lam * sv + 2 * mu * A[1])
+    F[5] = omega * A[5] - k * (lam * sv + 2 * mu * A[2])
+    F[6] = omega * A[6] + mu * k * (A[0] + A[1])
+    F[7] = omega * A[7] + mu * k * (A[0] + A[2])
+    F[8] = omega * A[8] + mu * k * (A[1] + A[2])
+    return F
+
+
+def mms_exact(mesh, A, k, omega, t):
+    x, y, z = mesh.coords()
+    S = [np.sin(k * x), np.sin(k * y), np.sin(k * z)]
+    Cc = [np.cos(k * x), np.cos(k * y), np.cos(k * z)]
+    pat = [S[0] * Cc[1] * Cc[2], Cc[0] * S[1] * Cc[2], Cc[0] * Cc[1] * S[2],
+           Cc[0] * Cc[1] * Cc[2], Cc[0] * Cc[1] * Cc[2], Cc[0] * Cc[1] * Cc[2],
+           S[0] * S[1] * Cc[2], S[0] * Cc[1] * S[2], Cc[0] * S[1] * S[2]]
+    tv, ts = math.cos(omega * t), math.sin(omega * t)
+    return np.stack([A[f] * pat[f] * (tv if f < 3 else ts) for f in range(9)])
+
+
+def elastic3d_mms(n=32, N=7, cfl=0.5, rho=1.0, cp=math.sqrt(3.0), cs=1.0, omega=None):
+    """Config: manufactured smooth sin-product solution with forcing on the
+    periodic unit cube, homogeneous material."""
+    mesh = Mesh(3, [n, n, n], N, L=[1.0, 1.0, 1.0])
+    k = 2 * np.pi
+    omega = omega if omega is not None else 2 * np.pi
+    lam = rho * (cp * cp - 2 * cs * cs)
+    mu = rho * cs * cs
+    F = mms_forcing_coef(MMS_A, k, omega, rho, lam, mu)
+    K = mesh.K
+    dt = cfl_dt(1.0 / n, cp, N, cfl)
+    return {"mesh": mesh, "rho": np.full(K, rho), "cp": np.full(K, cp), "cs": np.full(K, cs),
+            "dt": dt, "k": k, "omega": omega, "F": F, "A": MMS_A.copy(),
+            "lam": lam, "mu": mu, "physics": bx.ELASTIC3D}
+
+
+def upload(dev, cfg, precision):
+    """bfam_amd.Subdomain for a config dict (source / forcing attached)."""
+    m = cfg["mesh"]
+    sub = bx.Subdomain(dev, cfg["physics"], precision, m.N, m.r, m.w, m.Dr, m.vmapM, m.vmapP,
+                       m.h, cfg["rho"], cfg["cp"], cfg.get("cs"), x0=m.x0)
+    if cfg.get("src") is not None:
+        s = cfg["src"]
+        sub.set_source(s["elem"], s["weights"], s["amp"], s["f0"], s["t0"])
+    if cfg.get("F") is not None:
+        sub.set_forcing_mms(cfg["k"], cfg["omega"], cfg["F"])
+    return sub

@@ -1,0 +1,272 @@
+/*
+ * bfamx.h -- C ABI of the MI355X-native DG right-hand side and low-storage
+ * Runge-Kutta hot path that replaces bfam's device layer.
+ *
+ * Everything crossing this boundary is plain C: int status codes, plain
+ * pointers and sizes, caller-owned host arrays.  Device buffers are owned by
+ * the handles.  One handle set per host thread; every device call of a
+ * bfamx_device_t is issued on that device's single HIP stream
+ * (bfamx_device_stream()).
+ *
+ * Reference interfaces replaced (src/bfam.h and src/bfam.c of bfam):
+ *   bfamx_dgx_t              data of bfam_subdomain_dgx_t        bfam.h:1364-1420
+ *   bfamx_glue_t             bfam_subdomain_dgx_glue_data_t       bfam.h:1319-1362
+ *   bfamx_lsrk_t / _init     bfamo_lsrk_t, bfamo_lsrk_init        bfam.h:1840-1863,
+ *                                                                 bfam.c:8996-9090
+ *   bfamx_device_create      bfamo_device_t + bfamo_device_malloc bfam.h:1740-1744,
+ *                            (mem_fudge budget)                   bfam.c:8561-8589
+ *   bfamx_subdomain_upload   bfamo_add_operator_* uploads and     bfam.c:8712-8828,
+ *                            field allocation                     bfam.c:6574-6597
+ *   bfamx_rhs/_stage/_step   the application RHS kernels launched through
+ *                            BFAMO_KERNEL_RUN                     bfam.h:1673-1719
+ *   (on one GPU the glue trace exchange bfamo_communicator_post_* and _wait,
+ *    bfam.c:8900-8929, becomes a device-side gather inside the stage kernel)
+ *   bfamx_timer_*            BFAMO_KERNEL_RUN's KERNEL_TIME path  bfam.h:1673-1708
+ *   host setup mirrors       bfam_jacobi_* (bfam.c:2681-3135), gmask/buildmaps
+ *                            (bfam.c:6599-6877), create_interpolators
+ *                            (bfam.c:5606-5691), bfam_pcg32_* (bfam.c:8328-8387)
+ */
+#ifndef BFAMX_H
+#define BFAMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* status codes (the reference aborts; bfamx returns, see abort_on_error)  */
+/* ---------------------------------------------------------------------- */
+#define BFAMX_OK 0
+#define BFAMX_EINVAL 1       /* invalid argument or inconsistent input */
+#define BFAMX_EBUDGET 2      /* HBM budget exceeded (bfam.c:8571-8578) */
+#define BFAMX_EHIP 3         /* HIP runtime error */
+#define BFAMX_ESTATE 4       /* object in the wrong state for the call */
+#define BFAMX_EUNSUPPORTED 5 /* order / physics / precision not built */
+
+/* message of the last error on the calling thread */
+const char *bfamx_last_error(void);
+/* 1: abort() on any error, as BFAM_ABORT does (bfam.h:36-38); default 0 */
+void bfamx_set_abort_on_error(int on);
+
+/* ---------------------------------------------------------------------- */
+/* LSRK coefficients -- bfamo_lsrk_t / bfamo_lsrk_init (bfam.c:8996-9090)  */
+/* ---------------------------------------------------------------------- */
+typedef enum
+{
+  BFAMX_LSRK_KC54 = 0, /* same order as bfamo_lsrk_method_t, bfam.h:1848 */
+  BFAMX_LSRK_FE = 1,
+  BFAMX_LSRK_HEUN = 2,
+  BFAMX_LSRK_W33 = 3,
+  BFAMX_LSRK_NOOP = 4 /* not a scheme: falls back to KC54 like bfam.c:9014 */
+} bfamx_lsrk_method_t;
+
+typedef struct
+{
+  int n_stages;
+  double A[8];
+  double B[8];
+  double C[9];
+} bfamx_lsrk_t;
+
+/* fills the FP64 tables bit-for-bit as bfamo_lsrk_init with REAL_DOUBLE */
+int bfamx_lsrk_init(bfamx_lsrk_t *ts, bfamx_lsrk_method_t method);
+
+/* ---------------------------------------------------------------------- */
+/* host setup: operators and maps (replaces the bfam setup the hot path     */
+/* consumes; results are checked against the reference restatement)       */
+/* ---------------------------------------------------------------------- */
+/* Legendre-Gauss-Lobatto nodes and weights, N >= 1 (bfam.c:3013) */
+int bfamx_lgl(int N, double *r, double *w);
+/* 1-D differentiation matrix, column-major Dr[i + j*(N+1)] = l_j'(r_i)
+ * (bfam.c:3082, stored as Dr_N by generic_init bfam.c:7087-7155) */
+int bfamx_diff_matrix(int N, const double *r, double *Dr);
+/* Lagrange basis of the LGL nodes r (N+1) evaluated at x: out[j] = l_j(x) */
+int bfamx_lagrange_eval(int N, const double *r, double x, double *out);
+/*
+ * 1-D mortar operators from order Na to Nb (create_interpolators,
+ * bfam.c:5606-5691).  prj, mass_prj, wi_mass_prj each receive 5 column-major
+ * blocks of (Nb+1)x(Na+1): same, coarsen-from-bottom, coarsen-from-top,
+ * refine-to-bottom, refine-to-top.  *prj0_identity is 1 when Na == Nb, and
+ * block 0 of prj is then left as the identity.
+ */
+int bfamx_mortar_ops(int Na, int Nb, double *prj, double *mass_prj,
+                     double *wi_mass_prj, int *prj0_identity);
+
+/* face node lists: gmask[f*Nfp + n], faces -x,+x,-y,+y(,-z,+z) (bfam.c:6599) */
+int bfamx_face_mask(int dim, int N, int32_t *gmask);
+/*
+ * vmapM/vmapP from element-to-element connectivity (bfam.c:6783-6877).
+ * EToF[k*Nfaces+f] = orientation*Nfaces + neighbour face, p4est codes.
+ */
+int bfamx_build_vmaps(int dim, int N, int32_t K, const int32_t *EToE,
+                      const int8_t *EToF, int32_t *vmapM, int32_t *vmapP);
+/*
+ * Compress per-node vmapP into per-face connectivity: nbr_elem[k*Nfaces+f]
+ * and nbr_code[k*Nfaces+f] = perm*8 + neighbour face, where perm (0..7 in 3-D,
+ * 0..1 in 2-D) is bit0 = reverse first face index, bit1 = reverse second,
+ * bit2 = swap.  Returns BFAMX_EINVAL if some face's plus nodes are not one
+ * whole neighbour face under one of those permutations.
+ */
+int bfamx_face_connectivity(int dim, int N, int32_t K, const int32_t *vmapM,
+                            const int32_t *vmapP, int32_t *nbr_elem,
+                            uint8_t *nbr_code);
+
+/* PCG32 (O'Neill 2014), bfam_pcg32_* (bfam.c:8328-8387) */
+typedef struct
+{
+  uint64_t state;
+  uint64_t inc;
+} bfamx_pcg32_t;
+void bfamx_pcg32_seed(bfamx_pcg32_t *rng, uint64_t initstate,
+                      uint64_t initseq);
+uint32_t bfamx_pcg32_next(bfamx_pcg32_t *rng);
+uint32_t bfamx_pcg32_bounded(bfamx_pcg32_t *rng, uint32_t bound);
+/* uniform double in [0,1) from one 32-bit draw */
+double bfamx_pcg32_uniform(bfamx_pcg32_t *rng);
+
+/*
+ * Uniform brick of n[0] x n[1] (x n[2]) elements on [0,L]^dim.  Elements are
+ * stored in Morton (z-)order when morton != 0 (as p4est stores a uniform
+ * forest), else lexicographically.  Fills EToE/EToF (K*2*dim) with p4est
+ * semantics (conforming, orientation 0; non-periodic faces map to
+ * themselves), lex[k] = lexicographic index of element k, and per-element
+ * lower corner x0 and edge lengths h (K*dim each).
+ */
+int bfamx_brick(int dim, const int *n, const int *periodic, const double *L,
+                int morton, int32_t *EToE, int8_t *EToF, int32_t *lex,
+                double *x0, double *h);
+
+/* ---------------------------------------------------------------------- */
+/* device                                                                  */
+/* ---------------------------------------------------------------------- */
+typedef struct bfamx_device bfamx_device_t;
+
+/* hbm_budget_bytes = 0: 90% of the device's memory (cf. mem_fudge) */
+int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
+                        bfamx_device_t **dev);
+int bfamx_device_destroy(bfamx_device_t *dev);
+int bfamx_device_memory(const bfamx_device_t *dev, size_t *used,
+                        size_t *budget);
+int bfamx_device_synchronize(bfamx_device_t *dev);
+/* the device's HIP stream, as a void* (hipStream_t) */
+void *bfamx_device_stream(bfamx_device_t *dev);
+/* number of compute units and XCDs the kernels are tiled for */
+int bfamx_device_info(const bfamx_device_t *dev, int *num_cu, int *num_xcd);
+
+/* ---------------------------------------------------------------------- */
+/* volume subdomain + physics                                             */
+/* ---------------------------------------------------------------------- */
+typedef enum
+{
+  BFAMX_ACOUSTIC2D = 0, /* fields p, u, v */
+  BFAMX_ELASTIC3D = 1   /* fields v1 v2 v3 s11 s22 s33 s12 s13 s23 */
+} bfamx_physics_t;
+
+typedef enum
+{
+  BFAMX_FP64 = 0,       /* fused stage, FP64 state, rates and arithmetic */
+  BFAMX_FP64_PLAIN = 1, /* unfused volume / surface / update kernels, FP64 */
+  BFAMX_FP32_RHS = 2,   /* fused, FP64 state, FP32 rates and arithmetic */
+  BFAMX_FP32 = 3        /* fused, FP32 state, rates and arithmetic */
+} bfamx_precision_t;
+
+/* the hot-path data of bfam_subdomain_dgx_t (bfam.h:1364-1420) */
+typedef struct
+{
+  int dim;
+  int N;
+  int Np;
+  int32_t K;
+  const double *r;  /* N+1 LGL nodes */
+  const double *w;  /* N+1 LGL weights */
+  const double *wi; /* N+1 inverse weights */
+  const double *Dr; /* (N+1)^2, column-major */
+  const int32_t *vmapM;
+  const int32_t *vmapP;
+} bfamx_dgx_t;
+
+typedef struct
+{
+  bfamx_physics_t physics;
+  bfamx_precision_t precision;
+  const double *elem_h;  /* K*dim affine element edge lengths */
+  const double *elem_x0; /* K*dim lower corners (needed for MMS forcing) */
+  const double *rho;     /* K */
+  const double *cp;      /* K: P-wave (acoustic: sound) speed */
+  const double *cs;      /* K: S-wave speed (ignored for acoustics) */
+} bfamx_physics_desc_t;
+
+typedef struct bfamx_subdomain bfamx_subdomain_t;
+
+int bfamx_subdomain_upload(bfamx_device_t *dev, const bfamx_dgx_t *sub,
+                           const bfamx_physics_desc_t *phys,
+                           const bfamx_lsrk_t *lsrk, bfamx_subdomain_t **out);
+int bfamx_subdomain_free(bfamx_subdomain_t *sub);
+int bfamx_subdomain_info(const bfamx_subdomain_t *sub, int *nfields,
+                         size_t *device_bytes);
+/* device bytes bfamx_subdomain_upload would allocate (admission control) */
+size_t bfamx_subdomain_bytes(int dim, int N, int32_t K, bfamx_physics_t ph,
+                             bfamx_precision_t prec);
+
+/* Ricker point source: rate[f][elem, n] += weights[n]*amp[f]*ricker(t) with
+ * ricker = (1-2a)exp(-a), a = (pi f0 (t-t0))^2; weights = l_n(x_s)/(J w_n) */
+int bfamx_set_source(bfamx_subdomain_t *sub, int32_t elem,
+                     const double *weights, const double *amp, double f0,
+                     double t0);
+/* manufactured forcing (elastic): rate[f] += F[f] P_f(x) tau_f(t),
+ * P_f = products of sin/cos(k x_d), tau = sin(wt) for v, cos(wt) for sigma */
+int bfamx_set_forcing_mms(bfamx_subdomain_t *sub, double k, double omega,
+                          const double *F);
+
+/* state: nfields host arrays of K*Np doubles (bfam field layout) */
+int bfamx_set_fields(bfamx_subdomain_t *sub, const double *const *fields,
+                     double t0, double dt);
+int bfamx_download_fields(bfamx_subdomain_t *sub, double *const *fields);
+int bfamx_get_time(bfamx_subdomain_t *sub, double *t, int64_t *stages);
+
+/* L(q, t) of the current state into host arrays (parity checks) */
+int bfamx_rhs(bfamx_subdomain_t *sub, double t, double *const *rhs);
+/* one LSRK stage / nsteps full steps with the subdomain's dt */
+int bfamx_stage(bfamx_subdomain_t *sub);
+int bfamx_step(bfamx_subdomain_t *sub, int nsteps);
+
+/* FP64 reductions with compensated sums */
+int bfamx_reduce_energy(bfamx_subdomain_t *sub, double *energy);
+/* sum J w |qa - qb|^2 and sum J w |qb|^2 over all fields */
+int bfamx_reduce_l2_diff(bfamx_subdomain_t *a, bfamx_subdomain_t *b,
+                         double *diff2, double *norm2);
+/* same against the manufactured solution with amplitudes A at the current t */
+int bfamx_reduce_mms_error(bfamx_subdomain_t *sub, const double *A,
+                           double *diff2, double *norm2);
+
+/* ---------------------------------------------------------------------- */
+/* batches: many subdomains (mixed N, independent problems) advanced by one */
+/* launch per LSRK stage, and HIP-graph capture of the stage loop          */
+/* ---------------------------------------------------------------------- */
+typedef struct bfamx_batch bfamx_batch_t;
+
+int bfamx_batch_create(bfamx_device_t *dev, bfamx_subdomain_t *const *subs,
+                       int nsubs, bfamx_batch_t **out);
+int bfamx_batch_free(bfamx_batch_t *b);
+int bfamx_batch_step(bfamx_batch_t *b, int nsteps);
+/* capture nsteps steps of the batch into a HIP graph; launch replays it */
+int bfamx_batch_capture(bfamx_batch_t *b, int nsteps);
+int bfamx_batch_launch_graph(bfamx_batch_t *b);
+
+/* ---------------------------------------------------------------------- */
+/* timing on the device stream (BFAMO_KERNEL_RUN's KERNEL_TIME analogue)   */
+/* ---------------------------------------------------------------------- */
+int bfamx_timer_start(bfamx_device_t *dev);
+int bfamx_timer_stop(bfamx_device_t *dev, float *ms);
+
+/* launch tuning: blocks remapped so each XCD walks a contiguous element
+ * range (1, default) or hardware round-robin order (0) */
+int bfamx_set_tuning(bfamx_device_t *dev, int xcd_remap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BFAMX_H */

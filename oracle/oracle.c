@@ -1,0 +1,1263 @@
+/*
+ * oracle.c -- CPU FP64 restatement of bfam's DG hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  It is never linked into or
+ * called by the shipped library.  Each function cites the reference lines it
+ * follows (src/bfam.c unless stated).  Compiled with -ffp-contract=off so that
+ * the operator builders round like the reference's gnu99/-O2 build.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAX(a, b) ((a) > (b) ? (a) : (b))
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* BFAM_LONG_REAL_APPROX_EQ (bfam.h:182, bfam.c:23) with long real = double */
+static int approx_eq(double x, double y, double K)
+{
+  const double eps = DBL_EPSILON;
+  return fabs(x - y) < eps * eps + K * eps * ORC_MAX(fabs(x), fabs(y));
+}
+
+/* ------------------------------------------------------------------------ */
+/* dense helpers (1091-1306)                                                */
+/* ------------------------------------------------------------------------ */
+
+/* B (n x m, ldb) = A^T where A is m x n with leading dim lda (1091) */
+static void mtranspose(int m, int n, const double *A, int lda, double *B,
+                       int ldb)
+{
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j)
+      B[i * ldb + j] = A[j * lda + i];
+}
+
+/* C += A^T B, A is k x m (1100) */
+static void mTmmult(int m, int n, int k, const double *A, int lda,
+                    const double *B, int ldb, double *C, int ldc)
+{
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int p = 0; p < k; ++p)
+        C[j * ldc + i] += A[i * lda + p] * B[j * ldb + p];
+}
+
+/* C += A B (1111) */
+static void mmmult(int m, int n, int k, const double *A, int lda,
+                   const double *B, int ldb, double *C, int ldc)
+{
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int p = 0; p < k; ++p)
+        C[j * ldc + i] += A[p * lda + i] * B[j * ldb + p];
+}
+
+/* Gaussian elimination with complete pivoting, Golub & van Loan 3.4.2 (1122) */
+static void lu_factor(int n, double *A, int *p, int *q)
+{
+  for (int k = 0; k < n; ++k)
+    p[k] = q[k] = k;
+
+  for (int k = 0; k < n - 1; ++k)
+  {
+    int mu = k, lambda = k;
+    double a_max = 0.0;
+    for (int j = k; j < n; ++j)
+      for (int i = k; i < n; ++i)
+      {
+        const double a_abs = fabs(A[i + n * j]);
+        if (a_abs > a_max)
+        {
+          a_max = a_abs;
+          mu = i;
+          lambda = j;
+        }
+      }
+
+    int t = p[k];
+    p[k] = p[mu];
+    p[mu] = t;
+    for (int j = 0; j < n; ++j)
+    {
+      const double r = A[k + n * j];
+      A[k + n * j] = A[mu + n * j];
+      A[mu + n * j] = r;
+    }
+
+    t = q[k];
+    q[k] = q[lambda];
+    q[lambda] = t;
+    for (int i = 0; i < n; ++i)
+    {
+      const double r = A[i + n * k];
+      A[i + n * k] = A[i + n * lambda];
+      A[i + n * lambda] = r;
+    }
+
+    if (fabs(A[k + n * k]) > DBL_EPSILON * DBL_EPSILON)
+    {
+      for (int i = k + 1; i < n; ++i)
+        A[i + n * k] = A[i + n * k] / A[k + n * k];
+      for (int i = k + 1; i < n; ++i)
+        for (int j = k + 1; j < n; ++j)
+          A[i + n * j] = A[i + n * j] - A[i + n * k] * A[k + n * j];
+    }
+  }
+}
+
+/* forward (unit L) then backward substitution, column versions (1202) */
+static void lu_solve(int n, const double *LU, double *x, const int *p,
+                     const int *q, double *work)
+{
+  for (int k = 0; k < n; ++k)
+    work[k] = x[p[k]];
+  for (int j = 0; j < n - 1; ++j)
+    for (int i = j + 1; i < n; ++i)
+      work[i] = work[i] - work[j] * LU[i + n * j];
+  for (int j = n - 1; j > 0; --j)
+  {
+    work[j] = work[j] / LU[j + n * j];
+    for (int i = 0; i < j; ++i)
+      work[i] = work[i] - work[j] * LU[i + n * j];
+  }
+  work[0] = work[0] / LU[0];
+  for (int k = 0; k < n; ++k)
+    x[q[k]] = work[k];
+}
+
+/* C = A \ B, A is m x m, B is m x n (1262) */
+void orc_backslash(int m, int n, const double *A, const double *B, double *C)
+{
+  double *LU = malloc(sizeof(double) * m * m);
+  double *work = malloc(sizeof(double) * m);
+  int *p = malloc(sizeof(int) * m);
+  int *q = malloc(sizeof(int) * m);
+  memcpy(LU, A, sizeof(double) * m * m);
+  memcpy(C, B, sizeof(double) * m * n);
+  lu_factor(m, LU, p, q);
+  for (int j = 0; j < n; ++j)
+    lu_solve(m, LU, C + j * m, p, q, work);
+  free(q);
+  free(p);
+  free(work);
+  free(LU);
+}
+
+/* C = A / B = A B^{-1}, A is m x n, B is n x n (1287) */
+static void forwardslash(int m, int n, const double *A, const double *B,
+                         double *C)
+{
+  double *AT = malloc(sizeof(double) * n * m);
+  double *BT = malloc(sizeof(double) * n * n);
+  double *CT = malloc(sizeof(double) * n * m);
+  mtranspose(m, n, A, m, AT, n);
+  mtranspose(n, n, B, n, BT, n);
+  orc_backslash(n, m, BT, AT, CT);
+  mtranspose(n, m, CT, n, C, m);
+  free(CT);
+  free(BT);
+  free(AT);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Jacobi polynomials and quadrature (2681-3135)                            */
+/* ------------------------------------------------------------------------ */
+
+/* {h_N^{(a,b)}}^{-1/2}, Szego (4.3.3) (2700) */
+static double jacobi_h_inv_sqrt(double alpha, double beta, int N)
+{
+  double lgn = -(alpha + beta + 1) * log(2) - lgamma(N + alpha + 1) -
+               lgamma(N + beta + 1) + log(2 * N + alpha + beta + 1) +
+               lgamma(N + 1) + lgamma(N + alpha + beta + 1);
+  return sqrt(exp(lgn));
+}
+
+/* orthonormal Jacobi polynomial by the three-term recurrence (2754) */
+void orc_jacobi_p(double alpha, double beta, int N, int nx, const double *x,
+                  double *P)
+{
+  for (int i = 0; i < nx; ++i)
+  {
+    double P_n_2;
+    double P_n_1 = 1;
+    double P_n_0 = ((alpha + beta + 2) / 2) * x[i] + (alpha - beta) / 2;
+    if (N == 0)
+      P[i] = P_n_1;
+    else if (N == 1)
+      P[i] = P_n_0;
+    else
+    {
+      for (int n = 2; n < N + 1; ++n)
+      {
+        double a = (2 * n + alpha + beta - 1) * (2 * n + alpha + beta) /
+                   (2 * n * (n + alpha + beta));
+        double b = (beta * beta - alpha * alpha) * (2 * n + alpha + beta - 1) /
+                   (2 * n * (n + alpha + beta) * (2 * n + alpha + beta - 2));
+        double c = (n + alpha - 1) * (n + beta - 1) * (2 * n + alpha + beta) /
+                   (n * (n + alpha + beta) * (2 * n + alpha + beta - 2));
+        P_n_2 = P_n_1;
+        P_n_1 = P_n_0;
+        P_n_0 = (a * x[i] - b) * P_n_1 - c * P_n_2;
+      }
+      P[i] = P_n_0;
+    }
+  }
+  const double h = jacobi_h_inv_sqrt(alpha, beta, N);
+  for (int i = 0; i < nx; ++i)
+    P[i] *= h;
+}
+
+/* d/dx p_N = sqrt(N(N+a+b+1)) p_{N-1}^{(a+1,b+1)} (2843) */
+void orc_grad_jacobi_p(double alpha, double beta, int N, int nx,
+                       const double *x, double *dP)
+{
+  if (N == 0)
+  {
+    for (int i = 0; i < nx; ++i)
+      dP[i] = 0.0;
+    return;
+  }
+  orc_jacobi_p(alpha + 1, beta + 1, N - 1, nx, x, dP);
+  const double scale = sqrt(N * (N + alpha + beta + 1));
+  for (int i = 0; i < nx; ++i)
+    dP[i] *= scale;
+}
+
+/*
+ * One half of the Gauss nodes of P_{N+1}: Gatteschi-Pittaluga initial guess
+ * (Hale & Townsend, SISC 35 (2013) eq. 3.19) then Newton in theta (2871).
+ * Returns 0 on success, -1 if Newton does not converge in 200 iterations
+ * (the reference aborts there).
+ */
+static int gauss_half(double alpha, double beta, int N, int half, double *x,
+                      double *w)
+{
+  const int MAX_ITERATIONS = 200;
+  const int nk = half ? (((N + 1) % 2) ? (N + 1) / 2 + 1 : (N + 1) / 2)
+                      : (N + 1) / 2;
+  if (nk == 0)
+    return 0;
+
+  const double tworho = 2 * (N + 1) + alpha + beta + 1;
+  const double pi = 4 * atan(1.0);
+  double *th0 = malloc(sizeof(double) * nk);
+  double *th1 = malloc(sizeof(double) * nk);
+  double *p0 = malloc(sizeof(double) * nk);
+  double *dp0 = malloc(sizeof(double) * nk);
+
+  for (int k = nk; k > 0; --k)
+  {
+    const int khat = half ? nk - k : k - 1;
+    const double phik = (2 * k + alpha - 0.5) * pi / tworho;
+    th1[khat] = phik + 1 / (tworho * tworho) *
+                           ((0.25 - alpha * alpha) * 1 / tan(0.5 * phik) -
+                            (0.25 - beta * beta) * tan(0.5 * phik));
+  }
+
+  int converged = 0;
+  for (int it = 0; it < MAX_ITERATIONS; ++it)
+  {
+    double *t = th0;
+    th0 = th1;
+    th1 = t;
+    for (int k = 0; k < nk; ++k)
+      x[k] = cos(th0[k]);
+    orc_jacobi_p(alpha, beta, N + 1, nk, x, p0);
+    orc_grad_jacobi_p(alpha, beta, N + 1, nk, x, dp0);
+    for (int k = 0; k < nk; ++k)
+      th1[k] = th0[k] - p0[k] / (-sin(th0[k]) * dp0[k]);
+    int diff = 0;
+    for (int k = 0; k < nk; ++k)
+      diff += !approx_eq(th0[k], th1[k], 10);
+    if (!diff)
+    {
+      converged = 1;
+      break;
+    }
+  }
+
+  if (converged)
+  {
+    for (int k = 0; k < nk; ++k)
+      x[k] = cos(th1[k]);
+    orc_grad_jacobi_p(alpha, beta, N + 1, nk, x, dp0);
+    for (int k = 0; k < nk; ++k)
+    {
+      const double s = sin(th1[k]);
+      w[k] = tworho / (s * s * dp0[k] * dp0[k]);
+    }
+  }
+  free(th0);
+  free(th1);
+  free(p0);
+  free(dp0);
+  return converged ? 0 : -1;
+}
+
+/* Gauss nodes/weights of P_{N+1}^{(a,b)} (2996) */
+int orc_jacobi_gauss(double alpha, double beta, int N, double *x, double *w)
+{
+  const int nk_floor = (N + 1) / 2;
+  if (gauss_half(alpha, beta, N, 1, x + nk_floor, w + nk_floor))
+    return -1;
+  if (gauss_half(beta, alpha, N, 0, x, w))
+    return -1;
+  for (int k = 0; k < nk_floor; ++k)
+    x[k] *= -1;
+  return 0;
+}
+
+/* Gauss-Lobatto: interior = Gauss of (a+1,b+1), weights from P_N (3013) */
+int orc_jacobi_gauss_lobatto(double alpha, double beta, int N, double *x,
+                             double *w)
+{
+  if (N < 1)
+    return -1;
+  x[0] = -1;
+  x[N] = 1;
+  if (N > 1 && orc_jacobi_gauss(alpha + 1, beta + 1, N - 2, x + 1, w + 1))
+    return -1;
+  orc_jacobi_p(alpha, beta, N, N + 1, x, w);
+  const double fac = (2 * N + alpha + beta + 1) / (N * (N + alpha + beta + 1));
+  for (int k = 0; k < N + 1; ++k)
+    w[k] = fac / (w[k] * w[k]);
+  w[0] *= (1 + beta);
+  w[N] *= (1 + alpha);
+  return 0;
+}
+
+/* V[j*nx + i] = p_j(x_i) (3042) */
+void orc_vandermonde(int N, int nx, const double *x, double *V)
+{
+  for (int j = 0; j <= N; ++j)
+    orc_jacobi_p(0, 0, j, nx, x, V + j * nx);
+}
+
+void orc_grad_vandermonde(int N, int nx, const double *x, double *V)
+{
+  for (int j = 0; j <= N; ++j)
+    orc_grad_jacobi_p(0, 0, j, nx, x, V + j * nx);
+}
+
+/* I = V_x / V (3063) */
+void orc_interpolation(int N, int nx, const double *x, const double *V,
+                       double *I)
+{
+  double *Vx = malloc(sizeof(double) * (N + 1) * nx);
+  orc_vandermonde(N, nx, x, Vx);
+  forwardslash(nx, N + 1, Vx, V, I);
+  free(Vx);
+}
+
+/* D = V_r / V (3082) */
+void orc_differentiation(int N, const double *r, const double *V, double *D)
+{
+  const int nx = N + 1;
+  double *Vx = malloc(sizeof(double) * (N + 1) * nx);
+  orc_grad_vandermonde(N, nx, r, Vx);
+  forwardslash(nx, N + 1, Vx, V, D);
+  free(Vx);
+}
+
+/* M = V^{-T} V^{-1} (3101) */
+void orc_mass(int N, const double *V, double *M)
+{
+  const int n = N + 1;
+  double *I = calloc((size_t)n * n, sizeof(double));
+  double *invV = malloc(sizeof(double) * n * n);
+  double *invVT = malloc(sizeof(double) * n * n);
+  for (int i = 0; i < n * n; ++i)
+    M[i] = 0;
+  for (int i = 0; i < n; ++i)
+    I[n * i + i] = 1;
+  orc_backslash(n, n, V, I, invV);
+  mtranspose(n, n, invV, n, invVT, n);
+  mmmult(n, n, n, invVT, n, invV, n, M, n);
+  free(I);
+  free(invV);
+  free(invVT);
+}
+
+/* per-N volume operators as stored by generic_init (7087-7155) */
+int orc_dgx_ops(int N, double *r, double *w, double *wi, double *Dr)
+{
+  const int n = N + 1;
+  double *V = malloc(sizeof(double) * n * n);
+  if (orc_jacobi_gauss_lobatto(0, 0, N, r, w))
+  {
+    free(V);
+    return -1;
+  }
+  orc_vandermonde(N, n, r, V);
+  orc_differentiation(N, r, V, Dr);
+  for (int i = 0; i < n; ++i)
+    wi[i] = 1.0 / w[i];
+  free(V);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mortar operators (5424-5735)                                             */
+/* ------------------------------------------------------------------------ */
+
+/* GLL grid, Vandermonde and exact mass of order N (5459) */
+static int grid_data(int N, double *r, double *w, double *V, double *M)
+{
+  if (orc_jacobi_gauss_lobatto(0, 0, N, r, w))
+    return -1;
+  orc_vandermonde(N, N + 1, r, V);
+  orc_mass(N, V, M);
+  return 0;
+}
+
+/* I_a2g = interpolation a -> g grid, P_g2a = V_a V_a^T I_a2g^T M_g (5471) */
+static void interp_proj_data(int Na, int Ng, const double *V_a,
+                             const double *M_g, const double *r_g,
+                             double *I_a2g, double *P_g2a)
+{
+  const int na = Na + 1, ng = Ng + 1;
+  double *MP = calloc((size_t)na * ng, sizeof(double));
+  double *VtMP = calloc((size_t)na * ng, sizeof(double));
+  orc_interpolation(Na, ng, r_g, V_a, I_a2g);
+  for (int i = 0; i < na * ng; ++i)
+    P_g2a[i] = 0;
+  mTmmult(na, ng, ng, I_a2g, ng, M_g, ng, MP, na);
+  mTmmult(na, ng, na, V_a, na, MP, na, VtMP, na);
+  mmmult(na, ng, na, V_a, na, VtMP, na, P_g2a, na);
+  free(MP);
+  free(VtMP);
+}
+
+/* children grids at (r -+ 1)/2 with half mass (5516) */
+static void hanging_data(int N, const double *r_f, const double *M,
+                         const double *V, double *P_b2f, double *P_t2f,
+                         double *I_f2b, double *I_f2t)
+{
+  const int n = N + 1;
+  double *r_t = malloc(sizeof(double) * n);
+  double *r_b = malloc(sizeof(double) * n);
+  double *Mh = malloc(sizeof(double) * n * n);
+  for (int k = 0; k < n; ++k)
+  {
+    r_t[k] = 0.5 * (r_f[k] + 1);
+    r_b[k] = 0.5 * (r_f[k] - 1);
+  }
+  for (int k = 0; k < n * n; ++k)
+    Mh[k] = 0.5 * M[k];
+  interp_proj_data(N, N, V, Mh, r_t, I_f2t, P_t2f);
+  interp_proj_data(N, N, V, Mh, r_b, I_f2b, P_b2f);
+  free(r_t);
+  free(r_b);
+  free(Mh);
+}
+
+/* compose glue operators into a -> b and scale by masses (5543) */
+static int multiply_projections(int Nb, int Na, int Ng, const double *P_g2b,
+                                const double *P_g2g, const double *P_a2g,
+                                double *P_a2b, const double *M_b,
+                                double *MP_a2b, const double *w_b,
+                                double *wiMP_a2b)
+{
+  const int nb = Nb + 1, ng = Ng + 1, na = Na + 1;
+  if (!P_a2b)
+  {
+    for (int i = 0; i < na * nb; ++i)
+      MP_a2b[i] = M_b[i];
+    for (int j = 0; j < na; ++j)
+      for (int i = 0; i < nb; ++i)
+        wiMP_a2b[i + j * nb] = M_b[i + j * nb] / w_b[i];
+    return 0;
+  }
+  double *tmp = calloc((size_t)nb * na, sizeof(double));
+  double *l_MP = calloc((size_t)nb * na, sizeof(double));
+  const double *l_P = tmp;
+  if (!P_g2b && P_g2g && P_a2g)
+    mmmult(nb, na, ng, P_g2g, nb, P_a2g, ng, tmp, nb);
+  else if (P_g2b && P_g2g && !P_a2g)
+    mmmult(nb, na, ng, P_g2b, nb, P_g2g, ng, tmp, nb);
+  else if (!P_g2b && !P_g2g && P_a2g)
+    l_P = P_a2g;
+  else if (!P_g2b && P_g2g && !P_a2g)
+    l_P = P_g2g;
+  else if (P_g2b && !P_g2g && !P_a2g)
+    l_P = P_g2b;
+  else
+  {
+    free(tmp);
+    free(l_MP);
+    return -1;
+  }
+  mmmult(nb, na, nb, M_b, nb, l_P, nb, l_MP, nb);
+  for (int i = 0; i < na * nb; ++i)
+  {
+    P_a2b[i] = l_P[i];
+    MP_a2b[i] = l_MP[i];
+  }
+  for (int j = 0; j < na; ++j)
+    for (int i = 0; i < nb; ++i)
+      wiMP_a2b[i + j * nb] = l_MP[i + j * nb] / w_b[i];
+  free(tmp);
+  free(l_MP);
+  return 0;
+}
+
+/* create_interpolators (5606), a -> b direction only */
+int orc_interpolator(int Na, int Nb, double *prj, double *mass_prj,
+                     double *wi_mass_prj, int *prj0_null)
+{
+  const int Ng = ORC_MAX(Na, Nb);
+  const int na = Na + 1, nb = Nb + 1, ng = Ng + 1;
+  const size_t blk = (size_t)na * nb;
+  int rc = 0;
+
+  double *r_a = malloc(sizeof(double) * na), *w_a = malloc(sizeof(double) * na);
+  double *V_a = malloc(sizeof(double) * na * na);
+  double *M_a = malloc(sizeof(double) * na * na);
+  double *r_b = malloc(sizeof(double) * nb), *w_b = malloc(sizeof(double) * nb);
+  double *V_b = malloc(sizeof(double) * nb * nb);
+  double *M_b = malloc(sizeof(double) * nb * nb);
+  double *r_g = malloc(sizeof(double) * ng), *w_g = malloc(sizeof(double) * ng);
+  double *V_g = malloc(sizeof(double) * ng * ng);
+  double *M_g = malloc(sizeof(double) * ng * ng);
+  rc |= grid_data(Na, r_a, w_a, V_a, M_a);
+  rc |= grid_data(Nb, r_b, w_b, V_b, M_b);
+  rc |= grid_data(Ng, r_g, w_g, V_g, M_g);
+
+  double *prj_g[5] = {NULL, NULL, NULL, NULL, NULL};
+  for (int k = 1; k < 5; ++k)
+    prj_g[k] = malloc(sizeof(double) * ng * ng);
+  hanging_data(Ng, r_g, M_g, V_g, prj_g[1], prj_g[2], prj_g[3], prj_g[4]);
+
+  double *I_a2g = NULL, *P_g2a = NULL, *I_b2g = NULL, *P_g2b = NULL;
+  if (Na < Ng && Nb == Ng)
+  {
+    I_a2g = malloc(sizeof(double) * ng * na);
+    P_g2a = malloc(sizeof(double) * ng * na);
+    interp_proj_data(Na, Ng, V_a, M_g, r_g, I_a2g, P_g2a);
+  }
+  if (Na == Ng && Nb < Ng)
+  {
+    I_b2g = malloc(sizeof(double) * ng * nb);
+    P_g2b = malloc(sizeof(double) * ng * nb);
+    interp_proj_data(Nb, Ng, V_b, M_g, r_g, I_b2g, P_g2b);
+  }
+
+  *prj0_null = (Na == Nb);
+  for (int k = 0; k < 5; ++k)
+  {
+    double *target = (k == 0 && Na == Nb) ? NULL : prj + k * blk;
+    if (!target)
+      memset(prj, 0, sizeof(double) * blk);
+    rc |= multiply_projections(Nb, Na, Ng, P_g2b, prj_g[k], I_a2g, target, M_b,
+                               mass_prj + k * blk, w_b, wi_mass_prj + k * blk);
+  }
+
+  free(I_a2g);
+  free(P_g2a);
+  free(I_b2g);
+  free(P_g2b);
+  for (int k = 1; k < 5; ++k)
+    free(prj_g[k]);
+  free(r_a), free(w_a), free(V_a), free(M_a);
+  free(r_b), free(w_b), free(V_b), free(M_b);
+  free(r_g), free(w_g), free(V_g), free(M_g);
+  return rc ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LSRK coefficient tables (8996-9090)                                      */
+/* ------------------------------------------------------------------------ */
+
+int orc_lsrk_table(int method, double *A, double *B, double *C)
+{
+  switch (method)
+  {
+  case ORC_LSRK_W33:
+    A[0] = (double)(0.0L);
+    A[1] = (double)(-5.0L / 9.0L);
+    A[2] = (double)(-153.0L / 128.0L);
+    B[0] = (double)(1.0L / 3.0L);
+    B[1] = (double)(15.0L / 16.0L);
+    B[2] = (double)(8.0L / 15.0L);
+    C[0] = (double)(0.0L);
+    C[1] = (double)(1.0L / 3.0L);
+    C[2] = (double)(3.0L / 4.0L);
+    C[3] = (double)(1.0L);
+    return 3;
+  case ORC_LSRK_HEUN:
+    A[0] = (double)(0.0L);
+    A[1] = -(double)(1.0L);
+    B[0] = (double)(1.0L);
+    B[1] = (double)(1.0L / 2.0L);
+    C[0] = (double)(0.0L);
+    C[1] = (double)(1.0L);
+    C[2] = (double)(1.0L);
+    return 2;
+  case ORC_LSRK_FE:
+    A[0] = (double)(0.0L);
+    B[0] = (double)(1.0L);
+    C[0] = (double)(0.0L);
+    C[1] = (double)(1.0L);
+    return 1;
+  case ORC_LSRK_KC54:
+  default: /* the reference warns and falls back to KC54 (9014) */
+    A[0] = (double)(0.0L);
+    A[1] = (double)(-567301805773.0L / 1357537059087.0L);
+    A[2] = (double)(-2404267990393.0L / 2016746695238.0L);
+    A[3] = (double)(-3550918686646.0L / 2091501179385.0L);
+    A[4] = (double)(-1275806237668.0L / 842570457699.0L);
+    B[0] = (double)(1432997174477.0L / 9575080441755.0L);
+    B[1] = (double)(5161836677717.0L / 13612068292357.0L);
+    B[2] = (double)(1720146321549.0L / 2090206949498.0L);
+    B[3] = (double)(3134564353537.0L / 4481467310338.0L);
+    B[4] = (double)(2277821191437.0L / 14882151754819.0L);
+    C[0] = (double)(0.0L);
+    C[1] = (double)(1432997174477.0L / 9575080441755.0L);
+    C[2] = (double)(2526269341429.0L / 6820363962896.0L);
+    C[3] = (double)(2006345519317.0L / 3224310063776.0L);
+    C[4] = (double)(2802321613138.0L / 2924317926251.0L);
+    C[5] = (double)(1.0L);
+    return 5;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* integer maps (3630-3732, 6599-6877, 7430-7463)                           */
+/* ------------------------------------------------------------------------ */
+
+static const int8_t FToF_code[6][6] = {
+    {0, 1, 1, 0, 0, 1}, {2, 0, 0, 1, 1, 0}, {2, 0, 0, 1, 1, 0},
+    {0, 2, 2, 0, 0, 1}, {0, 2, 2, 0, 0, 1}, {2, 0, 0, 2, 2, 0}};
+static const int8_t code_to_perm[3][4] = {
+    {1, 2, 5, 6}, {0, 3, 4, 7}, {0, 4, 3, 7}};
+static const int8_t perm_to_order[8][4] = {
+    {0, 1, 2, 3}, {0, 2, 1, 3}, {1, 0, 3, 2}, {1, 3, 0, 2},
+    {2, 0, 3, 1}, {2, 3, 0, 1}, {3, 1, 2, 0}, {3, 2, 1, 0}};
+static const int8_t perm_to_order_inv[8][4] = {
+    {0, 1, 2, 3}, {0, 2, 1, 3}, {1, 0, 3, 2}, {2, 0, 3, 1},
+    {1, 3, 0, 2}, {2, 3, 0, 1}, {3, 1, 2, 0}, {3, 2, 1, 0}};
+
+int orc_p8est_orientation(int f, int nf, int o)
+{
+  return code_to_perm[FToF_code[f][nf]][o];
+}
+int orc_p8est_perm_to_order(int bo, int h) { return perm_to_order[bo][h]; }
+int orc_p8est_perm_to_order_inv(int bo, int h)
+{
+  return perm_to_order_inv[bo][h];
+}
+
+/* face masks of gmask_set (6599-6781); faces -x,+x,-y,+y(,-z,+z) */
+void orc_gmask_faces(int dim, int N, int32_t *gmask)
+{
+  const int n1 = N + 1;
+  if (dim == 2)
+  {
+    for (int i = 0; i < n1; ++i)
+      gmask[0 * n1 + i] = i * n1;
+    for (int i = 0; i < n1; ++i)
+      gmask[1 * n1 + i] = (i + 1) * n1 - 1;
+    for (int i = 0; i < n1; ++i)
+      gmask[2 * n1 + i] = i;
+    for (int i = 0; i < n1; ++i)
+      gmask[3 * n1 + i] = n1 * N + i;
+    return;
+  }
+  const int nfp = n1 * n1;
+  int n;
+  for (int f = 0; f < 6; ++f)
+  {
+    n = 0;
+    const int fixed = (f % 2) ? N : 0;
+    if (f / 2 == 0)
+      for (int k = 0; k < n1; ++k)
+        for (int j = 0; j < n1; ++j)
+          gmask[f * nfp + n++] = fixed + j * n1 + k * n1 * n1;
+    else if (f / 2 == 1)
+      for (int k = 0; k < n1; ++k)
+        for (int i = 0; i < n1; ++i)
+          gmask[f * nfp + n++] = i + fixed * n1 + k * n1 * n1;
+    else
+      for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n1; ++i)
+          gmask[f * nfp + n++] = i + j * n1 + fixed * n1 * n1;
+  }
+}
+
+/* buildmaps (6783-6877) */
+void orc_buildmaps(int dim, int N, int32_t K, const int32_t *EToE,
+                   const int8_t *EToF, int32_t *vmapM, int32_t *vmapP)
+{
+  const int n1 = N + 1;
+  const int Nfaces = 2 * dim;
+  const int Nfp = dim == 2 ? n1 : n1 * n1;
+  const int Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  int32_t *gmask = malloc(sizeof(int32_t) * Nfaces * Nfp);
+  orc_gmask_faces(dim, N, gmask);
+
+  for (int32_t k1 = 0, sk = 0; k1 < K; ++k1)
+  {
+    for (int f1 = 0; f1 < Nfaces; ++f1)
+    {
+      const int32_t k2 = EToE[Nfaces * k1 + f1];
+      const int f2 = EToF[Nfaces * k1 + f1] % Nfaces;
+      const int o = EToF[Nfaces * k1 + f1] / Nfaces;
+      for (int n = 0; n < Nfp; ++n)
+        vmapM[sk + n] = Np * k1 + gmask[f1 * Nfp + n];
+      if (dim == 2)
+      {
+        for (int n = 0; n < Nfp; ++n)
+          vmapP[sk + n] =
+              Np * k2 + gmask[f2 * Nfp + (o ? Nfp - 1 - n : n)];
+      }
+      else
+      {
+        const int nr = FToF_code[f1][f2];
+        const int ns = (k1 == k2 && f1 == f2) ? 0 : code_to_perm[nr][o];
+        for (int j = 0, n = 0; j < n1; ++j)
+          for (int i = 0; i < n1; ++i, ++n)
+          {
+            const int ir = n1 - (i + 1), jr = n1 - (j + 1);
+            int oidx;
+            switch (ns)
+            {
+            case 0: oidx = i + j * n1; break;
+            case 1: oidx = j + i * n1; break;
+            case 2: oidx = ir + j * n1; break;
+            case 3: oidx = jr + i * n1; break;
+            case 4: oidx = j + ir * n1; break;
+            case 5: oidx = i + jr * n1; break;
+            case 6: oidx = jr + ir * n1; break;
+            default: oidx = ir + jr * n1; break;
+            }
+            vmapP[sk + n] = Np * k2 + gmask[f2 * Nfp + oidx];
+          }
+      }
+      sk += Nfp;
+    }
+  }
+  free(gmask);
+}
+
+/* glue mapOp (7430-7463) */
+void orc_glue_mapOp(int dim_glue, int N, int32_t *mapOp)
+{
+  const int n1 = N + 1;
+  if (dim_glue == 1)
+  {
+    for (int n = 0; n < n1; ++n)
+    {
+      mapOp[0 * n1 + n] = n;
+      mapOp[1 * n1 + n] = n1 - (n + 1);
+    }
+    return;
+  }
+  const int nfp = n1 * n1;
+  for (int j = 0; j < n1; ++j)
+    for (int i = 0; i < n1; ++i)
+    {
+      const int ir = n1 - (i + 1), jr = n1 - (j + 1), n = i + j * n1;
+      mapOp[0 * nfp + n] = i + j * n1;
+      mapOp[1 * nfp + n] = j + i * n1;
+      mapOp[2 * nfp + n] = ir + j * n1;
+      mapOp[3 * nfp + n] = j + ir * n1;
+      mapOp[4 * nfp + n] = jr + i * n1;
+      mapOp[5 * nfp + n] = i + jr * n1;
+      mapOp[6 * nfp + n] = jr + ir * n1;
+      mapOp[7 * nfp + n] = ir + jr * n1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PCG32 (8328-8387), O'Neill 2014                                          */
+/* ------------------------------------------------------------------------ */
+
+uint32_t orc_pcg32_next(orc_pcg32_t *rng)
+{
+  const uint64_t old = rng->state;
+  rng->state = old * 6364136223846793005ULL + (rng->inc | 1);
+  const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+  const uint32_t rot = (uint32_t)(old >> 59u);
+  return (xs >> rot) | (xs << ((-rot) & 31));
+}
+
+void orc_pcg32_seed(orc_pcg32_t *rng, uint64_t initstate, uint64_t initseq)
+{
+  rng->state = 0U;
+  rng->inc = (initseq << 1u) | 1u;
+  orc_pcg32_next(rng);
+  rng->state += initstate;
+  orc_pcg32_next(rng);
+}
+
+uint32_t orc_pcg32_bounded(orc_pcg32_t *rng, uint32_t bound)
+{
+  const uint32_t threshold = -bound % bound;
+  for (;;)
+  {
+    const uint32_t r = orc_pcg32_next(rng);
+    if (r >= threshold)
+      return r % bound;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* brick connectivity (stands in for p4est_mesh_new on a brick forest)      */
+/* ------------------------------------------------------------------------ */
+
+void orc_brick(int dim, const int *n, const int *periodic, int32_t *EToE,
+               int8_t *EToF)
+{
+  const int nz = dim == 3 ? n[2] : 1;
+  const int Nfaces = 2 * dim;
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < n[1]; ++iy)
+      for (int ix = 0; ix < n[0]; ++ix)
+      {
+        const int idx[3] = {ix, iy, iz};
+        const int32_t e = ix + n[0] * (iy + n[1] * iz);
+        for (int f = 0; f < Nfaces; ++f)
+        {
+          const int d = f / 2, s = (f % 2) ? 1 : -1;
+          int j[3] = {ix, iy, iz};
+          j[d] = idx[d] + s;
+          int boundary = 0;
+          if (j[d] < 0 || j[d] >= n[d])
+          {
+            if (periodic[d])
+              j[d] = (j[d] + n[d]) % n[d];
+            else
+              boundary = 1;
+          }
+          if (boundary)
+          {
+            EToE[Nfaces * e + f] = e;
+            EToF[Nfaces * e + f] = (int8_t)f;
+          }
+          else
+          {
+            EToE[Nfaces * e + f] = j[0] + n[0] * (j[1] + n[1] * j[2]);
+            EToF[Nfaces * e + f] = (int8_t)(f ^ 1);
+          }
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* right-hand side                                                          */
+/* ------------------------------------------------------------------------ */
+
+int orc_nfields(int physics) { return physics == ORC_ACOUSTIC2D ? 3 : 9; }
+
+double orc_ricker(double f0, double t0, double t)
+{
+  const double pi = 4 * atan(1.0);
+  const double a = (pi * f0 * (t - t0)) * (pi * f0 * (t - t0));
+  return (1 - 2 * a) * exp(-a);
+}
+
+/* spatial pattern of manufactured field f at a point (sin/cos products) */
+static double mms_pattern(int f, const double *S, const double *Cc)
+{
+  switch (f)
+  {
+  case 0: return S[0] * Cc[1] * Cc[2];
+  case 1: return Cc[0] * S[1] * Cc[2];
+  case 2: return Cc[0] * Cc[1] * S[2];
+  case 3:
+  case 4:
+  case 5: return Cc[0] * Cc[1] * Cc[2];
+  case 6: return S[0] * S[1] * Cc[2];
+  case 7: return S[0] * Cc[1] * S[2];
+  default: return Cc[0] * S[1] * S[2];
+  }
+}
+
+/*
+ * Manufactured elastic solution with amplitudes A[9]:
+ *   v_f = A_f P_f(x) cos(w t), sigma_f = A_f P_f(x) sin(w t)
+ * and F = dq/dt - L q, which is again F_f P_f(x) times sin (velocities) or
+ * cos (stresses).
+ */
+void orc_mms_forcing_coef(const double *A, double k, double omega, double rho,
+                          double lam, double mu, double *F)
+{
+  const double sv = A[0] + A[1] + A[2];
+  F[0] = -omega * A[0] - k * (-A[3] + A[6] + A[7]) / rho;
+  F[1] = -omega * A[1] - k * (A[6] - A[4] + A[8]) / rho;
+  F[2] = -omega * A[2] - k * (A[7] + A[8] - A[5]) / rho;
+  F[3] = omega * A[3] - k * (lam * sv + 2 * mu * A[0]);
+  F[4] = omega * A[4] - k * (lam * sv + 2 * mu * A[1]);
+  F[5] = omega * A[5] - k * (lam * sv + 2 * mu * A[2]);
+  F[6] = omega * A[6] + mu * k * (A[0] + A[1]);
+  F[7] = omega * A[7] + mu * k * (A[0] + A[2]);
+  F[8] = omega * A[8] + mu * k * (A[1] + A[2]);
+}
+
+static void node_coords(const orc_problem_t *p, int dim, int32_t e, int n,
+                        double *x)
+{
+  const int n1 = p->N + 1;
+  int idx[3] = {n % n1, (n / n1) % n1, n / (n1 * n1)};
+  for (int d = 0; d < dim; ++d)
+    x[d] = p->x0[dim * e + d] + 0.5 * (p->r[idx[d]] + 1) * p->h[dim * e + d];
+}
+
+void orc_mms_exact(const orc_problem_t *p, const double *A, double t,
+                   double *q)
+{
+  const int Np = (p->N + 1) * (p->N + 1) * (p->N + 1);
+  const size_t KNp = (size_t)p->K * Np;
+  for (int32_t e = 0; e < p->K; ++e)
+    for (int n = 0; n < Np; ++n)
+    {
+      double x[3], S[3], Cc[3];
+      node_coords(p, 3, e, n, x);
+      for (int d = 0; d < 3; ++d)
+      {
+        S[d] = sin(p->mms_k * x[d]);
+        Cc[d] = cos(p->mms_k * x[d]);
+      }
+      for (int f = 0; f < 9; ++f)
+        q[f * KNp + (size_t)e * Np + n] =
+            A[f] * mms_pattern(f, S, Cc) *
+            (f < 3 ? cos(p->mms_omega * t) : sin(p->mms_omega * t));
+    }
+}
+
+/* derivative of u along axis a at node (i,j,k):  sum_m D(idx_a, m) u(..m..) */
+static double dr_axis(const double *Dr, const double *u, int n1, int a, int i,
+                      int j, int k)
+{
+  double s = 0;
+  const int idx = a == 0 ? i : (a == 1 ? j : k);
+  for (int m = 0; m < n1; ++m)
+  {
+    const int node = a == 0 ? m + n1 * (j + n1 * k)
+                            : (a == 1 ? i + n1 * (m + n1 * k)
+                                      : i + n1 * (j + n1 * m));
+    s += Dr[idx + m * n1] * u[node];
+  }
+  return s;
+}
+
+/*
+ * Upwind fluctuations for normal n between the minus and plus states, given
+ * as velocity v[3] and traction T[3] = sigma n.  Returns dv = v* - v- and
+ * dT = T* - T- (Wilcox et al. 2010; exact Riemann solution, P and S parts).
+ */
+static void riemann(const double *n, const double *vm, const double *Tm,
+                    double Zpm, double Zsm, const double *vp, const double *Tp,
+                    double Zpp, double Zsp, double *dv, double *dT)
+{
+  const double vnm = vm[0] * n[0] + vm[1] * n[1] + vm[2] * n[2];
+  const double vnp = vp[0] * n[0] + vp[1] * n[1] + vp[2] * n[2];
+  const double Tnm = Tm[0] * n[0] + Tm[1] * n[1] + Tm[2] * n[2];
+  const double Tnp = Tp[0] * n[0] + Tp[1] * n[1] + Tp[2] * n[2];
+  const double dvn = (Zpp * (vnp - vnm) + (Tnp - Tnm)) / (Zpm + Zpp);
+  for (int c = 0; c < 3; ++c)
+  {
+    const double vtm = vm[c] - vnm * n[c], vtp = vp[c] - vnp * n[c];
+    const double Ttm = Tm[c] - Tnm * n[c], Ttp = Tp[c] - Tnp * n[c];
+    const double zs = Zsm + Zsp;
+    const double dvt = zs > 0 ? (Zsp * (vtp - vtm) + (Ttp - Ttm)) / zs : 0.0;
+    dv[c] = dvn * n[c] + dvt;
+    dT[c] = Zpm * dvn * n[c] + Zsm * dvt;
+  }
+}
+
+static void rhs_elastic3d(const orc_problem_t *p, double t, const double *q,
+                          double *rhs)
+{
+  const int N = p->N, n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
+  const size_t KNp = (size_t)p->K * Np;
+  const double wend = p->w[0];
+  double g = 0;
+  if (p->src_elem >= 0)
+    g = orc_ricker(p->src_f0, p->src_t0, t);
+  const double tv = p->mms ? sin(p->mms_omega * t) : 0;
+  const double ts = p->mms ? cos(p->mms_omega * t) : 0;
+
+#pragma omp parallel for schedule(static)
+  for (int32_t e = 0; e < p->K; ++e)
+  {
+    const double *u[9];
+    double *r[9];
+    for (int f = 0; f < 9; ++f)
+    {
+      u[f] = q + f * KNp + (size_t)e * Np;
+      r[f] = rhs + f * KNp + (size_t)e * Np;
+    }
+    const double rho = p->rho[e], lam = p->lam[e], mu = p->mu[e];
+    const double gx = 2 / p->h[3 * e + 0], gy = 2 / p->h[3 * e + 1],
+                 gz = 2 / p->h[3 * e + 2];
+    for (int k = 0; k < n1; ++k)
+      for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n1; ++i)
+        {
+          const int n = i + n1 * (j + n1 * k);
+          double dv[3][3]; /* dv[c][a] = d v_c / d x_a */
+          for (int c = 0; c < 3; ++c)
+          {
+            dv[c][0] = gx * dr_axis(p->Dr, u[c], n1, 0, i, j, k);
+            dv[c][1] = gy * dr_axis(p->Dr, u[c], n1, 1, i, j, k);
+            dv[c][2] = gz * dr_axis(p->Dr, u[c], n1, 2, i, j, k);
+          }
+          const double ds11 = gx * dr_axis(p->Dr, u[3], n1, 0, i, j, k);
+          const double ds22 = gy * dr_axis(p->Dr, u[4], n1, 1, i, j, k);
+          const double ds33 = gz * dr_axis(p->Dr, u[5], n1, 2, i, j, k);
+          const double ds12x = gx * dr_axis(p->Dr, u[6], n1, 0, i, j, k);
+          const double ds12y = gy * dr_axis(p->Dr, u[6], n1, 1, i, j, k);
+          const double ds13x = gx * dr_axis(p->Dr, u[7], n1, 0, i, j, k);
+          const double ds13z = gz * dr_axis(p->Dr, u[7], n1, 2, i, j, k);
+          const double ds23y = gy * dr_axis(p->Dr, u[8], n1, 1, i, j, k);
+          const double ds23z = gz * dr_axis(p->Dr, u[8], n1, 2, i, j, k);
+          const double div = dv[0][0] + dv[1][1] + dv[2][2];
+          r[0][n] = (ds11 + ds12y + ds13z) / rho;
+          r[1][n] = (ds12x + ds22 + ds23z) / rho;
+          r[2][n] = (ds13x + ds23y + ds33) / rho;
+          r[3][n] = lam * div + 2 * mu * dv[0][0];
+          r[4][n] = lam * div + 2 * mu * dv[1][1];
+          r[5][n] = lam * div + 2 * mu * dv[2][2];
+          r[6][n] = mu * (dv[0][1] + dv[1][0]);
+          r[7][n] = mu * (dv[0][2] + dv[2][0]);
+          r[8][n] = mu * (dv[1][2] + dv[2][1]);
+        }
+
+    const double Zpm = sqrt(rho * (lam + 2 * mu)), Zsm = sqrt(rho * mu);
+    for (int f = 0; f < 6; ++f)
+    {
+      const int a = f / 2;
+      double nrm[3] = {0, 0, 0};
+      nrm[a] = (f % 2) ? 1.0 : -1.0;
+      const double lift = (2 / p->h[3 * e + a]) / wend;
+      for (int fn = 0; fn < Nfp; ++fn)
+      {
+        const size_t s = ((size_t)e * 6 + f) * Nfp + fn;
+        const int32_t idM = p->vmapM[s], idP = p->vmapP[s];
+        const int32_t ep = idP / Np;
+        const int nm = idM - e * Np;
+        const double rhop = p->rho[ep], lamp = p->lam[ep], mup = p->mu[ep];
+        const double Zpp = sqrt(rhop * (lamp + 2 * mup)), Zsp = sqrt(rhop * mup);
+        double vm[3], vp[3], Tm[3], Tp[3], S[2][3][3];
+        for (int side = 0; side < 2; ++side)
+        {
+          const size_t id = side ? (size_t)idP : (size_t)idM;
+          double(*Sg)[3] = S[side];
+          Sg[0][0] = q[3 * KNp + id];
+          Sg[1][1] = q[4 * KNp + id];
+          Sg[2][2] = q[5 * KNp + id];
+          Sg[0][1] = Sg[1][0] = q[6 * KNp + id];
+          Sg[0][2] = Sg[2][0] = q[7 * KNp + id];
+          Sg[1][2] = Sg[2][1] = q[8 * KNp + id];
+          double *v = side ? vp : vm, *T = side ? Tp : Tm;
+          for (int c = 0; c < 3; ++c)
+          {
+            v[c] = q[c * KNp + id];
+            T[c] = Sg[c][0] * nrm[0] + Sg[c][1] * nrm[1] + Sg[c][2] * nrm[2];
+          }
+        }
+        double dvv[3], dT[3];
+        riemann(nrm, vm, Tm, Zpm, Zsm, vp, Tp, Zpp, Zsp, dvv, dT);
+        const double dvn = dvv[0] * nrm[0] + dvv[1] * nrm[1] + dvv[2] * nrm[2];
+        r[0][nm] += lift * dT[0] / rho;
+        r[1][nm] += lift * dT[1] / rho;
+        r[2][nm] += lift * dT[2] / rho;
+        r[3][nm] += lift * (lam * dvn + 2 * mu * dvv[0] * nrm[0]);
+        r[4][nm] += lift * (lam * dvn + 2 * mu * dvv[1] * nrm[1]);
+        r[5][nm] += lift * (lam * dvn + 2 * mu * dvv[2] * nrm[2]);
+        r[6][nm] += lift * mu * (dvv[0] * nrm[1] + dvv[1] * nrm[0]);
+        r[7][nm] += lift * mu * (dvv[0] * nrm[2] + dvv[2] * nrm[0]);
+        r[8][nm] += lift * mu * (dvv[1] * nrm[2] + dvv[2] * nrm[1]);
+      }
+    }
+
+    if (e == p->src_elem)
+      for (int f = 0; f < 9; ++f)
+        if (p->src_amp[f] != 0)
+          for (int n = 0; n < Np; ++n)
+            r[f][n] += p->src_wt[n] * p->src_amp[f] * g;
+
+    if (p->mms)
+      for (int n = 0; n < Np; ++n)
+      {
+        double x[3], S[3], Cc[3];
+        node_coords(p, 3, e, n, x);
+        for (int d = 0; d < 3; ++d)
+        {
+          S[d] = sin(p->mms_k * x[d]);
+          Cc[d] = cos(p->mms_k * x[d]);
+        }
+        for (int f = 0; f < 9; ++f)
+          r[f][n] += p->mms_F[f] * mms_pattern(f, S, Cc) * (f < 3 ? tv : ts);
+      }
+  }
+}
+
+static void rhs_acoustic2d(const orc_problem_t *p, double t, const double *q,
+                           double *rhs)
+{
+  const int N = p->N, n1 = N + 1, Np = n1 * n1, Nfp = n1;
+  const size_t KNp = (size_t)p->K * Np;
+  const double wend = p->w[0];
+  double g = 0;
+  if (p->src_elem >= 0)
+    g = orc_ricker(p->src_f0, p->src_t0, t);
+
+#pragma omp parallel for schedule(static)
+  for (int32_t e = 0; e < p->K; ++e)
+  {
+    const double *P = q + 0 * KNp + (size_t)e * Np;
+    const double *U = q + 1 * KNp + (size_t)e * Np;
+    const double *V = q + 2 * KNp + (size_t)e * Np;
+    double *rP = rhs + 0 * KNp + (size_t)e * Np;
+    double *rU = rhs + 1 * KNp + (size_t)e * Np;
+    double *rV = rhs + 2 * KNp + (size_t)e * Np;
+    const double rho = p->rho[e], K = p->lam[e];
+    const double gx = 2 / p->h[2 * e + 0], gy = 2 / p->h[2 * e + 1];
+    for (int j = 0; j < n1; ++j)
+      for (int i = 0; i < n1; ++i)
+      {
+        const int n = i + n1 * j;
+        const double px = gx * dr_axis(p->Dr, P, n1, 0, i, j, 0);
+        const double py = gy * dr_axis(p->Dr, P, n1, 1, i, j, 0);
+        const double ux = gx * dr_axis(p->Dr, U, n1, 0, i, j, 0);
+        const double vy = gy * dr_axis(p->Dr, V, n1, 1, i, j, 0);
+        rP[n] = -K * (ux + vy);
+        rU[n] = -px / rho;
+        rV[n] = -py / rho;
+      }
+    const double Zm = sqrt(rho * K);
+    for (int f = 0; f < 4; ++f)
+    {
+      const int a = f / 2;
+      double nrm[2] = {0, 0};
+      nrm[a] = (f % 2) ? 1.0 : -1.0;
+      const double lift = (2 / p->h[2 * e + a]) / wend;
+      for (int fn = 0; fn < Nfp; ++fn)
+      {
+        const size_t s = ((size_t)e * 4 + f) * Nfp + fn;
+        const int32_t idM = p->vmapM[s], idP = p->vmapP[s];
+        const int32_t ep = idP / Np;
+        const int nm = idM - e * Np;
+        const double Zp = sqrt(p->rho[ep] * p->lam[ep]);
+        const double vnm = q[1 * KNp + idM] * nrm[0] + q[2 * KNp + idM] * nrm[1];
+        const double vnp = q[1 * KNp + idP] * nrm[0] + q[2 * KNp + idP] * nrm[1];
+        /* traction normal component T_n = -p */
+        const double dvn =
+            (Zp * (vnp - vnm) + (-q[idP] + q[idM])) / (Zm + Zp);
+        rU[nm] += lift * Zm * dvn * nrm[0] / rho;
+        rV[nm] += lift * Zm * dvn * nrm[1] / rho;
+        rP[nm] += -lift * K * dvn;
+      }
+    }
+    if (e == p->src_elem)
+      for (int f = 0; f < 3; ++f)
+        if (p->src_amp[f] != 0)
+          for (int n = 0; n < Np; ++n)
+            rhs[f * KNp + (size_t)e * Np + n] +=
+                p->src_wt[n] * p->src_amp[f] * g;
+  }
+}
+
+void orc_rhs(const orc_problem_t *p, double t, const double *q, double *rhs)
+{
+  if (p->physics == ORC_ACOUSTIC2D)
+    rhs_acoustic2d(p, t, q, rhs);
+  else
+    rhs_elastic3d(p, t, q, rhs);
+}
+
+void orc_lsrk_run(const orc_problem_t *p, int method, double t0, double dt,
+                  int nsteps, double *q, double *dq, double *work)
+{
+  double A[6], B[6], C[6];
+  const int ns = orc_lsrk_table(method, A, B, C);
+  const int dim = p->physics == ORC_ACOUSTIC2D ? 2 : 3;
+  size_t Np = 1;
+  for (int d = 0; d < dim; ++d)
+    Np *= (size_t)(p->N + 1);
+  const size_t n = (size_t)orc_nfields(p->physics) * p->K * Np;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; ++i)
+    dq[i] = 0;
+  double t = t0;
+  for (int step = 0; step < nsteps; ++step)
+  {
+    for (int s = 0; s < ns; ++s)
+    {
+      orc_rhs(p, t + C[s] * dt, q, work);
+      const double a = A[s], b = B[s];
+#pragma omp parallel for schedule(static)
+      for (size_t i = 0; i < n; ++i)
+      {
+        dq[i] = a * dq[i] + dt * work[i];
+        q[i] = q[i] + b * dq[i];
+      }
+    }
+    t += dt;
+  }
+}
+
+double orc_energy(const orc_problem_t *p, const double *q)
+{
+  const int dim = p->physics == ORC_ACOUSTIC2D ? 2 : 3;
+  const int n1 = p->N + 1;
+  const int Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  const size_t KNp = (size_t)p->K * Np;
+  double sum = 0, comp = 0;
+  for (int32_t e = 0; e < p->K; ++e)
+  {
+    double J = 1;
+    for (int d = 0; d < dim; ++d)
+      J *= 0.5 * p->h[dim * e + d];
+    const double rho = p->rho[e], lam = p->lam[e], mu = p->mu ? p->mu[e] : 0;
+    for (int n = 0; n < Np; ++n)
+    {
+      const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
+      double wq = p->w[i] * p->w[j];
+      if (dim == 3)
+        wq *= p->w[k];
+      const size_t id = (size_t)e * Np + n;
+      double dens;
+      if (dim == 2)
+      {
+        const double P = q[id], U = q[KNp + id], V = q[2 * KNp + id];
+        dens = rho * (U * U + V * V) + P * P / lam;
+      }
+      else
+      {
+        const double v1 = q[id], v2 = q[KNp + id], v3 = q[2 * KNp + id];
+        const double s11 = q[3 * KNp + id], s22 = q[4 * KNp + id],
+                     s33 = q[5 * KNp + id], s12 = q[6 * KNp + id],
+                     s13 = q[7 * KNp + id], s23 = q[8 * KNp + id];
+        const double tr = s11 + s22 + s33;
+        const double ss = s11 * s11 + s22 * s22 + s33 * s33 +
+                          2 * (s12 * s12 + s13 * s13 + s23 * s23);
+        dens = rho * (v1 * v1 + v2 * v2 + v3 * v3) +
+               (ss - lam / (3 * lam + 2 * mu) * tr * tr) / (2 * mu);
+      }
+      const double y = 0.5 * J * wq * dens - comp;
+      const double tt = sum + y;
+      comp = (tt - sum) - y;
+      sum = tt;
+    }
+  }
+  return sum;
+}

@@ -1,0 +1,174 @@
+"""Device parity against the CPU FP64 oracle (oracle/oracle.c), through the
+C ABI.  Tolerances: FP64 paths 1e-12 relative (max norm over all fields),
+FP32 arithmetic 2e-5 relative; results that must not depend on launch shape
+(batching, graphs, XCD remapping) are compared bit for bit."""
+import numpy as np
+import pytest
+
+import bfam_amd as bx
+import bfam_amd.workloads as W
+from util import oracle_problem, random_acoustic, random_elastic, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+TOL = {bx.FP64: 1e-12, bx.FP64_PLAIN: 1e-12, bx.FP32_RHS: 2e-5, bx.FP32: 2e-5}
+PRECS = [bx.FP64, bx.FP64_PLAIN, bx.FP32_RHS, bx.FP32]
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = bx.Device(0)
+    yield d
+    d.close()
+
+
+@pytest.mark.parametrize("N", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("prec", PRECS)
+def test_elastic_rhs(dev, N, prec):
+    print(f"elastic rhs N={N} prec={prec}", flush=True)
+    cfg, q0 = random_elastic((3, 2, 4), N, seed=N)
+    sub = W.upload(dev, cfg, prec)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    got = sub.rhs(0.25)
+    sub.close()
+    want = oracle_problem(cfg).rhs(q0, 0.25)
+    assert rel_max(got, want) < TOL[prec]
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("prec", PRECS)
+def test_acoustic_rhs(dev, N, prec):
+    print(f"acoustic rhs N={N} prec={prec}", flush=True)
+    cfg, q0 = random_acoustic((5, 3), N, seed=N)
+    sub = W.upload(dev, cfg, prec)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    got = sub.rhs(0.05)
+    sub.close()
+    want = oracle_problem(cfg).rhs(q0, 0.05)
+    assert rel_max(got, want) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_elastic_lsrk_steps(dev, prec):
+    print(f"elastic lsrk 4 steps prec={prec}", flush=True)
+    cfg, q0 = random_elastic((4, 4, 3), 5, seed=3)
+    sub = W.upload(dev, cfg, prec)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    sub.step(4)
+    t, st = sub.time()
+    got = sub.download()
+    sub.close()
+    want = oracle_problem(cfg).run(q0, cfg["dt"], 4)
+    assert st == 20 and abs(t - 4 * cfg["dt"]) < 1e-15
+    assert rel_max(got, want) < TOL[prec]
+
+
+def test_acoustic_plane_wave(dev):
+    """Config acoustic2d_plane_wave: 100 KC54 steps, FP64, against the oracle
+    and against the exact plane wave."""
+    print("acoustic plane wave 16x16 N=4", flush=True)
+    cfg = W.acoustic2d_plane_wave()
+    sub = W.upload(dev, cfg, bx.FP64)
+    sub.set_fields(cfg["q0"], 0.0, cfg["dt"])
+    sub.step(cfg["nsteps"])
+    got = sub.download()
+    E = sub.energy()
+    sub.close()
+    P = oracle_problem(cfg)
+    want = P.run(cfg["q0"], cfg["dt"], cfg["nsteps"])
+    assert rel_max(got, want) < 1e-12
+    ex = W.acoustic2d_exact(cfg, cfg["dt"] * cfg["nsteps"])
+    assert rel_l2(got, ex) < 2e-6
+    assert abs(E - P.energy(want)) < 1e-12 * P.energy(want)
+
+
+def test_energy_and_l2(dev):
+    cfg, q0 = random_elastic((3, 3, 3), 4, seed=9)
+    a = W.upload(dev, cfg, bx.FP64)
+    b = W.upload(dev, cfg, bx.FP32)
+    a.set_fields(q0)
+    b.set_fields(q0)
+    P = oracle_problem(cfg)
+    assert abs(a.energy() - P.energy(q0)) < 1e-12 * P.energy(q0)
+    d2, n2 = b.l2_diff(a)
+    assert n2 > 0 and np.sqrt(d2 / n2) < 1e-6
+    a.close()
+    b.close()
+
+
+def test_batch_equals_single(dev):
+    """Same-order batch and mixed-order batch reproduce single launches bit
+    for bit (each element is computed by the same code)."""
+    print("batch vs single", flush=True)
+    subs, ref = [], []
+    for i, N in enumerate([3, 5, 7, 5]):
+        cfg, q0 = random_elastic((2, 3, 2), N, seed=20 + i)
+        s = W.upload(dev, cfg, bx.FP32_RHS)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(3)
+        ref.append(s.download())
+        s.set_fields(q0, 0.0, cfg["dt"])
+        subs.append(s)
+    b = bx.Batch(dev, subs)
+    b.step(3)
+    for s, r in zip(subs, ref):
+        assert np.array_equal(s.download(), r)
+    b.close()
+    for s in subs:
+        s.close()
+
+
+def test_graph_replay_matches_eager(dev):
+    print("graph replay", flush=True)
+    cfg, q0 = random_elastic((4, 2, 3), 6, seed=31)
+    a = W.upload(dev, cfg, bx.FP64)
+    a.set_fields(q0, 0.0, cfg["dt"])
+    a.step(3)
+    eager = a.download()
+    a.set_fields(q0, 0.0, cfg["dt"])
+    b = bx.Batch(dev, [a])
+    b.capture(1)
+    outs = []
+    for _ in range(3):
+        b.launch_graph()
+        dev.sync()
+        outs.append(a.download())
+    assert np.array_equal(outs[-1], eager)
+    assert not np.array_equal(outs[0], outs[1])
+    b.close()
+    a.close()
+
+
+def test_xcd_remap_invariant(dev):
+    cfg, q0 = random_elastic((5, 3, 3), 7, seed=41)
+    res = []
+    for remap in (1, 0):
+        dev.set_tuning(remap)
+        s = W.upload(dev, cfg, bx.FP32)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(2)
+        res.append(s.download())
+        s.close()
+    dev.set_tuning(1)
+    assert np.array_equal(res[0], res[1])
+
+
+def test_mms_forcing(dev):
+    """Manufactured solution: device RHS with forcing equals the oracle's, and
+    a few steps track the exact solution."""
+    print("mms 4^3 N=5", flush=True)
+    cfg = W.elastic3d_mms(n=4, N=5)
+    m = cfg["mesh"]
+    q0 = W.mms_exact(m, cfg["A"], cfg["k"], cfg["omega"], 0.0)
+    sub = W.upload(dev, cfg, bx.FP64)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    got = sub.rhs(0.1)
+    P = oracle_problem(cfg)
+    assert rel_max(got, P.rhs(q0, 0.1)) < 1e-12
+    sub.step(10)
+    d2, n2 = sub.mms_error(cfg["A"])
+    q = sub.download()
+    sub.close()
+    want = P.run(q0, cfg["dt"], 10)
+    assert rel_max(q, want) < 1e-12
+    assert np.sqrt(d2 / n2) < 1e-3
