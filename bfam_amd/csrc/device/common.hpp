@@ -18,6 +18,18 @@
 namespace bfamx
 {
 
+/* Pointers read out of a descriptor in memory are generic (flat) to the
+ * compiler; casting them into the global address space lets loads and stores
+ * use global_* instructions with a scalar base and a 32-bit lane offset
+ * instead of 64-bit flat addresses per lane. */
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+template <class T> __device__ __forceinline__ gptr<T> glob(T *p)
+{
+  return (gptr<T>)(p);
+}
+
 enum
 {
   GEO_GX = 0, /* 2/h_x, 2/h_y, 2/h_z : reference -> physical derivative */
@@ -43,6 +55,15 @@ enum
   MAT_ZP = 3, /* rho cp */
   MAT_ZS = 4, /* rho cs */
   MAT_STRIDE = 8
+};
+
+enum
+{
+  FMAT_ZP = 0,   /* plus-side rho cp */
+  FMAT_ZS = 1,   /* plus-side rho cs */
+  FMAT_IZP = 2,  /* 1 / (Zp- + Zp+) */
+  FMAT_IZS = 3,  /* 1 / (Zs- + Zs+), 0 when both sides are fluid */
+  FMAT_STRIDE = 4
 };
 
 enum
@@ -85,13 +106,18 @@ struct SubDev
   const float *geo_f;
   const double *mat;
   const float *mat_f;
+  /* per element face: Zp+, Zs+, 1/(Zp- + Zp+), 1/(Zs- + Zs+) (0 if no S) */
+  const double *fmat;
+  const float *fmat_f;
   const double *Dr; /* (N+1)^2 column-major */
   const float *Dr_f;
   const double *r; /* LGL nodes */
   const double *w; /* LGL weights */
+  double wi0;      /* 1 / w_0: lift weight of a face node relative to 2/h */
   const double *src_wt;
   const float *src_wt_f;
   const double *mms_tab; /* K x 3 axes x {sin, cos} x (N+1): sin/cos(k x) */
+  const float *mms_tab_f;
   Clock *clock;
   double A[8], B[8], C[9];
   double src_amp[9];
@@ -109,11 +135,35 @@ struct Task
   int N;
 };
 
+/* a value known to be the same in every lane, moved to a scalar register */
+__device__ __forceinline__ float uniform(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform(double v)
+{
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+/* field-wise load of a task from the global task table */
+__device__ __forceinline__ Task load_task(const Task *tasks, int i)
+{
+  gptr<const Task> t = glob(tasks) + i;
+  Task r;
+  r.sub = t->sub;
+  r.e0 = t->e0;
+  r.ne = t->ne;
+  r.N = t->N;
+  return r;
+}
+
 struct LaunchArgs
 {
   const SubDev *subs;       /* device table (batched) */
   const Task *tasks;        /* device table */
-  const int *sub_ntasks;    /* tasks per subdomain in this launch */
   int ntasks;
   int mode;
   int xcd_remap;
@@ -184,24 +234,17 @@ __device__ __noinline__ void clock_scalars(const SubDev &sd, Clock *c)
   }
 }
 
-/* called by one thread after a task's results are stored: the last task of
- * a subdomain in this launch advances its clock */
-__device__ __forceinline__ void task_done(const SubDev &sd, int ntasks_sub)
+/* advance a subdomain's clock by one stage (run by one thread, in a kernel
+ * launched after the stage kernel, so no block of the stage reads the clock
+ * while it changes) */
+__device__ __forceinline__ void clock_advance(const SubDev &sd)
 {
-  __threadfence();
-  const unsigned prev = atomicAdd(&sd.clock->arrive, 1u);
-  if (prev == (unsigned)ntasks_sub - 1u)
-  {
-    __threadfence();
-    Clock *c = sd.clock;
-    const long long st = c->stage + 1;
-    c->arrive = 0u;
-    c->stage = st;
-    if (st % sd.nstages == 0)
-      c->t = c->t + c->dt;
-    clock_scalars(sd, c);
-    __threadfence();
-  }
+  gptr<Clock> c = glob(sd.clock);
+  const long long st = c->stage + 1;
+  c->stage = st;
+  if (st % sd.nstages == 0)
+    c->t = c->t + c->dt;
+  clock_scalars(sd, (Clock *)c);
 }
 
 /* stage parameters of the subdomain read at the start of a task */
@@ -220,7 +263,7 @@ __device__ __forceinline__ StageCtx stage_ctx(const SubDev &sd,
   const double t_rhs = a.t_rhs;
   if (mode == MODE_RHS)
   {
-    c.parity = (int)(sd.clock->stage & 1);
+    c.parity = (int)(glob(sd.clock)->stage & 1);
     c.a = 0;
     c.b = 0;
     c.dt = 0;

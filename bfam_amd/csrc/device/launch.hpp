@@ -14,6 +14,17 @@ inline int elastic_epb(int N)
   const int e = 512 / np;
   return e < 1 ? 1 : e;
 }
+/* line-formulation elastic stage: elements per task (one wavefront) */
+inline int elastic_line_epw(int N)
+{
+  const int nl = (N + 1) * (N + 1);
+  return 64 / nl;
+}
+/* workgroups of the line stage for ntasks tasks (one task per wave) */
+constexpr int LINE_TASKS_PER_BLOCK = 1;
+hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,
+                               hipStream_t s);
+
 inline int acoustic_epb(int N)
 {
   const int np = (N + 1) * (N + 1);
@@ -45,6 +56,8 @@ hipError_t launch_acoustic_plain(int N, const LaunchArgs &a, int grid,
                                  hipStream_t s);
 hipError_t launch_plain_update(const SubDev *sd_dev, size_t ndof,
                                hipStream_t s, int nblocks);
+/* advance the clocks of nsubs device descriptors by one stage */
+hipError_t launch_clock_advance(const SubDev *subs, int nsubs, hipStream_t s);
 
 /* reductions: per-element partial sums (FP64) of the energy density (K), or
  * interleaved pairs (2K) of |qa - qb|^2, |qb|^2 and of the error against the

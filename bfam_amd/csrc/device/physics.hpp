@@ -18,42 +18,26 @@ using P64 = Prec<double, double, double>;
 using P32R = Prec<double, float, float>;
 using P32 = Prec<float, float, float>;
 
-template <class T> __device__ __forceinline__ const T *geo_tab(const SubDev &s);
-template <> __device__ __forceinline__ const double *geo_tab(const SubDev &s)
-{
-  return s.geo;
-}
-template <> __device__ __forceinline__ const float *geo_tab(const SubDev &s)
-{
-  return s.geo_f;
-}
-template <class T> __device__ __forceinline__ const T *mat_tab(const SubDev &s);
-template <> __device__ __forceinline__ const double *mat_tab(const SubDev &s)
-{
-  return s.mat;
-}
-template <> __device__ __forceinline__ const float *mat_tab(const SubDev &s)
-{
-  return s.mat_f;
-}
-template <class T> __device__ __forceinline__ const T *dr_tab(const SubDev &s);
-template <> __device__ __forceinline__ const double *dr_tab(const SubDev &s)
-{
-  return s.Dr;
-}
-template <> __device__ __forceinline__ const float *dr_tab(const SubDev &s)
-{
-  return s.Dr_f;
-}
-template <class T> __device__ __forceinline__ const T *src_tab(const SubDev &s);
-template <> __device__ __forceinline__ const double *src_tab(const SubDev &s)
-{
-  return s.src_wt;
-}
-template <> __device__ __forceinline__ const float *src_tab(const SubDev &s)
-{
-  return s.src_wt_f;
-}
+/* per-precision copies of the element tables, as global pointers */
+#define BFAMX_TAB(fn, dfield, ffield)                                          \
+  template <class T> __device__ __forceinline__ gptr<const T> fn(const SubDev &s); \
+  template <>                                                                  \
+  __device__ __forceinline__ gptr<const double> fn<double>(const SubDev &s)    \
+  {                                                                            \
+    return glob(s.dfield);                                                     \
+  }                                                                            \
+  template <>                                                                  \
+  __device__ __forceinline__ gptr<const float> fn<float>(const SubDev &s)      \
+  {                                                                            \
+    return glob(s.ffield);                                                     \
+  }
+BFAMX_TAB(geo_tab, geo, geo_f)
+BFAMX_TAB(mat_tab, mat, mat_f)
+BFAMX_TAB(fmat_tab, fmat, fmat_f)
+BFAMX_TAB(dr_tab, Dr, Dr_f)
+BFAMX_TAB(src_tab, src_wt, src_wt_f)
+BFAMX_TAB(mms_tab, mms_tab, mms_tab_f)
+#undef BFAMX_TAB
 
 /* which part of the stage a task body runs */
 enum
@@ -66,12 +50,10 @@ enum
 template <int N, int EPB, class T> struct ElasticSmem
 {
   static constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
-  static constexpr int QW = EPB * 9 * Np;  /* staged fields */
-  static constexpr int FW = EPB * 36 * Nfp; /* face fluctuations (aliased) */
-  static constexpr int W = QW > FW ? QW : FW;
-  /* the double-precision tables start 16-byte aligned after fields + D */
-  static constexpr size_t moff = (sizeof(T) * (W + n1 * n1) + 15) / 16 * 16;
-  static constexpr size_t bytes = moff + sizeof(double) * (EPB * 6 * n1 + 8);
+  static constexpr int QW = EPB * 9 * Np;   /* staged fields */
+  static constexpr int FW = EPB * 36 * Nfp; /* face fluctuations */
+  /* LDS: fields, face fluctuations, D, sin/cos tables */
+  static constexpr size_t bytes = sizeof(T) * (QW + FW + n1 * n1 + EPB * 6 * n1);
 };
 
 /*
@@ -129,8 +111,8 @@ __device__ __forceinline__ void elastic_lift(int axis, T s, T lift, T rhoi,
   rate[8] += lift * (axis == 1 ? ms * dv2 : (axis == 2 ? ms * dv1 : T(0)));
 }
 
-__device__ __forceinline__ double mms_pattern(int f, const double *Sv,
-                                              const double *Cv)
+template <class T>
+__device__ __forceinline__ T mms_pattern(int f, const T *Sv, const T *Cv)
 {
   switch (f)
   {

@@ -12,17 +12,18 @@
 namespace bfamx
 {
 
-/* dq = A dq + dt rhs;  q = q + B dq  over all DOF, then advance the clock */
+/* dq = A dq + dt rhs;  q = q + B dq  over all DOF */
 __global__ __launch_bounds__(256) void plain_update_kernel(const SubDev *sdp,
                                                           size_t ndof)
 {
-  const SubDev &sd = *sdp;
-  const long long st = sd.clock->stage;
+  const SubDev &sd = *(const SubDev *)glob(sdp);
+  gptr<const Clock> clk = glob((const Clock *)sd.clock);
+  const long long st = clk->stage;
   const int s = (int)(st % sd.nstages);
-  const double a = sd.A[s], b = sd.B[s], dt = sd.clock->dt;
-  double *q = static_cast<double *>(sd.q[0]);
-  double *dq = static_cast<double *>(sd.dq);
-  const double *rhs = sd.rhs;
+  const double a = sd.A[s], b = sd.B[s], dt = clk->dt;
+  gptr<double> q = glob(static_cast<double *>(sd.q[0]));
+  gptr<double> dq = glob(static_cast<double *>(sd.dq));
+  gptr<const double> rhs = glob((const double *)sd.rhs);
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ndof;
        i += (size_t)gridDim.x * blockDim.x)
   {
@@ -30,9 +31,21 @@ __global__ __launch_bounds__(256) void plain_update_kernel(const SubDev *sdp,
     dq[i] = d;
     q[i] = q[i] + b * d;
   }
-  __syncthreads();
-  if (threadIdx.x == 0)
-    task_done(sd, gridDim.x);
+}
+
+/* one thread per subdomain: advance its clock past the stage just run */
+__global__ void clock_advance_kernel(const SubDev *subs, int nsubs)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsubs)
+    clock_advance(*(const SubDev *)(glob(subs) + i));
+}
+
+hipError_t launch_clock_advance(const SubDev *subs, int nsubs, hipStream_t s)
+{
+  hipLaunchKernelGGL(clock_advance_kernel, dim3((nsubs + 63) / 64), dim3(64), 0,
+                     s, subs, nsubs);
+  return hipGetLastError();
 }
 
 hipError_t launch_plain_update(const SubDev *sd_dev, size_t ndof,
@@ -221,7 +234,7 @@ __global__ __launch_bounds__(256) void mms_error_kernel(SubDev sd, int parity,
   __shared__ double red[256];
   const int e = blockIdx.x;
   const int n1 = sd.N + 1, Np = n1 * n1 * n1;
-  const double *T = sd.mms_tab + (size_t)e * 6 * n1;
+  gptr<const double> T = glob(sd.mms_tab) + (size_t)e * 6 * n1;
   double d2 = 0, n2 = 0;
   for (int n = threadIdx.x; n < Np; n += blockDim.x)
   {

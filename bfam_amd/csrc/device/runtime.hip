@@ -233,9 +233,13 @@ static void dev_free_all(bfamx_device_t *d, std::vector<void *> &list,
 
 static int nfields_of(int physics) { return physics == BFAMX_ACOUSTIC2D ? 3 : 9; }
 
-static int epb_of(int physics, int N)
+/* elements per task: the fused elastic stage runs one wavefront per task
+ * (line formulation), the unfused path one node per thread */
+static int epb_of(int physics, int prec, int N)
 {
-  return physics == BFAMX_ACOUSTIC2D ? acoustic_epb(N) : elastic_epb(N);
+  if (physics == BFAMX_ACOUSTIC2D)
+    return acoustic_epb(N);
+  return prec == BFAMX_FP64_PLAIN ? elastic_epb(N) : elastic_line_epw(N);
 }
 
 static int order_supported(int physics, int N)
@@ -265,9 +269,10 @@ extern "C" size_t bfamx_subdomain_bytes(int dim, int N, int32_t K,
   add(4 * K * nfaces), add(K * nfaces);
   add(8 * K * GEO_STRIDE), add(4 * K * GEO_STRIDE);
   add(8 * K * MAT_STRIDE), add(4 * K * MAT_STRIDE);
+  add(8 * K * nfaces * FMAT_STRIDE), add(4 * K * nfaces * FMAT_STRIDE);
   add(8 * n1 * n1), add(4 * n1 * n1), add(8 * n1), add(8 * n1);
   add(sizeof(Clock)), add(sizeof(SubDev));
-  const int epb = dim == 2 ? acoustic_epb(N) : elastic_epb(N);
+  const int epb = dim == 2 ? acoustic_epb(N) : elastic_epb(N) < elastic_line_epw(N) ? elastic_epb(N) : elastic_line_epw(N);
   add(sizeof(Task) * ((K + epb - 1) / epb)), add(sizeof(int));
   add(16 * (size_t)K);
   return b;
@@ -345,6 +350,22 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     M[MAT_ZP] = rho * cp;
     M[MAT_ZS] = rho * cs;
   }
+  /* plus-side impedances and inverse sums per element face */
+  std::vector<double> fmat((size_t)K * nfaces * FMAT_STRIDE, 0.0);
+  for (int32_t e = 0; e < K; ++e)
+    for (int f = 0; f < nfaces; ++f)
+    {
+      const int32_t ep = nbr_e[(size_t)e * nfaces + f];
+      const double *Mm = &mat[(size_t)e * MAT_STRIDE];
+      const double *Mp = &mat[(size_t)ep * MAT_STRIDE];
+      double *F = &fmat[((size_t)e * nfaces + f) * FMAT_STRIDE];
+      F[FMAT_ZP] = Mp[MAT_ZP];
+      F[FMAT_ZS] = Mp[MAT_ZS];
+      F[FMAT_IZP] = 1.0 / (Mm[MAT_ZP] + Mp[MAT_ZP]);
+      const double zs = Mm[MAT_ZS] + Mp[MAT_ZS];
+      F[FMAT_IZS] = zs > 0 ? 1.0 / zs : 0.0;
+    }
+  std::vector<float> fmatf(fmat.begin(), fmat.end());
   std::vector<float> geof(geo.begin(), geo.end()), matf(mat.begin(), mat.end());
   std::vector<float> Drf(sub->Dr, sub->Dr + n1 * n1);
 
@@ -383,6 +404,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   h.nstages = s->lsrk.n_stages;
   h.src_elem = -1;
   h.mms = 0;
+  h.wi0 = 1.0 / sub->w[0];
   for (int i = 0; i < 8; ++i)
   {
     h.A[i] = s->lsrk.A[i];
@@ -391,7 +413,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   for (int i = 0; i < 9; ++i)
     h.C[i] = s->lsrk.C[i];
 
-  const int epb = epb_of(physics, N);
+  const int epb = epb_of(physics, prec, N);
   s->ntasks = (K + epb - 1) / epb;
   std::vector<Task> tasks(s->ntasks);
   for (int t = 0; t < s->ntasks; ++t)
@@ -423,14 +445,16 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   {
     int32_t *ne;
     uint8_t *nc;
-    double *g, *m, *D, *r, *w;
-    float *gf, *mf, *Df;
+    double *g, *m, *D, *r, *w, *fm;
+    float *gf, *mf, *Df, *fmf;
     ALLOC(ne, 4 * nbr_e.size());
     ALLOC(nc, nbr_c.size());
     ALLOC(g, 8 * geo.size());
     ALLOC(gf, 4 * geof.size());
     ALLOC(m, 8 * mat.size());
     ALLOC(mf, 4 * matf.size());
+    ALLOC(fm, 8 * fmat.size());
+    ALLOC(fmf, 4 * fmatf.size());
     ALLOC(D, 8 * n1 * n1);
     ALLOC(Df, 4 * n1 * n1);
     ALLOC(r, 8 * n1);
@@ -450,6 +474,8 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     CP(gf, geof.data(), 4 * geof.size());
     CP(m, mat.data(), 8 * mat.size());
     CP(mf, matf.data(), 4 * matf.size());
+    CP(fm, fmat.data(), 8 * fmat.size());
+    CP(fmf, fmatf.data(), 4 * fmatf.size());
     CP(D, sub->Dr, 8 * n1 * n1);
     CP(Df, Drf.data(), 4 * n1 * n1);
     CP(r, sub->r, 8 * n1);
@@ -468,6 +494,8 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     h.geo_f = gf;
     h.mat = m;
     h.mat_f = mf;
+    h.fmat = fm;
+    h.fmat_f = fmf;
     h.Dr = D;
     h.Dr_f = Df;
     h.r = r;
@@ -604,6 +632,12 @@ extern "C" int bfamx_set_forcing_mms(bfamx_subdomain_t *s, double k,
       return rc;
     HIPCHK(hipMemcpy(p, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice));
     s->h.mms_tab = (const double *)p;
+    std::vector<float> tabf(tab.begin(), tab.end());
+    rc = dev_alloc(s->dev, s->allocs, &s->bytes, 4 * tabf.size(), &p);
+    if (rc)
+      return rc;
+    HIPCHK(hipMemcpy(p, tabf.data(), 4 * tabf.size(), hipMemcpyHostToDevice));
+    s->h.mms_tab_f = (const float *)p;
   }
   s->h.mms = 1;
   s->h.mms_k = k;
@@ -714,7 +748,6 @@ static LaunchArgs launch_args(bfamx_subdomain_t *s, int mode)
   memset(&a, 0, sizeof(a));
   a.subs = s->d_sub;
   a.tasks = s->d_tasks;
-  a.sub_ntasks = s->d_ntasks;
   a.ntasks = s->ntasks;
   a.mode = mode;
   a.xcd_remap = s->dev->xcd_remap;
@@ -733,8 +766,10 @@ static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
 {
   const int grid = grid_for(s->dev, a.ntasks);
   hipError_t e = s->physics == BFAMX_ELASTIC3D
-                     ? launch_elastic_stage(s->prec, s->N, a, grid, s->dev->stream)
+                     ? launch_elastic_line(s->prec, s->N, a, grid, s->dev->stream)
                      : launch_acoustic_stage(s->prec, s->N, a, grid, s->dev->stream);
+  if (e == hipSuccess && a.mode == MODE_STAGE)
+    e = launch_clock_advance(s->d_sub, 1, s->dev->stream);
   if (e != hipSuccess)
     return bfamx_fail(BFAMX_EHIP, "stage launch failed: %s", hipGetErrorString(e));
   return BFAMX_OK;
@@ -763,6 +798,8 @@ static int stage_once(bfamx_subdomain_t *s)
                              ? (ndof + 255) / 256
                              : (size_t)s->dev->num_cu * 16);
     hipError_t e = launch_plain_update(s->d_sub, ndof, s->dev->stream, nb);
+    if (e == hipSuccess)
+      e = launch_clock_advance(s->d_sub, 1, s->dev->stream);
     if (e != hipSuccess)
       return bfamx_fail(BFAMX_EHIP, "update launch failed: %s", hipGetErrorString(e));
     return BFAMX_OK;
@@ -950,7 +987,7 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
   {
     bfamx_subdomain_t *s = subs[i];
     hs[i] = s->h;
-    const int epb = N < 0 ? elastic_epb(s->N) : epb_of(physics, s->N);
+    const int epb = epb_of(physics, prec, s->N);
     int cnt = 0;
     for (int32_t e0 = 0; e0 < s->K; e0 += epb, ++cnt)
     {
@@ -1019,15 +1056,16 @@ static int batch_stage(bfamx_batch_t *b)
   memset(&a, 0, sizeof(a));
   a.subs = b->d_subs;
   a.tasks = b->d_tasks;
-  a.sub_ntasks = b->d_ntasks;
   a.ntasks = b->ntasks;
   a.mode = MODE_STAGE;
   a.xcd_remap = b->dev->xcd_remap;
   a.nxcd = b->dev->num_xcd;
   const int grid = grid_for(b->dev, b->ntasks);
   hipError_t e = b->physics == BFAMX_ELASTIC3D
-                     ? launch_elastic_stage(b->prec, b->N, a, grid, b->dev->stream)
+                     ? launch_elastic_line(b->prec, b->N, a, grid, b->dev->stream)
                      : launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
+  if (e == hipSuccess)
+    e = launch_clock_advance(b->d_subs, (int)b->subs.size(), b->dev->stream);
   if (e != hipSuccess)
     return bfamx_fail(BFAMX_EHIP, "batch stage launch: %s", hipGetErrorString(e));
   return BFAMX_OK;

@@ -44,12 +44,13 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
 
   TC *sq = reinterpret_cast<TC *>(smem);
   TC *sD = sq + SM::W;
-  const TC *Dg = dr_tab<TC>(sd);
-  for (int t = tid; t < n1 * n1; t += blockDim.x)
-    sD[t] = Dg[t];
+  {
+    gptr<const TC> Dg = dr_tab<TC>(sd);
+    for (int t = tid; t < n1 * n1; t += blockDim.x)
+      sD[t] = Dg[t];
+  }
 
-  const TS *qin = static_cast<const TS *>(sd.q[cx.parity]);
-  TR *dqp = static_cast<TR *>(sd.dq);
+  gptr<const TS> qin = glob(static_cast<const TS *>(sd.q[cx.parity]));
   TS qs[NF];
   if (act)
   {
@@ -66,7 +67,7 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
   TC rhoi = 0, bulk = 0, Zm = 0;
   if (act)
   {
-    const TC *M = mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
+    gptr<const TC> M = mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
     rhoi = M[MAT_RHOI];
     bulk = M[MAT_LAM];
     Zm = M[MAT_ZP];
@@ -75,11 +76,11 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
   {
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      rate[f] = (TC)sd.rhs[qidx(e, f, n, NF, Np)];
+      rate[f] = (TC)glob(sd.rhs)[qidx(e, f, n, NF, Np)];
   }
   if (act && PART != PART_SURFACE)
   {
-    const TC *G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+    gptr<const TC> G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
     const TC *u = sq + le * NF * Np;
     TC px = 0, ux = 0, py = 0, vy = 0;
 #pragma unroll
@@ -119,8 +120,8 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
       const int f = rem / Nfp, a = rem - (rem / Nfp) * Nfp;
       const int et = tk.e0 + lt;
       const int nm = face_node<2>(f, n1, a, 0);
-      const int32_t ep = sd.nbr_elem[(size_t)et * 4 + f];
-      const int code = sd.nbr_code[(size_t)et * 4 + f];
+      const int32_t ep = glob(sd.nbr_elem)[(size_t)et * 4 + f];
+      const int code = glob(sd.nbr_code)[(size_t)et * 4 + f];
       const int oi = perm_apply(code >> 3, n1, a, 0);
       const int np = face_node<2>(code & 7, n1, oi, 0);
       const TC *u = sq + lt * NF * Np;
@@ -129,8 +130,8 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
       const TC pm = u[nm], vnm = s * u[(1 + axis) * Np + nm];
       const TC pp = (TC)qin[qidx(ep, 0, np, NF, Np)];
       const TC vnp = s * (TC)qin[qidx(ep, 1 + axis, np, NF, Np)];
-      const TC *Mm = mat_tab<TC>(sd) + (size_t)et * MAT_STRIDE;
-      const TC *Mp = mat_tab<TC>(sd) + (size_t)ep * MAT_STRIDE;
+      gptr<const TC> Mm = mat_tab<TC>(sd) + (size_t)et * MAT_STRIDE;
+      gptr<const TC> Mp = mat_tab<TC>(sd) + (size_t)ep * MAT_STRIDE;
       fl[r] = (Mp[MAT_ZP] * (vnp - vnm) + (pm - pp)) / (Mm[MAT_ZP] + Mp[MAT_ZP]);
     }
   }
@@ -147,7 +148,7 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
 
   if (act)
   {
-    const TC *G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+    gptr<const TC> G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
     const TC *F = sfl + le * 4 * Nfp;
 #define BFAMX_ALIFT(face, cond, fn_)                                           \
   if (cond)                                                                    \
@@ -182,10 +183,11 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
   {
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      sd.rhs[qidx(e, f, n, NF, Np)] = (double)rate[f];
+      glob(sd.rhs)[qidx(e, f, n, NF, Np)] = (double)rate[f];
     return;
   }
-  TS *qout = static_cast<TS *>(sd.q[cx.parity ^ 1]);
+  gptr<TS> qout = glob(static_cast<TS *>(sd.q[cx.parity ^ 1]));
+  gptr<TR> dqp = glob(static_cast<TR *>(sd.dq));
   const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
   const TS cb = (TS)cx.b;
 #pragma unroll
@@ -206,13 +208,11 @@ acoustic_stage_kernel(LaunchArgs a)
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
-    const Task tk = a.tasks[ti];
-    const SubDev &sd = a.subs[tk.sub];
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     acoustic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
     __syncthreads();
-    if (threadIdx.x == 0 && a.mode == MODE_STAGE)
-      task_done(sd, a.sub_ntasks[tk.sub]);
   }
 }
 
@@ -262,8 +262,8 @@ acoustic_part_kernel(LaunchArgs a)
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
-    const Task tk = a.tasks[ti];
-    const SubDev &sd = a.subs[tk.sub];
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     acoustic_task<N, EPB, P64, PART>(sd, tk, cx, a.mode, smem);
     __syncthreads();

@@ -13,13 +13,15 @@
  *   (bfamo_lsrk_t coefficients, bfam.c:9016-9040).
  *
  * One workgroup handles EPB elements, one thread per volume node:
- *   1. stream q[e] (NF*Np contiguous) and dq[e] from HBM; stage q in LDS
- *   2. 18 directional derivatives per node from LDS (2(N+1) flop each)
- *   3. face tasks (one thread per face node): minus trace from LDS, plus trace
+ *   1. stream q[e] (NF*Np contiguous) from HBM; stage it in LDS
+ *   2. face tasks (one thread per face node): minus trace from LDS, plus trace
  *      gathered from the neighbour's q through nbr_elem/nbr_code (the on-device
  *      replacement of the glue trace exchange), upwind fluctuations -> LDS
+ *   3. 18 directional derivatives per node from LDS (2(N+1) flop each),
+ *      folded into the rates one axis at a time
  *   4. node owners lift the fluctuations of their faces (fixed face order)
- *   5. dq and q_out written back (q ping-pongs so neighbours read stage input)
+ *   5. dq read, dq and q_out written (q ping-pongs so neighbours read the
+ *      stage input); nothing but the loop state is held across a barrier
  */
 #include "physics.hpp"
 
@@ -34,7 +36,8 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
   using TS = typename P::TS;
   using TR = typename P::TR;
   using TC = typename P::TC;
-  static_assert(N >= 3, "3-D face buffer aliasing needs N >= 3");
+  /* state re-read from LDS at the end when it is staged exactly */
+  constexpr bool KEEP_Q = !__is_same(TS, TC);
   constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1, NF = 9;
   constexpr int BS = EPB * Np;
   constexpr int NT = EPB * 6 * Nfp;
@@ -47,57 +50,116 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
   const int e = tk.e0 + le;
   const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
 
-  TC *sq = reinterpret_cast<TC *>(smem);
-  TC *sD = sq + SM::W;
-  double *smms = reinterpret_cast<double *>(smem + SM::moff);
+  TC *sq = reinterpret_cast<TC *>(smem); /* [EPB][9][Np] */
+  TC *sfl = sq + SM::QW;                 /* [EPB][6 faces][6 comps][Nfp] */
+  TC *sD = sfl + SM::FW;
+  TC *smms = sD + n1 * n1;
 
-  const TC *Dg = dr_tab<TC>(sd);
-  for (int t = tid; t < n1 * n1; t += blockDim.x)
-    sD[t] = Dg[t];
+  {
+    gptr<const TC> Dg = dr_tab<TC>(sd);
+    for (int t = tid; t < n1 * n1; t += blockDim.x)
+      sD[t] = Dg[t];
+  }
+  if (PART != PART_SURFACE && sd.mms)
+  {
+    gptr<const TC> mt = mms_tab<TC>(sd) + (size_t)tk.e0 * 6 * n1;
+    for (int t = tid; t < tk.ne * 6 * n1; t += blockDim.x)
+      smms[t] = mt[t];
+  }
 
-  const TS *qin = static_cast<const TS *>(sd.q[cx.parity]);
-  TR *dqp = static_cast<TR *>(sd.dq);
-  TS qs[NF];
-  TR dqv[NF];
+  /* ---- 1. stream the element's state into LDS ---- */
+  gptr<const TS> qin = glob(static_cast<const TS *>(sd.q[cx.parity]));
+  TS qs[KEEP_Q ? NF : 1];
   if (act)
   {
+    gptr<const TS> qe = qin + (size_t)e * NF * Np + n;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      qs[f] = qin[qidx(e, f, n, NF, Np)];
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-      sq[(le * NF + f) * Np + n] = (TC)qs[f];
+    {
+      const TS v = qe[f * Np];
+      if constexpr (KEEP_Q)
+        qs[f] = v;
+      sq[(le * NF + f) * Np + n] = (TC)v;
+    }
   }
-  if (sd.mms)
-    for (int t = tid; t < tk.ne * 6 * n1; t += blockDim.x)
-      smms[t] = sd.mms_tab[(size_t)tk.e0 * 6 * n1 + t];
   __syncthreads();
 
-  /* ---- volume: 18 directional derivatives, folded into the rates one axis
-   * at a time so only six accumulators are live ---- */
-  TC rate[NF];
-  TC rhoi = 0, lam = 0, mu = 0;
-  if (act)
+  /* ---- 2. faces: one task per face node, fluctuations straight to LDS ---- */
+  if constexpr (PART != PART_VOLUME)
   {
-    const TC *M = mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
-    rhoi = M[MAT_RHOI];
-    lam = M[MAT_LAM];
-    mu = M[MAT_MU];
+    gptr<const int32_t> nbe = glob(sd.nbr_elem);
+    gptr<const uint8_t> nbc = glob(sd.nbr_code);
+    gptr<const TC> mtab = mat_tab<TC>(sd);
+#pragma unroll
+    for (int r = 0; r < TPT; ++r)
+    {
+      const int t = tid + r * BS;
+      const int lt = t / (6 * Nfp);
+      if (tid < BS && t < NT && lt < tk.ne)
+      {
+        const int rem = t - lt * 6 * Nfp;
+        const int f = rem / Nfp, fn = rem - (rem / Nfp) * Nfp;
+        const int et = tk.e0 + lt;
+        const int a = fn % n1, b = fn / n1;
+        const int nm = face_node<3>(f, n1, a, b);
+        const int32_t ep = nbe[(size_t)et * 6 + f];
+        const int code = nbc[(size_t)et * 6 + f];
+        const int oi = perm_apply(code >> 3, n1, a, b);
+        const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
+        gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
+        const TC *u = sq + lt * NF * Np + nm;
+        TC vm[3], sm[6], vp[3], spl[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+          vm[c] = u[c * Np];
+          vp[c] = (TC)qp[c * Np];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+        {
+          sm[c] = u[(3 + c) * Np];
+          spl[c] = (TC)qp[(3 + c) * Np];
+        }
+        gptr<const TC> Mm = mtab + (size_t)et * MAT_STRIDE;
+        gptr<const TC> Mp = mtab + (size_t)ep * MAT_STRIDE;
+        TC dv[3], dT[3];
+        elastic_riemann<TC>(f >> 1, (f & 1) ? TC(1) : TC(-1), vm, sm, Mm[MAT_ZP],
+                            Mm[MAT_ZS], vp, spl, Mp[MAT_ZP], Mp[MAT_ZS], dv, dT);
+        TC *o = sfl + (lt * 6 + f) * 6 * Nfp + fn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+          o[c * Nfp] = dv[c];
+          o[(3 + c) * Nfp] = dT[c];
+        }
+      }
+    }
+    __syncthreads();
   }
-  if (act && PART == PART_SURFACE)
+  if (!act)
+    return;
+
+  TC rate[NF];
+  gptr<const TC> M = mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
+  const TC rhoi = M[MAT_RHOI], lam = M[MAT_LAM], mu = M[MAT_MU];
+  gptr<const TC> G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+  if (PART == PART_SURFACE)
   {
+    gptr<const double> rh = glob((const double *)sd.rhs) + (size_t)e * NF * Np + n;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      rate[f] = (TC)sd.rhs[qidx(e, f, n, NF, Np)];
+      rate[f] = (TC)rh[f * Np];
   }
-  if (act && PART != PART_SURFACE)
+  else
   {
-    const TC *G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+    /* ---- 3. volume: 18 directional derivatives, folded into the rates one
+     * axis at a time so only six accumulators are live ---- */
     const TC *u = sq + le * NF * Np;
     const TC l2m = lam + TC(2) * mu;
     {
       TC v1 = 0, v2 = 0, v3 = 0, s11 = 0, s12 = 0, s13 = 0;
-#pragma unroll
+#pragma unroll 2
       for (int m = 0; m < n1; ++m)
       {
         const TC c = sD[i + m * n1];
@@ -123,7 +185,7 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
     }
     {
       TC v1 = 0, v2 = 0, v3 = 0, s22 = 0, s12 = 0, s23 = 0;
-#pragma unroll
+#pragma unroll 2
       for (int m = 0; m < n1; ++m)
       {
         const TC c = sD[j + m * n1];
@@ -148,7 +210,7 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
     }
     {
       TC v1 = 0, v2 = 0, v3 = 0, s33 = 0, s13 = 0, s23 = 0;
-#pragma unroll
+#pragma unroll 2
       for (int m = 0; m < n1; ++m)
       {
         const TC c = sD[k + m * n1];
@@ -176,72 +238,9 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
     rate[2] *= rhoi;
   }
 
+  /* ---- 4. lift: faces in the fixed order -x,+x,-y,+y,-z,+z ---- */
   if constexpr (PART != PART_VOLUME)
   {
-  /* ---- faces: one task per face node, fluctuations kept in registers ---- */
-  TC fl[TPT][6];
-#pragma unroll
-  for (int r = 0; r < TPT; ++r)
-  {
-    const int t = tid + r * BS;
-    const int lt = t / (6 * Nfp);
-    if (tid < BS && t < NT && lt < tk.ne)
-    {
-      const int rem = t - lt * 6 * Nfp;
-      const int f = rem / Nfp, fn = rem - (rem / Nfp) * Nfp;
-      const int et = tk.e0 + lt;
-      const int a = fn % n1, b = fn / n1;
-      const int nm = face_node<3>(f, n1, a, b);
-      const int32_t ep = sd.nbr_elem[(size_t)et * 6 + f];
-      const int code = sd.nbr_code[(size_t)et * 6 + f];
-      const int oi = perm_apply(code >> 3, n1, a, b);
-      const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
-      const TC *u = sq + lt * NF * Np;
-      TC vm[3], sm[6], vp[3], spl[6];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-      {
-        vm[c] = u[c * Np + nm];
-        vp[c] = (TC)qin[qidx(ep, c, np, NF, Np)];
-      }
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-      {
-        sm[c] = u[(3 + c) * Np + nm];
-        spl[c] = (TC)qin[qidx(ep, 3 + c, np, NF, Np)];
-      }
-      const TC *Mm = mat_tab<TC>(sd) + (size_t)et * MAT_STRIDE;
-      const TC *Mp = mat_tab<TC>(sd) + (size_t)ep * MAT_STRIDE;
-      const int axis = f >> 1;
-      const TC s = (f & 1) ? TC(1) : TC(-1);
-      elastic_riemann<TC>(axis, s, vm, sm, Mm[MAT_ZP], Mm[MAT_ZS], vp, spl,
-                          Mp[MAT_ZP], Mp[MAT_ZS], &fl[r][0], &fl[r][3]);
-    }
-  }
-  __syncthreads(); /* every read of the staged fields is done */
-
-  TC *sfl = sq; /* [EPB][6 faces][6 comps][Nfp] */
-#pragma unroll
-  for (int r = 0; r < TPT; ++r)
-  {
-    const int t = tid + r * BS;
-    const int lt = t / (6 * Nfp);
-    if (tid < BS && t < NT && lt < tk.ne)
-    {
-      const int rem = t - lt * 6 * Nfp;
-      const int f = rem / Nfp, fn = rem - (rem / Nfp) * Nfp;
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-        sfl[((lt * 6 + f) * 6 + c) * Nfp + fn] = fl[r][c];
-    }
-  }
-  __syncthreads();
-
-
-  /* ---- lift: faces in the fixed order -x,+x,-y,+y,-z,+z ---- */
-  if (act)
-  {
-    const TC *G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
     const TC *F = sfl + le * 36 * Nfp;
     TC flv[6];
 #define BFAMX_LIFT(face, cond, fn_)                                            \
@@ -261,10 +260,6 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
 #undef BFAMX_LIFT
   }
 
-  }
-  if (!act)
-    return;
-
   /* ---- point source and manufactured forcing ---- */
   if (PART != PART_SURFACE && e == sd.src_elem)
   {
@@ -276,35 +271,39 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
   }
   if (PART != PART_SURFACE && sd.mms)
   {
-    const double *T = smms + le * 6 * n1;
-    const double Sv[3] = {T[0 * n1 + i], T[2 * n1 + j], T[4 * n1 + k]};
-    const double Cv[3] = {T[1 * n1 + i], T[3 * n1 + j], T[5 * n1 + k]};
-    const double sw = cx.sw, cw = cx.cw;
+    const TC *T = smms + le * 6 * n1;
+    const TC Sv[3] = {T[0 * n1 + i], T[2 * n1 + j], T[4 * n1 + k]};
+    const TC Cv[3] = {T[1 * n1 + i], T[3 * n1 + j], T[5 * n1 + k]};
+    const TC sw = (TC)cx.sw, cw = (TC)cx.cw;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      rate[f] += (TC)(sd.mms_F[f] * mms_pattern(f, Sv, Cv) * (f < 3 ? sw : cw));
+      rate[f] += (TC)sd.mms_F[f] * mms_pattern<TC>(f, Sv, Cv) * (f < 3 ? sw : cw);
   }
 
-  /* ---- output ---- */
+  /* ---- 5. output ---- */
   if (mode == MODE_RHS || PART != PART_FUSED)
   {
+    gptr<double> rh = glob(sd.rhs) + (size_t)e * NF * Np + n;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      sd.rhs[qidx(e, f, n, NF, Np)] = (double)rate[f];
+      rh[f * Np] = (double)rate[f];
     return;
   }
-  TS *qout = static_cast<TS *>(sd.q[cx.parity ^ 1]);
-#pragma unroll
-  for (int f = 0; f < NF; ++f)
-    dqv[f] = dqp[qidx(e, f, n, NF, Np)];
+  gptr<TS> qo = glob(static_cast<TS *>(sd.q[cx.parity ^ 1])) + (size_t)e * NF * Np + n;
+  gptr<TR> dqe = glob(static_cast<TR *>(sd.dq)) + (size_t)e * NF * Np + n;
   const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
   const TS cb = (TS)cx.b;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
   {
-    const TR d = ca * dqv[f] + cdt * (TR)rate[f];
-    dqp[qidx(e, f, n, NF, Np)] = d;
-    qout[qidx(e, f, n, NF, Np)] = qs[f] + cb * (TS)d;
+    TS qv;
+    if constexpr (KEEP_Q)
+      qv = qs[f];
+    else
+      qv = (TS)sq[(le * NF + f) * Np + n];
+    const TR d = ca * dqe[f * Np] + cdt * (TR)rate[f];
+    dqe[f * Np] = d;
+    qo[f * Np] = qv + cb * (TS)d;
   }
 }
 
@@ -316,13 +315,11 @@ elastic_stage_kernel(LaunchArgs a)
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
-    const Task tk = a.tasks[ti];
-    const SubDev &sd = a.subs[tk.sub];
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     elastic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
     __syncthreads();
-    if (threadIdx.x == 0 && a.mode == MODE_STAGE)
-      task_done(sd, a.sub_ntasks[tk.sub]);
   }
 }
 
@@ -341,8 +338,8 @@ __global__ __launch_bounds__(512) BFAMX_WAVES void elastic_stage_mixed(LaunchArg
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
-    const Task tk = a.tasks[ti];
-    const SubDev &sd = a.subs[tk.sub];
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     switch (tk.N)
     {
@@ -353,8 +350,6 @@ __global__ __launch_bounds__(512) BFAMX_WAVES void elastic_stage_mixed(LaunchArg
     default: elastic_task<7, 1, P>(sd, tk, cx, a.mode, smem); break;
     }
     __syncthreads();
-    if (threadIdx.x == 0 && a.mode == MODE_STAGE)
-      task_done(sd, a.sub_ntasks[tk.sub]);
   }
 }
 
@@ -403,8 +398,8 @@ elastic_part_kernel(LaunchArgs a)
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
-    const Task tk = a.tasks[ti];
-    const SubDev &sd = a.subs[tk.sub];
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     elastic_task<N, EPB, P64, PART>(sd, tk, cx, a.mode, smem);
     __syncthreads();

@@ -1,0 +1,625 @@
+/*
+ * Fused LSRK stage for 3-D isotropic velocity-stress elasticity, "line"
+ * formulation: one wavefront per element (N = 7; 64 / (N+1)^2 elements per
+ * wave for smaller N), each lane owning one line of N+1 nodes.
+ *
+ * Same method and result as the node-parallel body in stage_elastic.hip
+ * (strong-form LGL DG, exact-Riemann upwind fluctuations, LGL lift, LSRK
+ * 2N-storage update; see there for the references), organised for CDNA
+ * wavefronts:
+ *   - lane l = (a, b) = (l % n1, l / n1) of its element's n1 x n1 lines;
+ *   - x pass: lane owns the i-line (j, k) = (a, b); the state arrives as
+ *     contiguous n1-vectors straight into registers and the x derivative is a
+ *     register-only contraction with D (uniform, scalar-loaded);
+ *   - y pass: lane owns the j-line (i, k) = (a, b), read from a bank-skewed
+ *     LDS copy of the fields (node (i,j,k) at i + n1 j + S k, S = n1^2 + pad);
+ *   - z pass: lane owns the k-line (i, j) = (a, b);
+ *   - in each pass the lane also owns face node (a, b) of the two faces normal
+ *     to that axis, which are the two ends of its line, so the gather of the
+ *     neighbour trace, the Riemann solve and the lift are spread evenly over
+ *     all lanes with no divergence;
+ *   - y and z contributions (six per axis: the three traction components,
+ *     the normal velocity and the two tangential velocities, each
+ *     already including the face lift) return to the x-line owners through
+ *     a one-field LDS transpose, where the material combines them into the
+ *     nine rates;
+ *   - dq and q are updated in the x-line ownership with vector loads/stores.
+ * Nothing crosses a workgroup barrier: each wave only touches its own LDS
+ * region, and LDS operations of one wave execute in issue order.
+ */
+#include <type_traits>
+
+#include "physics.hpp"
+
+namespace bfamx
+{
+
+template <int N, class TL, class TC> struct LineCfg
+{
+  static constexpr int n1 = N + 1;
+  static constexpr int NL = n1 * n1;  /* lines per element */
+  static constexpr int Np = n1 * NL;
+  static constexpr int EPW = 64 / NL; /* elements per wavefront */
+  /* k stride of the skewed LDS layout: n1 = 8 gets 72 so that the strided
+   * j- and k-line reads of a half-wavefront hit 32 distinct banks */
+  static constexpr int S = NL + ((n1 % 4 == 0) ? n1 : 0);
+  static constexpr int FS = S * n1; /* words per field */
+  /* per wavefront: 9 staged state fields (type TL) + one transpose field */
+  static constexpr size_t bytes = EPW * FS * (9 * sizeof(TL) + sizeof(TC));
+};
+
+/* vector width of the coalesced row accesses: 16-byte loads when every
+ * element block of NF*Np values keeps that alignment */
+template <int N, class TS> struct CoalVW
+{
+  static constexpr int np = (N + 1) * (N + 1) * (N + 1);
+  static constexpr int w = 16 / (int)sizeof(TS);
+  static constexpr int value = (np % w == 0) ? w : 1;
+};
+
+/* stress field index of sigma[c][axis] (field order v1 v2 v3 s11 s22 s33 s12
+ * s13 s23) */
+__device__ __forceinline__ constexpr int sfield(int c, int a)
+{
+  return (c == a) ? 3 + c
+                  : ((c + a == 1) ? 6 : ((c + a == 2) ? 7 : 8));
+}
+
+/* the two velocity components tangential to axis a, and the shear rate they
+ * feed: a=0: (v2 -> s12, v3 -> s13); a=1: (v1 -> s12, v3 -> s23);
+ * a=2: (v1 -> s13, v2 -> s23) */
+__device__ __forceinline__ constexpr int tang(int a, int t)
+{
+  return a == 0 ? 1 + t : (a == 1 ? (t ? 2 : 0) : t);
+}
+__device__ __forceinline__ constexpr int shear_rate(int a, int t)
+{
+  return a == 0 ? (t ? 7 : 6) : (a == 1 ? (t ? 8 : 6) : (t ? 8 : 7));
+}
+
+/* upwind fluctuations for a face normal to axis A with sign s: from minus
+ * and plus velocity v and stress column Sc (sigma[.][A]); izp = 1/(Zp- + Zp+),
+ * izs = 1/(Zs- + Zs+) */
+template <int A, class TC>
+__device__ __forceinline__ void riemann_axis(TC s, const TC *vm, const TC *Sm,
+                                             const TC *vp, const TC *Sp,
+                                             TC Zpm, TC Zsm, TC Zpp, TC Zsp,
+                                             TC izp, TC izs, TC *dv, TC *dT)
+{
+  /* traction T = sigma n = s * column A; n.T = s*s*sigma_AA */
+  const TC dvn = (Zpp * (s * (vp[A] - vm[A])) + (Sp[A] - Sm[A])) * izp;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+  {
+    if (c == A)
+    {
+      dv[c] = dvn * s;
+      dT[c] = Zpm * dvn * s;
+    }
+    else
+    {
+      /* tangential: T_c = s sigma[c][A] */
+      const TC dvt = (Zsp * (vp[c] - vm[c]) + s * (Sp[c] - Sm[c])) * izs;
+      dv[c] = dvt;
+      dT[c] = Zsm * dvt;
+    }
+  }
+}
+
+/*
+ * The six per-axis raw quantities of a line, with face lifts folded in:
+ *   Q[0..2] = g D sigma[.][A] (+ lift dT)        -> rates 0..2 (times 1/rho)
+ *   Q[3]    = g D v_A        (+ lift s dv_A)     -> normal-stress rates
+ *   Q[4,5]  = g D v_tang     (+ lift s dv_tang)  -> shear rates
+ * w[f][m]: the line's values of the six fields (order of qfield()).
+ */
+template <int A>
+__device__ __forceinline__ constexpr int qfield(int c)
+{
+  return c < 3 ? sfield(c, A) : (c == 3 ? A : tang(A, c - 4));
+}
+
+/* fold the rate contribution of raw quantity c of axis A at one node;
+ * coefficients already carry the 2/h_A factor */
+template <int A, class TC>
+__device__ __forceinline__ void fold(int c, TC q, TC g, TC glam, TC gl2m,
+                                     TC gmu, TC *rate /* 9 */)
+{
+  if (c < 3)
+    rate[c] += g * q;
+  else if (c == 3)
+  {
+    rate[3] += (A == 0 ? gl2m : glam) * q;
+    rate[4] += (A == 1 ? gl2m : glam) * q;
+    rate[5] += (A == 2 ? gl2m : glam) * q;
+  }
+  else
+    rate[shear_rate(A, c - 4)] += gmu * q;
+}
+
+template <int N, class P, int PART = PART_FUSED>
+__device__ __forceinline__ void elastic_line_task(const SubDev &sd,
+                                                  const Task &tk,
+                                                  const StageCtx &cx, int mode,
+                                                  char *lds)
+{
+  using TS = typename P::TS;
+  using TR = typename P::TR;
+  using TC = typename P::TC;
+  using TL = TS; /* the state is staged exactly */
+  using CF = LineCfg<N, TL, TC>;
+  constexpr int n1 = CF::n1, NL = CF::NL, Np = CF::Np, S = CF::S, FS = CF::FS;
+  constexpr int NF = 9;
+
+  const int lane = threadIdx.x & 63;
+  const int le = lane / NL, L = lane - (lane / NL) * NL;
+  const bool act = le < tk.ne;
+  const int e = tk.e0 + (act ? le : 0);
+  const int pa = L % n1, pb = L / n1; /* (a, b) of this lane in every pass */
+  TL *sq = reinterpret_cast<TL *>(lds) + le * NF * FS; /* skewed state */
+  TC *tb = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS) +
+           le * FS; /* one-field transpose buffer */
+
+  gptr<const TS> qin = glob(static_cast<const TS *>(sd.q[cx.parity]));
+  gptr<const TC> mtab = mat_tab<TC>(sd);
+  gptr<const TC> fmt = fmat_tab<TC>(sd) + (size_t)e * 6 * FMAT_STRIDE;
+  /* D is uniform across the wavefront: keep it in scalar registers */
+  TC Dr[n1 * n1];
+  {
+    gptr<const TC> Dg = dr_tab<TC>(sd);
+#pragma unroll
+    for (int t = 0; t < n1 * n1; ++t)
+      Dr[t] = uniform(Dg[t]);
+  }
+  const TC wi0 = (TC)sd.wi0;
+  gptr<const TC> G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+  gptr<const int32_t> nbe = glob(sd.nbr_elem) + (size_t)e * 6;
+  gptr<const uint8_t> nbc = glob(sd.nbr_code) + (size_t)e * 6;
+
+  /* LDS word of node (i, j, k) */
+  auto at3 = [&](int i, int j, int k) -> int { return i + n1 * j + S * k; };
+
+  /* ---- neighbour traces at face node (pa, pb) of both faces of axis A:
+   * v1 v2 v3 and the stress column of A ---- */
+  auto gather = [&](int A, TC (*nb)[6]) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side)
+    {
+      const int f = 2 * A + side;
+      const int32_t ep = nbe[f];
+      const int code = nbc[f];
+      const int oi = perm_apply(code >> 3, n1, pa, pb);
+      const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
+      gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+      {
+        nb[side][c] = (TC)qp[c * Np];
+        nb[side][3 + c] = (TC)qp[sfield(c, A) * Np];
+      }
+    }
+  };
+
+  /* ---- 1. stream the state into LDS with coalesced row reads: lane l
+   * takes VW consecutive nodes at VW*(l + 64 r) of each field row of the
+   * task's elements (flat over element and node) ---- */
+  constexpr int VW = CoalVW<N, TS>::value;
+  constexpr int ROWS = (CF::EPW * Np + 64 * VW - 1) / (64 * VW);
+  const int nact = tk.ne * Np; /* valid (element, node) pairs of the task */
+  auto coal_pos = [&](int r, int *lel, int *n) -> bool {
+    const int q = VW * (lane + 64 * r);
+    *lel = q / Np;
+    *n = q - (q / Np) * Np;
+    return q < nact;
+  };
+  auto lds_node = [&](int n) -> int {
+    const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
+    return at3(i, j, k);
+  };
+  {
+    TS v[NF][ROWS][VW];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+      {
+        int lel, n;
+        if (coal_pos(r, &lel, &n))
+        {
+          gptr<const TS> src = qin + ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
+#pragma unroll
+          for (int t = 0; t < VW; ++t)
+            v[f][r][t] = src[t];
+        }
+      }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+      {
+        int lel, n;
+        if (coal_pos(r, &lel, &n))
+        {
+          TL *dst = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
+#pragma unroll
+          for (int t = 0; t < VW; ++t)
+            dst[lds_node(n + t)] = v[f][r][t];
+        }
+      }
+  }
+  TC nbx[2][6], nby[2][6];
+  if (act && PART != PART_VOLUME)
+  {
+    gather(0, nbx);
+    gather(1, nby);
+  }
+
+  TC rhoi = 0, lam = 0, mu = 0, Zpm = 0, Zsm = 0;
+  if (act)
+  {
+    gptr<const TC> M = mtab + (size_t)e * MAT_STRIDE;
+    rhoi = M[MAT_RHOI];
+    lam = M[MAT_LAM];
+    mu = M[MAT_MU];
+    Zpm = M[MAT_ZP];
+    Zsm = M[MAT_ZS];
+  }
+  const TC l2m = lam + TC(2) * mu;
+
+  TC rate[NF][n1];
+  if (PART == PART_SURFACE)
+  {
+    if (act)
+    {
+      gptr<const double> rh = glob((const double *)sd.rhs) + (size_t)e * NF * Np + n1 * L;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < n1; ++i)
+          rate[f][i] = (TC)rh[f * Np + i];
+    }
+  }
+  else
+  {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int i = 0; i < n1; ++i)
+        rate[f][i] = TC(0);
+  }
+
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  /*
+   * One axis pass.  The lane's line runs along axis A through (pa, pb); its
+   * two ends are face node (pa, pb) of faces 2A and 2A+1.  For each of the
+   * six raw quantities the line is contracted with D, the face lifts are
+   * added at the ends, and the result is folded into the rates of the x-line
+   * owners (directly for A = 0, through the transpose buffer otherwise).
+   */
+  auto pass = [&](auto AX, const TC (*nb)[6]) {
+    constexpr int A = decltype(AX)::value;
+    auto at = [&](int m) -> int {
+      return A == 0 ? at3(m, pa, pb) : (A == 1 ? at3(pa, m, pb) : at3(pa, pb, m));
+    };
+    TC fl[2][6];
+    if (act && PART != PART_VOLUME)
+    {
+#pragma unroll
+      for (int side = 0; side < 2; ++side)
+      {
+        const int m = side ? N : 0;
+        gptr<const TC> F = fmt + (2 * A + side) * FMAT_STRIDE;
+        TC vm[3], Sm[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+          vm[c] = (TC)sq[c * FS + at(m)];
+          Sm[c] = (TC)sq[sfield(c, A) * FS + at(m)];
+        }
+        riemann_axis<A, TC>(side ? TC(1) : TC(-1), vm, Sm, &nb[side][0],
+                            &nb[side][3], Zpm, Zsm, F[FMAT_ZP], F[FMAT_ZS],
+                            F[FMAT_IZP], F[FMAT_IZS], &fl[side][0], &fl[side][3]);
+      }
+    }
+    const TC g = G[GEO_GX + A];
+    const TC glam = g * lam, gl2m = g * l2m, gmu = g * mu;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+    {
+      const int f = qfield<A>(c);
+      TC q[n1];
+      if (act)
+      {
+        TC w[n1];
+#pragma unroll
+        for (int m = 0; m < n1; ++m)
+          w[m] = (TC)sq[f * FS + at(m)];
+#pragma unroll
+        for (int j = 0; j < n1; ++j)
+        {
+          TC acc = 0;
+          if (PART != PART_SURFACE)
+#pragma unroll
+            for (int m = 0; m < n1; ++m)
+              acc += Dr[j + m * n1] * w[m];
+          q[j] = acc;
+        }
+        /* lift (sJ/J)/w_0 = (2/h)/w_0: relative to the 2/h of the
+         * derivative it is 1/w_0 */
+        if (PART != PART_VOLUME)
+        {
+          const TC t0 = c < 3 ? fl[0][3 + c] : -fl[0][c == 3 ? A : tang(A, c - 4)];
+          const TC t1 = c < 3 ? fl[1][3 + c] : fl[1][c == 3 ? A : tang(A, c - 4)];
+          q[0] += wi0 * t0;
+          q[N] += wi0 * t1;
+        }
+      }
+      if constexpr (A == 0)
+      {
+        if (act)
+#pragma unroll
+          for (int i = 0; i < n1; ++i)
+          {
+            TC r9[NF];
+#pragma unroll
+            for (int ff = 0; ff < NF; ++ff)
+              r9[ff] = rate[ff][i];
+            fold<0, TC>(c, q[i], g, glam, gl2m, gmu, r9);
+#pragma unroll
+            for (int ff = 0; ff < NF; ++ff)
+              rate[ff][i] = r9[ff];
+          }
+      }
+      else
+      {
+        if (act)
+#pragma unroll
+          for (int j = 0; j < n1; ++j)
+            tb[at(j)] = q[j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (act)
+#pragma unroll
+          for (int i = 0; i < n1; ++i)
+          {
+            const TC qv = tb[at3(i, pa, pb)];
+            TC r9[NF];
+#pragma unroll
+            for (int ff = 0; ff < NF; ++ff)
+              r9[ff] = rate[ff][i];
+            fold<A, TC>(c, qv, g, glam, gl2m, gmu, r9);
+#pragma unroll
+            for (int ff = 0; ff < NF; ++ff)
+              rate[ff][i] = r9[ff];
+          }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+  };
+
+  pass(std::integral_constant<int, 0>{}, nbx);
+  TC nbz[2][6];
+  if (act && PART != PART_VOLUME)
+    gather(2, nbz);
+  pass(std::integral_constant<int, 1>{}, nby);
+  pass(std::integral_constant<int, 2>{}, nbz);
+
+#pragma unroll
+  for (int i = 0; i < n1; ++i)
+  {
+    rate[0][i] *= rhoi;
+    rate[1][i] *= rhoi;
+    rate[2][i] *= rhoi;
+  }
+
+  /* ---- point source and manufactured forcing ---- */
+  if (act && PART != PART_SURFACE && e == sd.src_elem)
+  {
+    gptr<const TC> w = src_tab<TC>(sd) + n1 * L;
+    const TC g = (TC)cx.g;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+    {
+      const TC ag = (TC)sd.src_amp[f] * g;
+#pragma unroll
+      for (int i = 0; i < n1; ++i)
+        rate[f][i] += w[i] * ag;
+    }
+  }
+  if (act && PART != PART_SURFACE && sd.mms)
+  {
+    gptr<const TC> T = mms_tab<TC>(sd) + (size_t)e * 6 * n1;
+    const TC sw = (TC)cx.sw, cw = (TC)cx.cw;
+    const TC Sy = T[2 * n1 + pa], Cy = T[3 * n1 + pa];
+    const TC Sz = T[4 * n1 + pb], Cz = T[5 * n1 + pb];
+#pragma unroll
+    for (int i = 0; i < n1; ++i)
+    {
+      const TC Sv[3] = {T[i], Sy, Sz};
+      const TC Cv[3] = {T[n1 + i], Cy, Cz};
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        rate[f][i] += (TC)sd.mms_F[f] * mms_pattern<TC>(f, Sv, Cv) * (f < 3 ? sw : cw);
+    }
+  }
+
+  /* ---- 5. output, one field at a time: the line's rates go through the
+   * transpose buffer so that dq and q are read and written as coalesced
+   * rows (same lane -> node assignment as the load) ---- */
+  const bool rhs_out = (mode == MODE_RHS || PART != PART_FUSED);
+  const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
+  const TS cb = (TS)cx.b;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+  {
+    if (act)
+#pragma unroll
+      for (int i = 0; i < n1; ++i)
+        tb[at3(i, pa, pb)] = rate[f][i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+    {
+      int lel, n;
+      if (!coal_pos(r, &lel, &n))
+        continue;
+      const size_t off = ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
+      const TC *tbe = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS) +
+                      lel * FS;
+      const TL *sqe = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
+      TC rv[VW];
+#pragma unroll
+      for (int t = 0; t < VW; ++t)
+        rv[t] = tbe[lds_node(n + t)];
+      if (rhs_out)
+      {
+        gptr<double> rh = glob(sd.rhs) + off;
+#pragma unroll
+        for (int t = 0; t < VW; ++t)
+          rh[t] = (double)rv[t];
+        continue;
+      }
+      gptr<TR> dqe = glob(static_cast<TR *>(sd.dq)) + off;
+      gptr<TS> qo = glob(static_cast<TS *>(sd.q[cx.parity ^ 1])) + off;
+      TR d[VW];
+#pragma unroll
+      for (int t = 0; t < VW; ++t)
+        d[t] = ca * dqe[t] + cdt * (TR)rv[t];
+#pragma unroll
+      for (int t = 0; t < VW; ++t)
+      {
+        dqe[t] = d[t];
+        qo[t] = sqe[lds_node(n + t)] + cb * (TS)d[t];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+/* one task (EPW elements) per wavefront; one-wave workgroups so that LDS,
+ * not the workgroup shape, bounds the wavefronts resident per CU */
+constexpr int LINE_WPB = 1;
+
+__device__ __forceinline__ int first_wave_task(const LaunchArgs &a, int *stride,
+                                               int *end)
+{
+  const int w = threadIdx.x >> 6;
+  const int b = blockIdx.x, G = gridDim.x;
+  if (!a.xcd_remap || G % a.nxcd)
+  {
+    *stride = G * LINE_WPB;
+    *end = a.ntasks;
+    return b * LINE_WPB + w;
+  }
+  /* XCD x owns the contiguous task slice [x*C, (x+1)*C) */
+  const int x = b % a.nxcd, s = b / a.nxcd, per = G / a.nxcd;
+  const int C = (a.ntasks + a.nxcd - 1) / a.nxcd;
+  const int lo = x * C;
+  int hi = lo + C;
+  if (hi > a.ntasks)
+    hi = a.ntasks;
+  *stride = per * LINE_WPB;
+  *end = hi;
+  return lo + s * LINE_WPB + w;
+}
+
+template <int N, class P, int PART>
+__global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_kernel(LaunchArgs a)
+{
+  using CF = LineCfg<N, typename P::TS, typename P::TC>;
+  __shared__ __align__(16) char lds[LINE_WPB * CF::bytes];
+  char *mine = lds + (threadIdx.x >> 6) * CF::bytes;
+  int stride, end;
+  for (int ti = first_wave_task(a, &stride, &end); ti < end; ti += stride)
+  {
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const StageCtx cx = stage_ctx(sd, a);
+    elastic_line_task<N, P, PART>(sd, tk, cx, a.mode, mine);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+constexpr int lmax(int a, int b) { return a > b ? a : b; }
+
+/* every order 3..7 in one launch: each wavefront switches on its task's N */
+template <class P>
+__global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_mixed(LaunchArgs a)
+{
+  using TS = typename P::TS;
+  using TC = typename P::TC;
+  constexpr int W = lmax(
+      lmax(lmax((int)LineCfg<3, TS, TC>::bytes, (int)LineCfg<4, TS, TC>::bytes),
+           lmax((int)LineCfg<5, TS, TC>::bytes, (int)LineCfg<6, TS, TC>::bytes)),
+      (int)LineCfg<7, TS, TC>::bytes);
+  __shared__ __align__(16) char lds[LINE_WPB * W];
+  char *mine = lds + (threadIdx.x >> 6) * W;
+  int stride, end;
+  for (int ti = first_wave_task(a, &stride, &end); ti < end; ti += stride)
+  {
+    const Task tk = load_task(a.tasks, ti);
+    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const StageCtx cx = stage_ctx(sd, a);
+    switch (tk.N)
+    {
+    case 3: elastic_line_task<3, P>(sd, tk, cx, a.mode, mine); break;
+    case 4: elastic_line_task<4, P>(sd, tk, cx, a.mode, mine); break;
+    case 5: elastic_line_task<5, P>(sd, tk, cx, a.mode, mine); break;
+    case 6: elastic_line_task<6, P>(sd, tk, cx, a.mode, mine); break;
+    default: elastic_line_task<7, P>(sd, tk, cx, a.mode, mine); break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+template <class P>
+static hipError_t launch_line_n(int N, const LaunchArgs &a, int grid,
+                                hipStream_t s)
+{
+  switch (N)
+  {
+#define BFAMX_CASE(NN)                                                         \
+  case NN:                                                                     \
+    hipLaunchKernelGGL((elastic_line_kernel<NN, P, PART_FUSED>), dim3(grid),   \
+                       dim3(64 * LINE_WPB), 0, s, a);                          \
+    break;
+    BFAMX_CASE(3)
+    BFAMX_CASE(4)
+    BFAMX_CASE(5)
+    BFAMX_CASE(6)
+    BFAMX_CASE(7)
+#undef BFAMX_CASE
+  default:
+    hipLaunchKernelGGL((elastic_line_mixed<P>), dim3(grid), dim3(64 * LINE_WPB),
+                       0, s, a);
+    break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,
+                               hipStream_t s)
+{
+  switch (prec)
+  {
+  case PREC_FP32: return launch_line_n<P32>(N, a, grid, s);
+  case PREC_FP32_RHS: return launch_line_n<P32R>(N, a, grid, s);
+  default: return launch_line_n<P64>(N, a, grid, s);
+  }
+}
+
+} // namespace bfamx
