@@ -7,12 +7,15 @@
  * (strong-form LGL DG, exact-Riemann upwind fluctuations, LGL lift, LSRK
  * 2N-storage update; see there for the references), organised for CDNA
  * wavefronts:
+ *   - the state moves between HBM and LDS as coalesced rows (lane l takes
+ *     nodes VW*(l + 64 r) .. +VW-1 of each field row, 16-byte accesses);
+ *     LDS holds it bank-skewed (node (i,j,k) at i + n1 j + S k,
+ *     S = n1^2 + pad) in the arithmetic type;
  *   - lane l = (a, b) = (l % n1, l / n1) of its element's n1 x n1 lines;
- *   - x pass: lane owns the i-line (j, k) = (a, b); the state arrives as
- *     contiguous n1-vectors straight into registers and the x derivative is a
- *     register-only contraction with D (uniform, scalar-loaded);
- *   - y pass: lane owns the j-line (i, k) = (a, b), read from a bank-skewed
- *     LDS copy of the fields (node (i,j,k) at i + n1 j + S k, S = n1^2 + pad);
+ *   - x pass: lane owns the i-line (j, k) = (a, b) and accumulates the nine
+ *     rates of that line in registers; the contraction with D uses the
+ *     wave-uniform D held in scalar registers;
+ *   - y pass: lane owns the j-line (i, k) = (a, b);
  *   - z pass: lane owns the k-line (i, j) = (a, b);
  *   - in each pass the lane also owns face node (a, b) of the two faces normal
  *     to that axis, which are the two ends of its line, so the gather of the
@@ -23,7 +26,9 @@
  *     already including the face lift) return to the x-line owners through
  *     a one-field LDS transpose, where the material combines them into the
  *     nine rates;
- *   - dq and q are updated in the x-line ownership with vector loads/stores.
+ *   - the rates go back to row order through the same transpose buffer, one
+ *     field at a time, and dq and q are updated with coalesced row accesses
+ *     (dq fetched ahead of use).
  * Nothing crosses a workgroup barrier: each wave only touches its own LDS
  * region, and LDS operations of one wave execute in issue order.
  */
@@ -146,7 +151,10 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   using TS = typename P::TS;
   using TR = typename P::TR;
   using TC = typename P::TC;
-  using TL = TS; /* the state is staged exactly */
+  /* LDS copy of the state in the arithmetic type; the update takes q_old
+   * from that copy when it is exact (TS == TC), else from global memory */
+  using TL = TC;
+  constexpr bool q_from_lds = std::is_same<TS, TL>::value;
   using CF = LineCfg<N, TL, TC>;
   constexpr int n1 = CF::n1, NL = CF::NL, Np = CF::Np, S = CF::S, FS = CF::FS;
   constexpr int NF = 9;
@@ -216,6 +224,11 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
     const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
     return at3(i, j, k);
   };
+  /* VW consecutive nodes n..n+VW-1 (n % VW == 0) stay on one x-line when
+   * VW divides n1; they are then contiguous in LDS and move as one vector */
+  constexpr bool vec_lds = (n1 % VW) == 0;
+  using VL = TL __attribute__((ext_vector_type(VW)));
+  using VC = TC __attribute__((ext_vector_type(VW)));
   {
     TS v[NF][ROWS][VW];
 #pragma unroll
@@ -241,9 +254,20 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         if (coal_pos(r, &lel, &n))
         {
           TL *dst = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
+          if constexpr (vec_lds)
+          {
+            VL x;
 #pragma unroll
-          for (int t = 0; t < VW; ++t)
-            dst[lds_node(n + t)] = v[f][r][t];
+            for (int t = 0; t < VW; ++t)
+              x[t] = (TL)v[f][r][t];
+            *reinterpret_cast<VL *>(dst + lds_node(n)) = x;
+          }
+          else
+          {
+#pragma unroll
+            for (int t = 0; t < VW; ++t)
+              dst[lds_node(n + t)] = (TL)v[f][r][t];
+          }
         }
       }
   }
@@ -451,10 +475,42 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
 
   /* ---- 5. output, one field at a time: the line's rates go through the
    * transpose buffer so that dq and q are read and written as coalesced
-   * rows (same lane -> node assignment as the load) ---- */
+   * rows (same lane -> node assignment as the load).  The dq (and, when the
+   * LDS copy is rounded, q_old) rows are fetched PD fields ahead so that the
+   * per-field LDS round trips do not wait on global memory ---- */
   const bool rhs_out = (mode == MODE_RHS || PART != PART_FUSED);
   const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
   const TS cb = (TS)cx.b;
+  constexpr int PD = sizeof(TS) == 4 ? NF : 3;
+  TR dqv[PD][ROWS][VW];
+  TS qv[q_from_lds ? 1 : PD][ROWS][VW];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+    {
+      int lel, n;
+      if (!coal_pos(r, &lel, &n))
+        continue;
+      const size_t off = ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
+      gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + off;
+#pragma unroll
+      for (int t = 0; t < VW; ++t)
+        dqv[f % PD][r][t] = dqe[t];
+      if constexpr (!q_from_lds)
+      {
+#pragma unroll
+        for (int t = 0; t < VW; ++t)
+          qv[f % PD][r][t] = qin[off + t];
+      }
+    }
+  };
+  if (!rhs_out)
+  {
+#pragma unroll
+    for (int f = 0; f < PD; ++f)
+      fetch(f);
+  }
+  const TC *tb0 = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS);
 #pragma unroll
   for (int f = 0; f < NF; ++f)
   {
@@ -465,6 +521,45 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    TC rv[ROWS][VW];
+    TL qo_l[ROWS][VW];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+    {
+      int lel, n;
+      if (!coal_pos(r, &lel, &n))
+        continue;
+      const TC *tbe = tb0 + lel * FS;
+      const TL *sqe = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
+      if constexpr (vec_lds)
+      {
+        const VC x = *reinterpret_cast<const VC *>(tbe + lds_node(n));
+#pragma unroll
+        for (int t = 0; t < VW; ++t)
+          rv[r][t] = x[t];
+        if constexpr (q_from_lds)
+        {
+          const VL y = *reinterpret_cast<const VL *>(sqe + lds_node(n));
+#pragma unroll
+          for (int t = 0; t < VW; ++t)
+            qo_l[r][t] = y[t];
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int t = 0; t < VW; ++t)
+        {
+          rv[r][t] = tbe[lds_node(n + t)];
+          if constexpr (q_from_lds)
+            qo_l[r][t] = sqe[lds_node(n + t)];
+        }
+      }
+    }
+    /* every lane has its rows of this field: the buffer may be refilled */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int r = 0; r < ROWS; ++r)
     {
@@ -472,37 +567,27 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       if (!coal_pos(r, &lel, &n))
         continue;
       const size_t off = ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
-      const TC *tbe = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS) +
-                      lel * FS;
-      const TL *sqe = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
-      TC rv[VW];
-#pragma unroll
-      for (int t = 0; t < VW; ++t)
-        rv[t] = tbe[lds_node(n + t)];
       if (rhs_out)
       {
         gptr<double> rh = glob(sd.rhs) + off;
 #pragma unroll
         for (int t = 0; t < VW; ++t)
-          rh[t] = (double)rv[t];
+          rh[t] = (double)rv[r][t];
         continue;
       }
       gptr<TR> dqe = glob(static_cast<TR *>(sd.dq)) + off;
       gptr<TS> qo = glob(static_cast<TS *>(sd.q[cx.parity ^ 1])) + off;
-      TR d[VW];
-#pragma unroll
-      for (int t = 0; t < VW; ++t)
-        d[t] = ca * dqe[t] + cdt * (TR)rv[t];
 #pragma unroll
       for (int t = 0; t < VW; ++t)
       {
-        dqe[t] = d[t];
-        qo[t] = sqe[lds_node(n + t)] + cb * (TS)d[t];
+        const TR d = ca * dqv[f % PD][r][t] + cdt * (TR)rv[r][t];
+        const TS qold = q_from_lds ? (TS)qo_l[r][t] : qv[q_from_lds ? 0 : f % PD][r][t];
+        dqe[t] = d;
+        qo[t] = qold + cb * (TS)d;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!rhs_out && f + PD < NF)
+      fetch(f + PD);
   }
 }
 
@@ -536,7 +621,7 @@ __device__ __forceinline__ int first_wave_task(const LaunchArgs &a, int *stride,
 template <int N, class P, int PART>
 __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_kernel(LaunchArgs a)
 {
-  using CF = LineCfg<N, typename P::TS, typename P::TC>;
+  using CF = LineCfg<N, typename P::TC, typename P::TC>;
   __shared__ __align__(16) char lds[LINE_WPB * CF::bytes];
   char *mine = lds + (threadIdx.x >> 6) * CF::bytes;
   int stride, end;
@@ -561,9 +646,9 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_mixed(LaunchArgs a
   using TS = typename P::TS;
   using TC = typename P::TC;
   constexpr int W = lmax(
-      lmax(lmax((int)LineCfg<3, TS, TC>::bytes, (int)LineCfg<4, TS, TC>::bytes),
-           lmax((int)LineCfg<5, TS, TC>::bytes, (int)LineCfg<6, TS, TC>::bytes)),
-      (int)LineCfg<7, TS, TC>::bytes);
+      lmax(lmax((int)LineCfg<3, TC, TC>::bytes, (int)LineCfg<4, TC, TC>::bytes),
+           lmax((int)LineCfg<5, TC, TC>::bytes, (int)LineCfg<6, TC, TC>::bytes)),
+      (int)LineCfg<7, TC, TC>::bytes);
   __shared__ __align__(16) char lds[LINE_WPB * W];
   char *mine = lds + (threadIdx.x >> 6) * W;
   int stride, end;
