@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Headline benchmark: DOF-stage-updates/s of the fused FP32 elastic DG stage
+against the plain FP64 GPU form, on a 48^3 periodic hex brick at LGL order 7
+(9 fields, 509,607,936 DOF), PCG32 per-element material, Ricker point source,
+LSRK4(5) (Carpenter-Kennedy 5-stage, 2N storage).
+
+A step is one LSRK step (5 stages) over the whole brick.  The FP32 run goes
+through the batched path (one launch per stage over all resident subdomains,
+the step captured in a HIP graph and replayed).  Inputs are resident in HBM
+before the timed region; the clock is a pair of HIP events on the stream the
+kernels run on.
+
+The JSON line also carries:
+  fp64 / fp64_plain  rates of the fused FP64 stage and of the unfused FP64
+                     form (volume+surface kernel, then the update kernel);
+  accuracy           relative L2 of FP32 against FP64 after 200 steps from
+                     the same initial state (LGL-quadrature weighted, all
+                     nine fields);
+  roofline           the FP32 stage kernel against HBM: algorithmic bytes
+                     (q read + write, dq read + write, 4 B each = 16 B per
+                     DOF) over the mean stage duration, and the HBM traffic
+                     per launch measured with FETCH_SIZE / WRITE_SIZE,
+                     calibrated on a plain copy kernel (read from the text
+                     summary under profiles/);
+  cpu_baseline       the FP64 CPU restatement (OpenMP) timed on this host's
+                     cores over one LSRK step of the same brick.
+
+--copy-calibration runs only a 1 GiB device-to-device copy kernel (torch)
+so that the counters of a kernel of known traffic can be recorded.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_DOF = {"fp32": 16, "fp32_rhs": 24, "fp64": 32, "fp64_plain": 80}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_elastic_line_fp32_48cube_N7.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 28  # 2^28 float32 = 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 x 1 GiB device copies done")
+
+
+def read_traffic():
+    """HBM bytes per launch of the FP32 stage kernel from the committed
+    counter summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
+    kernel, and of a 1 GiB copy kernel, which reads and writes exactly
+    2^30 bytes."""
+    try:
+        kv = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        fk, wk = float(kv["kernel_FETCH_SIZE_KiB"]), float(kv["kernel_WRITE_SIZE_KiB"])
+        fc, wc = float(kv["copy_FETCH_SIZE_KiB"]), float(kv["copy_WRITE_SIZE_KiB"])
+    except (OSError, KeyError, ValueError):
+        return None
+    GiB = float(1 << 30)
+    return fk * 1024.0 * (GiB / (fc * 1024.0)) + wk * 1024.0 * (GiB / (wc * 1024.0))
+
+
+def run_cpu_baseline(cfg, budget_s):
+    """FP64 C restatement (OpenMP) over whole LSRK steps of the same brick."""
+    import numpy as np
+
+    import oracle
+
+    m = cfg["mesh"]
+    rho, cp, cs = cfg["rho"], cfg["cp"], cfg["cs"]
+    lam, mu = rho * (cp * cp - 2 * cs * cs), rho * cs * cs
+    P = oracle.OracleProblem(oracle.ELASTIC3D, m.N, m.vmapM, m.vmapP, m.h, rho, lam, mu,
+                             x0=m.x0, src=cfg["src"], ops=(m.r, m.w, 1 / m.w, m.Dr))
+    q = np.zeros((9, m.K * m.Np))
+    ndof = 9 * m.K * m.Np
+    steps, el = 0, 0.0
+    while steps == 0 or (el < budget_s and steps < 4):
+        t0 = time.perf_counter()
+        q = P.run(q, cfg["dt"], 1, t0=steps * cfg["dt"])
+        el += time.perf_counter() - t0
+        steps += 1
+        log(f"cpu baseline: {steps} step(s), {el:.1f} s")
+    return {"value": ndof * 5 * steps / el, "unit": "DOF-stage-updates/s",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{steps} LSRK4(5) step(s) ({5 * steps} stages) of the same 48^3 N=7 "
+                      f"elastic brick, FP64, C restatement with OpenMP, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=48, help="elements per side")
+    ap.add_argument("--order", type=int, default=7)
+    ap.add_argument("--accuracy-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "fp64_plain", "fp32_rhs"],
+                    help="time one form only (for profiling runs)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch stage by stage instead of replaying a captured graph")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the elastic brick would shard by element slabs with a face-trace exchange "
+              "per stage, which is not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+
+    import numpy as np
+
+    import bfam_amd as bx
+    import bfam_amd.workloads as W
+
+    t0 = time.perf_counter()
+    cfg = W.elastic3d_ricker(n=args.n, N=args.order, nsteps=200)
+    m = cfg["mesh"]
+    ndof = 9 * m.K * m.Np
+    log(f"setup: {args.n}^3 N={args.order} K={m.K} DOF={ndof} ({time.perf_counter() - t0:.1f} s)")
+    q0 = np.zeros((9, m.K * m.Np))
+    dev = bx.Device(0)
+    K, Wm = args.steps, args.warmup
+
+    def timed(sub, batched):
+        sub.set_fields(q0, 0.0, cfg["dt"])
+        if batched:
+            b = bx.Batch(dev, [sub])
+            b.capture(1)
+            for _ in range(Wm):
+                b.launch_graph()
+            dev.sync()
+            dev.timer_start()
+            for _ in range(K):
+                b.launch_graph()
+            ms = dev.timer_stop()
+            b.close()
+        else:
+            sub.step(Wm)
+            dev.sync()
+            dev.timer_start()
+            sub.step(K)
+            ms = dev.timer_stop()
+        return ms
+
+    forms = [("fp32", bx.FP32, True), ("fp64", bx.FP64, True),
+             ("fp64_plain", bx.FP64_PLAIN, False), ("fp32_rhs", bx.FP32_RHS, True)]
+    if args.only:
+        forms = [f for f in forms if f[0] == args.only]
+    res = {}
+    for name, prec, batched in forms:
+        sub = W.upload(dev, cfg, prec)
+        ms = timed(sub, batched and not args.eager)
+        sub.close()
+        res[name] = {"ms_per_step": ms / K, "ms_per_stage": ms / (5 * K),
+                     "dof_stage_per_s": ndof * 5 * K / (ms * 1e-3)}
+        log(f"{name}: {ms / K:.3f} ms/step, {res[name]['dof_stage_per_s']:.4e} DOF-stage/s")
+
+    accuracy = None
+    if not args.only:
+        a = W.upload(dev, cfg, bx.FP32)
+        b = W.upload(dev, cfg, bx.FP64)
+        for s in (a, b):
+            s.set_fields(q0, 0.0, cfg["dt"])
+            s.step(args.accuracy_steps)
+        d2, n2 = a.l2_diff(b)
+        accuracy = {"rel_l2_fp32_vs_fp64": float(np.sqrt(d2 / n2)) if n2 > 0 else None,
+                    "steps": args.accuracy_steps, "tolerance": 1e-5}
+        a.close()
+        b.close()
+        log(f"accuracy: {accuracy}")
+    dev.close()
+
+    head = res.get("fp32") or next(iter(res.values()))
+    head_name = "fp32" if "fp32" in res else next(iter(res))
+    launch_s = head["ms_per_stage"] * 1e-3
+    achieved = BYTES_PER_DOF[head_name] * ndof / launch_s / 1e9
+    traffic = read_traffic() if head_name == "fp32" else None
+
+    cpu = None
+    if not args.no_cpu and not args.only:
+        cpu = run_cpu_baseline(cfg, args.cpu_seconds)
+
+    out = {
+        "metric": "DOF-stage-updates/s FP32-batched vs FP64 @relL2<=1e-5",
+        "value": head["dof_stage_per_s"],
+        "unit": "DOF-stage-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head_name == "fp32" else "f64",
+        "data": "synthetic: PCG32 material, Ricker point source from rest",
+        "config": {"workload": f"elastic3d_ricker {args.n}^3 periodic hex brick, N={args.order}, "
+                               "9 fields, LSRK4(5)", "elements": m.K, "dof": ndof,
+                   "form": f"{head_name} fused stage, batched + HIP graph"},
+        "forms": res,
+        "fp32_over_fp64_plain": (res["fp32"]["dof_stage_per_s"] / res["fp64_plain"]["dof_stage_per_s"]
+                                 if "fp32" in res and "fp64_plain" in res else None),
+        "accuracy": accuracy,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"elastic_line_kernel<{args.order}, fp32>",
+                     "bytes_per_dof": BYTES_PER_DOF[head_name]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

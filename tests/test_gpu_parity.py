@@ -172,3 +172,45 @@ def test_mms_forcing(dev):
     want = P.run(q0, cfg["dt"], 10)
     assert rel_max(q, want) < 1e-12
     assert np.sqrt(d2 / n2) < 1e-3
+
+
+def test_full_size_rhs_against_oracle(dev):
+    """The 48^3 N=7 elastic brick at full size: one FP64 right-hand side
+    evaluation (with the Ricker source active) against the oracle on the
+    host's cores."""
+    print("full size 48^3 N=7 rhs vs oracle", flush=True)
+    cfg = W.elastic3d_ricker()
+    m = cfg["mesh"]
+    rng = np.random.default_rng(12)
+    q0 = rng.standard_normal((9, m.K * m.Np))
+    t = 0.5 * cfg["src"]["t0"]
+    sub = W.upload(dev, cfg, bx.FP64)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    got = sub.rhs(t)
+    sub.close()
+    want = oracle_problem(cfg).rhs(q0, t)
+    assert rel_max(got, want) < 1e-12
+
+
+def test_full_size_fp32_accuracy(dev):
+    """48^3 N=7, 200 LSRK4(5) steps from rest with the Ricker source: the
+    FP32 path stays within relative L2 1e-5 of the FP64 path (which matches
+    the oracle to 1e-12), and FP32 state with FP32 rates is no worse than an
+    order of magnitude beyond FP64-state / FP32-rate."""
+    print("full size 48^3 N=7 fp32 vs fp64, 200 steps", flush=True)
+    cfg = W.elastic3d_ricker()
+    m = cfg["mesh"]
+    q0 = np.zeros((9, m.K * m.Np))
+    subs = {p: W.upload(dev, cfg, p) for p in (bx.FP64, bx.FP32, bx.FP32_RHS)}
+    for s in subs.values():
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(cfg["nsteps"])
+    d2, n2 = subs[bx.FP32].l2_diff(subs[bx.FP64])
+    e32 = np.sqrt(d2 / n2)
+    d2, n2 = subs[bx.FP32_RHS].l2_diff(subs[bx.FP64])
+    e32r = np.sqrt(d2 / n2)
+    print(f"rel L2: fp32 {e32:.3e}, fp32_rhs {e32r:.3e}", flush=True)
+    for s in subs.values():
+        s.close()
+    assert n2 > 0
+    assert e32 < 1e-5 and e32r < 1e-5
