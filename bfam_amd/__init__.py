@@ -50,6 +50,22 @@ class PhysDesc(C.Structure):
                 ("rho", C.c_void_p), ("cp", C.c_void_p), ("cs", C.c_void_p)]
 
 
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int), ("N", C.c_int), ("nsteps", C.c_int), ("seed", C.c_uint64),
+                ("arrival", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
+                ("latency", C.c_double), ("dof", C.c_int64), ("energy", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("dof_stage_per_s", C.c_double), ("p50_s", C.c_double),
+                ("p99_s", C.c_double), ("max_s", C.c_double), ("peak_bytes", C.c_size_t),
+                ("max_active", C.c_int), ("nlaunch", C.c_int), ("nbatches", C.c_int)]
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
@@ -104,6 +120,14 @@ SIGNATURES = {
     "bfamx_timer_start": (C.c_int, [_vp]),
     "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
+    "bfamx_serve_generate": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(Request)]),
+    "bfamx_request_bytes": (C.c_size_t, [C.POINTER(Request), C.c_int]),
+    "bfamx_request_build": (C.c_int, [_vp, C.POINTER(Request), C.c_int, _PP,
+                                      C.POINTER(C.c_double)]),
+    "bfamx_serve": (C.c_int, [_vp, C.POINTER(Request), C.c_int, C.c_int, C.c_size_t, C.c_int,
+                              C.POINTER(RequestResult), C.POINTER(ServeStats)]),
 }
 
 
@@ -308,6 +332,17 @@ class Subdomain:
         self.h = hh
         self.nstages = t.n_stages
 
+    @classmethod
+    def _adopt(cls, dev, h, physics, precision, N, K, nstages=5):
+        """wrap a handle the library created (bfamx_request_build)"""
+        self = cls.__new__(cls)
+        self.dev, self.physics, self.precision, self.N = dev, physics, precision, N
+        self.dim = 2 if physics == ACOUSTIC2D else 3
+        self.nf = 3 if physics == ACOUSTIC2D else 9
+        self.Np = (N + 1) ** self.dim
+        self.K, self.h, self.nstages = K, h, nstages
+        return self
+
     def close(self):
         if self.h:
             lib().bfamx_subdomain_free(self.h)
@@ -400,3 +435,37 @@ class Batch:
 
     def launch_graph(self):
         _check(lib().bfamx_batch_launch_graph(self.h))
+
+
+# ------------------------------------------------------------------ serving
+
+def serve_generate(seed, nreq, rate_per_s, n=(8, 32), N=(3, 7), steps=(200, 2000)):
+    """Seeded Poisson stream of elastic requests (bfamx_serve_generate)."""
+    arr = (Request * max(nreq, 1))()
+    _check(lib().bfamx_serve_generate(seed, nreq, rate_per_s, n[0], n[1], N[0], N[1],
+                                      steps[0], steps[1], arr))
+    return [arr[i] for i in range(nreq)]
+
+
+def request_bytes(req, precision):
+    return lib().bfamx_request_bytes(C.byref(req), precision)
+
+
+def request_build(dev, req, precision):
+    """The request's subdomain at rest at t = 0, and its dt."""
+    h, dt = C.c_void_p(), C.c_double()
+    _check(lib().bfamx_request_build(dev.h, C.byref(req), precision, C.byref(h), C.byref(dt)))
+    return Subdomain._adopt(dev, h, ELASTIC3D, precision, req.N, req.n ** 3), dt.value
+
+
+def serve(dev, reqs, precision=FP32_RHS, budget_bytes=32 << 30, max_chunk=10):
+    """Run a request stream in real time; returns (per-request dicts, stats)."""
+    n = len(reqs)
+    arr = (Request * max(n, 1))(*reqs)
+    res = (RequestResult * max(n, 1))()
+    st = ServeStats()
+    _check(lib().bfamx_serve(dev.h, arr, n, precision, budget_bytes, max_chunk, res,
+                             C.byref(st)))
+    out = [{k: getattr(res[i], k) for k, _ in RequestResult._fields_} for i in range(n)]
+    stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
+    return out, stats

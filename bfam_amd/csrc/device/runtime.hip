@@ -654,15 +654,18 @@ extern "C" int bfamx_set_fields(bfamx_subdomain_t *s,
                                 const double *const *fields, double t0,
                                 double dt)
 {
-  if (!s || !fields)
+  if (!s)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_set_fields: NULL argument");
-  for (int f = 0; f < s->nf; ++f)
-    if (!fields[f])
-      return bfamx_fail(BFAMX_EINVAL, "bfamx_set_fields: field %d is NULL", f);
+  if (fields)
+    for (int f = 0; f < s->nf; ++f)
+      if (!fields[f])
+        return bfamx_fail(BFAMX_EINVAL, "bfamx_set_fields: field %d is NULL", f);
   const size_t Np = s->Np, nf = s->nf, K = s->K;
   const size_t ndof = K * nf * Np;
   HIPCHK(hipStreamSynchronize(s->dev->stream));
-  if (s->state_fp32)
+  if (!fields)
+    HIPCHK(hipMemset(s->h.q[0], 0, (s->state_fp32 ? 4 : 8) * ndof));
+  else if (s->state_fp32)
   {
     std::vector<float> buf(ndof);
     for (size_t e = 0; e < K; ++e)

@@ -221,7 +221,8 @@ int bfamx_set_source(bfamx_subdomain_t *sub, int32_t elem,
 int bfamx_set_forcing_mms(bfamx_subdomain_t *sub, double k, double omega,
                           const double *F);
 
-/* state: nfields host arrays of K*Np doubles (bfam field layout) */
+/* state: nfields host arrays of K*Np doubles (bfam field layout); fields =
+ * NULL starts from the zero state.  Resets the LSRK clock to t0, step dt. */
 int bfamx_set_fields(bfamx_subdomain_t *sub, const double *const *fields,
                      double t0, double dt);
 int bfamx_download_fields(bfamx_subdomain_t *sub, double *const *fields);
@@ -265,6 +266,57 @@ int bfamx_timer_stop(bfamx_device_t *dev, float *ms);
 /* launch tuning: blocks remapped so each XCD walks a contiguous element
  * range (1, default) or hardware round-robin order (0) */
 int bfamx_set_tuning(bfamx_device_t *dev, int xcd_remap);
+
+/* ---------------------------------------------------------------------- */
+/* serving: independent elastic solves admitted under an HBM budget as they */
+/* arrive and advanced together by one batched launch per LSRK stage.  No   */
+/* counterpart in the reference (the budget check follows                   */
+/* bfamo_device_malloc, bfam.c:8561-8589).                                  */
+/* ---------------------------------------------------------------------- */
+typedef struct
+{
+  int n;           /* elements per side of the periodic unit-cube brick */
+  int N;           /* LGL order, 3..7 */
+  int nsteps;      /* LSRK4(5) steps */
+  uint64_t seed;   /* layered material and source location */
+  double arrival;  /* seconds after the start of bfamx_serve */
+} bfamx_request_t;
+
+typedef struct
+{
+  double arrival, admitted, finished; /* seconds after the start */
+  double latency;                     /* finished - arrival */
+  int64_t dof;                        /* 9 K (N+1)^3 */
+  double energy;                      /* final energy (FP64, compensated) */
+} bfamx_request_result_t;
+
+typedef struct
+{
+  double wall_s;          /* start to last completion */
+  double dof_stage_per_s; /* sum of dof * 5 * nsteps over wall_s */
+  double p50_s, p99_s, max_s; /* latency quantiles (nearest rank) */
+  size_t peak_bytes;      /* largest resident request memory */
+  int max_active;         /* most requests advanced by one launch */
+  int nlaunch;            /* batched stage launches */
+  int nbatches;           /* batch rebuilds (joins / leaves) */
+} bfamx_serve_stats_t;
+
+/* a Poisson stream (seeded) of nreq requests: n ~ U{n_min..n_max},
+ * N ~ U{N_min..N_max}, nsteps ~ U{steps_min..steps_max} */
+int bfamx_serve_generate(uint64_t seed, int nreq, double rate_per_s,
+                         int n_min, int n_max, int N_min, int N_max,
+                         int steps_min, int steps_max, bfamx_request_t *reqs);
+/* device bytes one request holds while resident */
+size_t bfamx_request_bytes(const bfamx_request_t *rq, bfamx_precision_t prec);
+/* the request's subdomain, uploaded, source attached, at rest at t = 0 */
+int bfamx_request_build(bfamx_device_t *dev, const bfamx_request_t *rq,
+                        bfamx_precision_t prec, bfamx_subdomain_t **out,
+                        double *dt);
+/* run the stream in real time (arrivals sorted), advancing at most
+ * max_chunk steps between admissions; res[i] for reqs[i] */
+int bfamx_serve(bfamx_device_t *dev, const bfamx_request_t *reqs, int nreq,
+                bfamx_precision_t prec, size_t budget_bytes, int max_chunk,
+                bfamx_request_result_t *res, bfamx_serve_stats_t *stats);
 
 #ifdef __cplusplus
 }
