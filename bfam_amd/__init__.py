@@ -55,6 +55,22 @@ class Request(C.Structure):
                 ("arrival", C.c_double)]
 
 
+class OctreeDesc(C.Structure):
+    _fields_ = [("n_base", C.c_int), ("levels", C.c_int), ("center", C.c_double * 3),
+                ("radius", C.c_double), ("order_by_level", C.c_int * 4)]
+
+
+class MortarRec(C.Structure):
+    _fields_ = [("s_sub", C.c_int32), ("s_elem", C.c_int32), ("b_sub", C.c_int32),
+                ("b_elem", C.c_int32), ("s_face", C.c_int8), ("b_face", C.c_int8),
+                ("quad", C.c_int8), ("pad", C.c_int8)]
+
+
+MORTAR_DTYPE = np.dtype([("s_sub", np.int32), ("s_elem", np.int32), ("b_sub", np.int32),
+                         ("b_elem", np.int32), ("s_face", np.int8), ("b_face", np.int8),
+                         ("quad", np.int8), ("pad", np.int8)])
+
+
 class RequestResult(C.Structure):
     _fields_ = [("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
                 ("latency", C.c_double), ("dof", C.c_int64), ("energy", C.c_double)]
@@ -110,6 +126,7 @@ SIGNATURES = {
     "bfamx_stage": (C.c_int, [_vp]),
     "bfamx_step": (C.c_int, [_vp, C.c_int]),
     "bfamx_reduce_energy": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "bfamx_reduce_cfl_dt": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_double)]),
     "bfamx_reduce_l2_diff": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bfamx_reduce_mms_error": (C.c_int, [_vp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bfamx_batch_create": (C.c_int, [_vp, _PP, C.c_int, _PP]),
@@ -120,6 +137,13 @@ SIGNATURES = {
     "bfamx_timer_start": (C.c_int, [_vp]),
     "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
+    "bfamx_octree_build": (C.c_int, [C.POINTER(OctreeDesc), _PP]),
+    "bfamx_octree_free": (C.c_int, [_vp]),
+    "bfamx_octree_info": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int32), _ip]),
+    "bfamx_octree_sub": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int32),
+                                   _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bfamx_octree_mortars": (C.c_int, [_vp, _vp]),
     "bfamx_serve_generate": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(Request)]),
@@ -399,6 +423,11 @@ class Subdomain:
         _check(lib().bfamx_reduce_energy(self.h, C.byref(e)))
         return e.value
 
+    def cfl_dt(self, cfl=0.5):
+        dt = C.c_double()
+        _check(lib().bfamx_reduce_cfl_dt(self.h, float(cfl), C.byref(dt)))
+        return dt.value
+
     def l2_diff(self, other):
         """(sum Jw|self - other|^2, sum Jw|other|^2)"""
         d, n = C.c_double(), C.c_double()
@@ -435,6 +464,43 @@ class Batch:
 
     def launch_graph(self):
         _check(lib().bfamx_batch_launch_graph(self.h))
+
+
+# ------------------------------------------------------------- adapted mesh
+
+def octree(n_base, levels, center, radius, order_by_level):
+    """Adapted periodic brick forest (bfamx_octree_build): returns a dict with
+    per-subdomain data (N, K, EToE, EToF, glued, x0, h, gid), the mortar
+    records (structured array of MORTAR_DTYPE) and leaf counts per level."""
+    d = OctreeDesc(n_base, levels, (C.c_double * 3)(*center), radius,
+                   (C.c_int * 4)(*(list(order_by_level) + [1] * (4 - len(order_by_level)))))
+    h = C.c_void_p()
+    _check(lib().bfamx_octree_build(C.byref(d), C.byref(h)))
+    try:
+        K, ns, nm = C.c_int32(), C.c_int(), C.c_int32()
+        kl = np.zeros(4, np.int32)
+        _check(lib().bfamx_octree_info(h, C.byref(K), C.byref(ns), C.byref(nm), kl))
+        subs = []
+        for s in range(ns.value):
+            N, Ks = C.c_int(), C.c_int32()
+            _check(lib().bfamx_octree_sub(h, s, C.byref(N), C.byref(Ks), None, None, None,
+                                          None, None, None))
+            k = Ks.value
+            a = {"N": N.value, "K": k, "EToE": np.zeros(6 * k, np.int32),
+                 "EToF": np.zeros(6 * k, np.int8), "glued": np.zeros(6 * k, np.int8),
+                 "x0": np.zeros(3 * k), "h": np.zeros(3 * k), "gid": np.zeros(k, np.int32)}
+            _check(lib().bfamx_octree_sub(h, s, C.byref(N), C.byref(Ks),
+                                          *[a[k].ctypes.data for k in
+                                            ("EToE", "EToF", "glued", "x0", "h", "gid")]))
+            a["x0"] = a["x0"].reshape(k, 3)
+            a["h"] = a["h"].reshape(k, 3)
+            subs.append(a)
+        m = np.zeros(nm.value, MORTAR_DTYPE)
+        if nm.value:
+            _check(lib().bfamx_octree_mortars(h, m.ctypes.data))
+        return {"K": K.value, "subs": subs, "mortars": m, "K_level": kl}
+    finally:
+        lib().bfamx_octree_free(h)
 
 
 # ------------------------------------------------------------------ serving

@@ -62,6 +62,7 @@ hipError_t launch_clock_advance(const SubDev *subs, int nsubs, hipStream_t s);
 /* reductions: per-element partial sums (FP64) of the energy density (K), or
  * interleaved pairs (2K) of |qa - qb|^2, |qb|^2 and of the error against the
  * manufactured solution with time factors cos/sin(omega t) */
+hipError_t launch_cfl(const SubDev &sd, double *partial, hipStream_t s);
 hipError_t launch_energy(const SubDev &sd, int state_is_fp32, int parity,
                          double *partial, hipStream_t s);
 hipError_t launch_l2diff(const SubDev &a, int a_fp32, int pa, const SubDev &b,

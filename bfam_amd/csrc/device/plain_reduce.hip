@@ -278,4 +278,30 @@ hipError_t launch_mms_error(const SubDev &sd, int state_is_fp32, int parity,
   return hipGetLastError();
 }
 
+/* per element: min over axes of h_d / c_max, c_max = cp (rho cp / rho); the
+ * caller takes the minimum over elements and scales by CFL / (N+1)^2 */
+__global__ __launch_bounds__(256) void cfl_kernel(SubDev sd, double *partial)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= sd.K)
+    return;
+  gptr<const double> G = glob(sd.geo) + (size_t)e * GEO_STRIDE;
+  gptr<const double> M = glob(sd.mat) + (size_t)e * MAT_STRIDE;
+  const int dim = sd.physics == 0 ? 2 : 3;
+  const double c = M[MAT_ZP] * M[MAT_RHOI];
+  double h = G[GEO_HX];
+  if (G[GEO_HY] < h)
+    h = G[GEO_HY];
+  if (dim == 3 && G[GEO_HZ] < h)
+    h = G[GEO_HZ];
+  partial[e] = h / c;
+}
+
+hipError_t launch_cfl(const SubDev &sd, double *partial, hipStream_t s)
+{
+  hipLaunchKernelGGL(cfl_kernel, dim3((sd.K + 255) / 256), dim3(256), 0, s, sd,
+                     partial);
+  return hipGetLastError();
+}
+
 } // namespace bfamx

@@ -901,6 +901,24 @@ extern "C" int bfamx_reduce_energy(bfamx_subdomain_t *s, double *energy)
   return BFAMX_OK;
 }
 
+extern "C" int bfamx_reduce_cfl_dt(bfamx_subdomain_t *s, double cfl, double *dt)
+{
+  if (!s || !dt || !(cfl > 0))
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_cfl_dt: bad arguments");
+  hipError_t e = launch_cfl(s->h, s->d_partial, s->dev->stream);
+  if (e != hipSuccess)
+    return bfamx_fail(BFAMX_EHIP, "cfl launch: %s", hipGetErrorString(e));
+  std::vector<double> part(s->K);
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(part.data(), s->d_partial, 8 * part.size(), hipMemcpyDeviceToHost));
+  double m = part.empty() ? 0.0 : part[0];
+  for (double v : part)
+    m = v < m ? v : m;
+  const double n1 = s->N + 1;
+  *dt = cfl * m / (n1 * n1);
+  return BFAMX_OK;
+}
+
 extern "C" int bfamx_reduce_l2_diff(bfamx_subdomain_t *a, bfamx_subdomain_t *b,
                                     double *diff2, double *norm2)
 {

@@ -4,6 +4,8 @@ own host setup (operators, brick meshes, PCG32 materials, Ricker sources).
 Config names:
   acoustic2d_plane_wave   16x16 periodic quads, N=4, plane wave, 100 KC54 steps
   elastic3d_ricker        48^3 periodic hexes, N=7, PCG32 material, Ricker
+  elastic3d_octree        16^3 forest refined 2 levels inside a sphere, 2:1
+                          balanced, N = 7, 5, 3 by level, mortar glue
   elastic3d_mms           32^3 periodic hexes, N=7, manufactured solution
 The time step used everywhere is dt = CFL * h_min / (c_max (N+1)^2); the
 reference has no CFL rule, so this one is stated here and used by the
@@ -176,6 +178,59 @@ def elastic3d_mms(n=32, N=7, cfl=0.5, rho=1.0, cp=math.sqrt(3.0), cs=1.0, omega=
     return {"mesh": mesh, "rho": np.full(K, rho), "cp": np.full(K, cp), "cs": np.full(K, cs),
             "dt": dt, "k": k, "omega": omega, "F": F, "A": MMS_A.copy(),
             "lam": lam, "mu": mu, "physics": bx.ELASTIC3D}
+
+
+class SubMesh:
+    """One subdomain of an adapted mesh: order N, its elements' geometry,
+    in-subdomain maps (glued faces map to themselves) and glued flags."""
+
+    def __init__(self, d):
+        self.dim, self.N, self.K = 3, d["N"], d["K"]
+        self.EToE, self.EToF, self.glued = d["EToE"], d["EToF"], d["glued"]
+        self.x0, self.h, self.gid = d["x0"], d["h"], d["gid"]
+        self.r, self.w, self.Dr = ops(self.N)
+        self.vmapM, self.vmapP = bx.build_vmaps(3, self.N, self.EToE, self.EToF)
+        self.Np = (self.N + 1) ** 3
+
+    def coords(self):
+        return Mesh.coords(self)
+
+
+def elastic3d_octree(n_base=16, levels=2, seed=2024, radius=0.2, orders=(7, 5, 3),
+                     cfl=0.5, nsteps=200, material="pcg"):
+    """Config: periodic unit cube of n_base^3 trees, leaves meeting a seeded
+    sphere refined `levels` times, 2:1 face balanced; leaves of level l have
+    order orders[l]; one subdomain per order, glued by mortars.  Material per
+    element from PCG32 in global (Morton) element order, or homogeneous
+    (material = (rho, cp, cs)); Ricker source at a seeded point."""
+    g = bx.PCG32(seed, 13)
+    center = [g.uniform(0.3, 0.7) for _ in range(3)]
+    o = bx.octree(n_base, levels, center, radius, list(orders)[:levels + 1])
+    subs = [SubMesh(d) for d in o["subs"]]
+    if material == "pcg":
+        mat = pcg_material(o["K"], seed)
+    else:
+        mat = np.tile(np.asarray(material, dtype=float), (o["K"], 1))
+    rho = [mat[sm.gid, 0].copy() for sm in subs]
+    cp = [mat[sm.gid, 1].copy() for sm in subs]
+    cs = [mat[sm.gid, 2].copy() for sm in subs]
+    dt = min(cfl_dt(float(sm.h.min()), float(c.max()), sm.N, cfl) for sm, c in zip(subs, cp))
+    T = nsteps * dt
+    xs = np.array([g.uniform(0.25, 0.75) for _ in range(3)])
+    src = None
+    for si, sm in enumerate(subs):
+        inside = np.all((sm.x0 <= xs) & (xs < sm.x0 + sm.h), axis=1)
+        if inside.any():
+            e = int(np.nonzero(inside)[0][0])
+            rs = 2.0 * (xs - sm.x0[e]) / sm.h[e] - 1.0
+            src = {"sub": si, "elem": e, "weights": point_source_weights(sm, e, rs),
+                   "amp": [0, 0, 0, 1, 1, 1, 0, 0, 0], "t0": 0.5 * T, "f0": 2.4 / T,
+                   "x": xs.tolist()}
+            break
+    return {"octree": o, "subs": subs, "mortars": o["mortars"], "rho": rho, "cp": cp,
+            "cs": cs, "dt": dt, "nsteps": nsteps, "src": src, "center": center,
+            "radius": radius, "K_level": o["K_level"], "physics": bx.ELASTIC3D,
+            "dof": sum(9 * sm.K * sm.Np for sm in subs)}
 
 
 def upload(dev, cfg, precision):

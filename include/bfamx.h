@@ -236,6 +236,9 @@ int bfamx_step(bfamx_subdomain_t *sub, int nsteps);
 
 /* FP64 reductions with compensated sums */
 int bfamx_reduce_energy(bfamx_subdomain_t *sub, double *energy);
+/* stable step dt = cfl * min_e(min_d h_e,d / cp_e) / (N+1)^2 (FP64 device
+ * reduction; the reference has no CFL rule, this one is the build's) */
+int bfamx_reduce_cfl_dt(bfamx_subdomain_t *sub, double cfl, double *dt);
 /* sum J w |qa - qb|^2 and sum J w |qb|^2 over all fields */
 int bfamx_reduce_l2_diff(bfamx_subdomain_t *a, bfamx_subdomain_t *b,
                          double *diff2, double *norm2);
@@ -266,6 +269,49 @@ int bfamx_timer_stop(bfamx_device_t *dev, float *ms);
 /* launch tuning: blocks remapped so each XCD walks a contiguous element
  * range (1, default) or hardware round-robin order (0) */
 int bfamx_set_tuning(bfamx_device_t *dev, int xcd_remap);
+
+/* ---------------------------------------------------------------------- */
+/* adapted meshes: brick forest refined inside a sphere, 2:1 face balanced, */
+/* one subdomain per LGL order, and the mortars that glue them              */
+/* (replaces the p4est mesh + split_dgx_subdomains + glue construction,     */
+/* bfam.c:4684-4796, 5083-5165, 7328-7520, for aligned trees)               */
+/* ---------------------------------------------------------------------- */
+typedef struct
+{
+  int n_base;            /* trees per side of the periodic unit cube */
+  int levels;            /* refinement levels, 0..3 */
+  double center[3];      /* leaves meeting this sphere are refined */
+  double radius;
+  int order_by_level[4]; /* LGL order of the leaves of each level */
+} bfamx_octree_desc_t;
+
+/* one interface piece and both its sides (the reference keeps the same as
+ * two glue elements, EToEm/EToFm/EToHm and EToEp/EToHp/EToOp, bfam.h:1319):
+ * face s_face of element s_elem of subdomain s_sub coincides with the
+ * mortar; face b_face of (b_sub, b_elem) is the same face (quad = -1, the
+ * orders differ) or the big face of a 2:1 hanging interface, the mortar
+ * being its quadrant quad (bit 0: upper half along the face's first
+ * tangential axis, bit 1: along the second) */
+typedef struct
+{
+  int32_t s_sub, s_elem, b_sub, b_elem;
+  int8_t s_face, b_face, quad, pad;
+} bfamx_mortar_t;
+
+typedef struct bfamx_octree bfamx_octree_t;
+int bfamx_octree_build(const bfamx_octree_desc_t *desc, bfamx_octree_t **out);
+int bfamx_octree_free(bfamx_octree_t *o);
+/* K_level: 4 counts of leaves per level */
+int bfamx_octree_info(const bfamx_octree_t *o, int32_t *K, int *nsub,
+                      int32_t *nmortar, int32_t *K_level);
+/* subdomain s (increasing order N): element-local EToE/EToF (K*6, glued faces
+ * map to themselves), glued face flags (K*6), lower corners and edge lengths
+ * (K*3), and each element's index in the whole mesh */
+int bfamx_octree_sub(const bfamx_octree_t *o, int s, int *N, int32_t *K,
+                     int32_t *EToE, int8_t *EToF, int8_t *glued, double *x0,
+                     double *h, int32_t *gid);
+/* the mortars, sorted by big side */
+int bfamx_octree_mortars(const bfamx_octree_t *o, bfamx_mortar_t *m);
 
 /* ---------------------------------------------------------------------- */
 /* serving: independent elastic solves admitted under an HBM budget as they */

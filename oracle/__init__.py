@@ -33,7 +33,14 @@ class Problem(C.Structure):
         ("src_amp", C.c_double * 9), ("src_f0", C.c_double), ("src_t0", C.c_double),
         ("mms", C.c_int), ("mms_k", C.c_double), ("mms_omega", C.c_double),
         ("mms_F", C.c_double * 9),
+        ("glued", C.c_void_p),
     ]
+
+
+class Mortar(C.Structure):
+    _fields_ = [("s_sub", C.c_int32), ("s_elem", C.c_int32), ("b_sub", C.c_int32),
+                ("b_elem", C.c_int32), ("s_face", C.c_int8), ("b_face", C.c_int8),
+                ("quad", C.c_int8), ("pad", C.c_int8)]
 
 
 def build():
@@ -236,7 +243,7 @@ class OracleProblem:
     """
 
     def __init__(self, physics, N, vmapM, vmapP, h, rho, lam, mu=None, x0=None,
-                 src=None, mms=None, ops=None):
+                 src=None, mms=None, ops=None, glued=None):
         dim = 2 if physics == ACOUSTIC2D else 3
         self.dim, self.N, self.physics = dim, N, physics
         self.Np = (N + 1) ** dim
@@ -274,6 +281,9 @@ class OracleProblem:
             p.mms_k, p.mms_omega = float(mms["k"]), float(mms["omega"])
             for f, a in enumerate(mms["F"]):
                 p.mms_F[f] = float(a)
+        if glued is not None:
+            self._glued = np.ascontiguousarray(glued, dtype=np.int8).ravel()
+            p.glued = self._glued.ctypes.data
         self.p = p
 
     def zeros(self):
@@ -301,3 +311,42 @@ class OracleProblem:
         lib().orc_mms_exact(C.byref(self.p), np.ascontiguousarray(A, dtype=np.float64), t,
                             q.ravel())
         return q
+
+
+class OracleMulti:
+    """Elastic subdomains (OracleProblem, each built with its glued-face
+    flags) coupled by mortars (records of bfamx_mortar_t layout)."""
+
+    def __init__(self, problems, mortars):
+        self.problems = list(problems)
+        m = np.ascontiguousarray(mortars)
+        self._m = (Mortar * max(len(m), 1))()
+        for i, r in enumerate(m):
+            self._m[i] = Mortar(*[int(r[k]) for k in ("s_sub", "s_elem", "b_sub", "b_elem",
+                                                     "s_face", "b_face", "quad")], 0)
+        self.nm = len(m)
+        self._subs = (C.POINTER(Problem) * len(self.problems))(
+            *[C.pointer(P.p) for P in self.problems])
+
+    @staticmethod
+    def _rows(arrs):
+        return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+    def rhs(self, qs, t=0.0):
+        qs = [np.ascontiguousarray(q, dtype=np.float64) for q in qs]
+        out = [np.zeros_like(q) for q in qs]
+        lib().orc_rhs_multi(self._subs, len(qs), self._m, self.nm, C.c_double(t),
+                            self._rows(qs), self._rows(out))
+        return out
+
+    def run(self, qs, dt, nsteps, t0=0.0, method=LSRK_KC54):
+        qs = [np.ascontiguousarray(q, dtype=np.float64).copy() for q in qs]
+        dq = [np.zeros_like(q) for q in qs]
+        work = [np.zeros_like(q) for q in qs]
+        lib().orc_lsrk_run_multi(self._subs, len(qs), self._m, self.nm, method,
+                                 C.c_double(t0), C.c_double(dt), nsteps, self._rows(qs),
+                                 self._rows(dq), self._rows(work))
+        return qs
+
+    def energy(self, qs):
+        return sum(P.energy(q) for P, q in zip(self.problems, qs))

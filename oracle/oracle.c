@@ -982,6 +982,24 @@ static void riemann(const double *n, const double *vm, const double *Tm,
   }
 }
 
+/* strong-form lift of the flux fluctuation (dv, dT) at volume node nm:
+ * rates of v get dT / rho, stresses get the constitutive image of dv (x) n */
+static void lift_elastic(double *const *r, int nm, double lift, double rho,
+                         double lam, double mu, const double *nrm,
+                         const double *dvv, const double *dT)
+{
+  const double dvn = dvv[0] * nrm[0] + dvv[1] * nrm[1] + dvv[2] * nrm[2];
+  r[0][nm] += lift * dT[0] / rho;
+  r[1][nm] += lift * dT[1] / rho;
+  r[2][nm] += lift * dT[2] / rho;
+  r[3][nm] += lift * (lam * dvn + 2 * mu * dvv[0] * nrm[0]);
+  r[4][nm] += lift * (lam * dvn + 2 * mu * dvv[1] * nrm[1]);
+  r[5][nm] += lift * (lam * dvn + 2 * mu * dvv[2] * nrm[2]);
+  r[6][nm] += lift * mu * (dvv[0] * nrm[1] + dvv[1] * nrm[0]);
+  r[7][nm] += lift * mu * (dvv[0] * nrm[2] + dvv[2] * nrm[0]);
+  r[8][nm] += lift * mu * (dvv[1] * nrm[2] + dvv[2] * nrm[1]);
+}
+
 static void rhs_elastic3d(const orc_problem_t *p, double t, const double *q,
                           double *rhs)
 {
@@ -1043,6 +1061,8 @@ static void rhs_elastic3d(const orc_problem_t *p, double t, const double *q,
     const double Zpm = sqrt(rho * (lam + 2 * mu)), Zsm = sqrt(rho * mu);
     for (int f = 0; f < 6; ++f)
     {
+      if (p->glued && p->glued[6 * (size_t)e + f])
+        continue; /* flux from the mortars */
       const int a = f / 2;
       double nrm[3] = {0, 0, 0};
       nrm[a] = (f % 2) ? 1.0 : -1.0;
@@ -1075,16 +1095,7 @@ static void rhs_elastic3d(const orc_problem_t *p, double t, const double *q,
         }
         double dvv[3], dT[3];
         riemann(nrm, vm, Tm, Zpm, Zsm, vp, Tp, Zpp, Zsp, dvv, dT);
-        const double dvn = dvv[0] * nrm[0] + dvv[1] * nrm[1] + dvv[2] * nrm[2];
-        r[0][nm] += lift * dT[0] / rho;
-        r[1][nm] += lift * dT[1] / rho;
-        r[2][nm] += lift * dT[2] / rho;
-        r[3][nm] += lift * (lam * dvn + 2 * mu * dvv[0] * nrm[0]);
-        r[4][nm] += lift * (lam * dvn + 2 * mu * dvv[1] * nrm[1]);
-        r[5][nm] += lift * (lam * dvn + 2 * mu * dvv[2] * nrm[2]);
-        r[6][nm] += lift * mu * (dvv[0] * nrm[1] + dvv[1] * nrm[0]);
-        r[7][nm] += lift * mu * (dvv[0] * nrm[2] + dvv[2] * nrm[0]);
-        r[8][nm] += lift * mu * (dvv[1] * nrm[2] + dvv[2] * nrm[1]);
+        lift_elastic(r, nm, lift, rho, lam, mu, nrm, dvv, dT);
       }
     }
 
@@ -1260,4 +1271,243 @@ double orc_energy(const orc_problem_t *p, const double *q)
     }
   }
   return sum;
+}
+
+/* ------------------------------------------------------------------------ */
+/* subdomains glued by mortars.  The reference builds the glue operators but
+ * has no kernel that applies them; the use restated here follows their
+ * wiring in bfam_subdomain_dgx_glue_init (bfam.c:7381-7414): each side's
+ * trace is interpolated to the glue grid of order N_g = max(N_m, N_p) with
+ * interpolation{0,3,4} = prj{0,3,4}(N_m -> N_g) (the hanging halves of the
+ * big face by refine-to-bottom / -top along each face axis), the upwind flux
+ * is evaluated on the glue grid, and the result returns to each side's face
+ * nodes with the mass projection prj{0,1,2}(N_g -> N_m) scaled by the inverse
+ * LGL weight (wi_mass_prj, bfam.c:5543-5586), so that the LGL lift of the
+ * face values reproduces the exact surface integral of the glue flux. */
+
+typedef struct
+{
+  int ready;
+  double prj[5 * 81];
+  double wmp[5 * 81];
+  int null0;
+} orc_ops_pair_t;
+
+static const double *ops_block(orc_ops_pair_t (*cache)[9], int Na, int Nb,
+                               int which, int k)
+{
+  orc_ops_pair_t *c = &cache[Na][Nb];
+  if (!c->ready)
+  {
+    double mp[5 * 81];
+    orc_interpolator(Na, Nb, c->prj, mp, c->wmp, &c->null0);
+    if (c->null0)
+    {
+      const int n = Na + 1;
+      for (int i = 0; i < n * n; ++i)
+        c->prj[i] = 0;
+      for (int i = 0; i < n; ++i)
+        c->prj[i + i * n] = 1;
+    }
+    c->ready = 1;
+  }
+  const size_t blk = (size_t)(Na + 1) * (Nb + 1);
+  return (which ? c->wmp : c->prj) + k * blk;
+}
+
+/* out[g + ng*h] = sum_{a,b} X[g + ng*a] Y[h + ng*b] u[a + nu*b] */
+static void tensor2(int ng, int nu, const double *X, const double *Y,
+                    const double *u, double *out)
+{
+  for (int h = 0; h < ng; ++h)
+    for (int g = 0; g < ng; ++g)
+    {
+      double acc = 0;
+      for (int b = 0; b < nu; ++b)
+      {
+        double row = 0;
+        for (int a = 0; a < nu; ++a)
+          row += X[g + ng * a] * u[a + nu * b];
+        acc += Y[h + ng * b] * row;
+      }
+      out[g + ng * h] = acc;
+    }
+}
+
+static void face_trace(const orc_problem_t *p, const double *q, int32_t e,
+                       int f, double sgn, const int32_t *gm, double (*tr)[81])
+{
+  const int n1 = p->N + 1, Np = n1 * n1 * n1, nfp = n1 * n1, A = f / 2;
+  const size_t KNp = (size_t)p->K * Np;
+  static const int sf[3][3] = {{3, 6, 7}, {6, 4, 8}, {7, 8, 5}};
+  for (int n = 0; n < nfp; ++n)
+  {
+    const size_t id = (size_t)e * Np + gm[f * nfp + n];
+    for (int c = 0; c < 3; ++c)
+    {
+      tr[c][n] = q[c * KNp + id];
+      tr[3 + c][n] = sgn * q[sf[c][A] * KNp + id];
+    }
+  }
+}
+
+static void mortar_flux(const orc_problem_t *const *subs, const orc_mortar_t *m,
+                        orc_ops_pair_t (*cache)[9], const double *const *q,
+                        double *const *rhs)
+{
+  const orc_problem_t *Ps = subs[m->s_sub], *Pb = subs[m->b_sub];
+  const int Ns = Ps->N, Nb = Pb->N, Ng = ORC_MAX(Ns, Nb);
+  const int ns1 = Ns + 1, nb1 = Nb + 1, ng1 = Ng + 1;
+  const int A = m->s_face / 2;
+  const double sg = (m->s_face & 1) ? 1.0 : -1.0; /* n_s = sg e_A, n_b = -n_s */
+  int32_t gs[6 * 81], gb[6 * 81];
+  orc_gmask_faces(3, Ns, gs);
+  orc_gmask_faces(3, Nb, gb);
+  double ts[6][81], tb[6][81], ms[6][81], mb[6][81];
+  face_trace(Ps, q[m->s_sub], m->s_elem, m->s_face, sg, gs, ts);
+  face_trace(Pb, q[m->b_sub], m->b_elem, m->b_face, sg, gb, tb);
+  const int qa = m->quad < 0 ? 0 : 3 + (m->quad & 1);
+  const int qb = m->quad < 0 ? 0 : 3 + ((m->quad >> 1) & 1);
+  const double *Is = ops_block(cache, Ns, Ng, 0, 0);
+  const double *Ia = ops_block(cache, Nb, Ng, 0, qa);
+  const double *Ib = ops_block(cache, Nb, Ng, 0, qb);
+  for (int c = 0; c < 6; ++c)
+  {
+    tensor2(ng1, ns1, Is, Is, ts[c], ms[c]);
+    tensor2(ng1, nb1, Ia, Ib, tb[c], mb[c]);
+  }
+  const int32_t es = m->s_elem, eb = m->b_elem;
+  const double rs = Ps->rho[es], ls = Ps->lam[es], us = Ps->mu[es];
+  const double rb = Pb->rho[eb], lb = Pb->lam[eb], ub = Pb->mu[eb];
+  const double Zps = sqrt(rs * (ls + 2 * us)), Zss = sqrt(rs * us);
+  const double Zpb = sqrt(rb * (lb + 2 * ub)), Zsb = sqrt(rb * ub);
+  double ns_[3] = {0, 0, 0}, nb_[3] = {0, 0, 0};
+  ns_[A] = sg;
+  nb_[A] = -sg;
+  double fs[6][81], fb[6][81];
+  for (int g = 0; g < ng1 * ng1; ++g)
+  {
+    double vs[3], Ts[3], vb[3], Tb[3], Tsn[3], Tbn[3], dv[3], dT[3];
+    for (int c = 0; c < 3; ++c)
+    {
+      vs[c] = ms[c][g];
+      Ts[c] = ms[3 + c][g];
+      vb[c] = mb[c][g];
+      Tb[c] = mb[3 + c][g];
+      Tsn[c] = -Ts[c];
+      Tbn[c] = -Tb[c];
+    }
+    riemann(ns_, vs, Ts, Zps, Zss, vb, Tb, Zpb, Zsb, dv, dT);
+    for (int c = 0; c < 3; ++c)
+    {
+      fs[c][g] = dv[c];
+      fs[3 + c][g] = dT[c];
+    }
+    riemann(nb_, vb, Tbn, Zpb, Zsb, vs, Tsn, Zps, Zss, dv, dT);
+    for (int c = 0; c < 3; ++c)
+    {
+      fb[c][g] = dv[c];
+      fb[3 + c][g] = dT[c];
+    }
+  }
+  const int pa = m->quad < 0 ? 0 : 1 + (m->quad & 1);
+  const int pb = m->quad < 0 ? 0 : 1 + ((m->quad >> 1) & 1);
+  const double *Ws = ops_block(cache, Ng, Ns, 1, 0);
+  const double *Wa = ops_block(cache, Ng, Nb, 1, pa);
+  const double *Wb = ops_block(cache, Ng, Nb, 1, pb);
+  double Fs[6][81], Fb[6][81];
+  for (int c = 0; c < 6; ++c)
+  {
+    /* W is (N+1) x (Ng+1): tensor2 with the roles of the sizes swapped */
+    for (int j = 0; j < ns1; ++j)
+      for (int i = 0; i < ns1; ++i)
+      {
+        double acc = 0;
+        for (int h = 0; h < ng1; ++h)
+          for (int g = 0; g < ng1; ++g)
+            acc += Ws[i + ns1 * g] * Ws[j + ns1 * h] * fs[c][g + ng1 * h];
+        Fs[c][i + ns1 * j] = acc;
+      }
+    for (int j = 0; j < nb1; ++j)
+      for (int i = 0; i < nb1; ++i)
+      {
+        double acc = 0;
+        for (int h = 0; h < ng1; ++h)
+          for (int g = 0; g < ng1; ++g)
+            acc += Wa[i + nb1 * g] * Wb[j + nb1 * h] * fb[c][g + ng1 * h];
+        Fb[c][i + nb1 * j] = acc;
+      }
+  }
+  for (int side = 0; side < 2; ++side)
+  {
+    const orc_problem_t *P = side ? Pb : Ps;
+    const int32_t e = side ? eb : es;
+    const int f = side ? m->b_face : m->s_face;
+    const int n1 = P->N + 1, Np = n1 * n1 * n1, nfp = n1 * n1;
+    const int32_t *gm = side ? gb : gs;
+    const size_t KNp = (size_t)P->K * Np;
+    double *r[9];
+    for (int c = 0; c < 9; ++c)
+      r[c] = rhs[side ? m->b_sub : m->s_sub] + c * KNp + (size_t)e * Np;
+    const double lift = (2 / P->h[3 * e + A]) / P->w[0];
+    const double *nrm = side ? nb_ : ns_;
+    double(*F)[81] = side ? Fb : Fs;
+    for (int n = 0; n < nfp; ++n)
+    {
+      const double dv[3] = {F[0][n], F[1][n], F[2][n]};
+      const double dT[3] = {F[3][n], F[4][n], F[5][n]};
+      lift_elastic(r, gm[f * nfp + n], lift, P->rho[e], P->lam[e], P->mu[e], nrm,
+                   dv, dT);
+    }
+  }
+}
+
+void orc_rhs_multi(const orc_problem_t *const *subs, int nsub,
+                   const orc_mortar_t *m, int32_t nm, double t,
+                   const double *const *q, double *const *rhs)
+{
+  for (int s = 0; s < nsub; ++s)
+    orc_rhs(subs[s], t, q[s], rhs[s]);
+  orc_ops_pair_t(*cache)[9] = calloc(9, sizeof(*cache));
+  for (int32_t i = 0; i < nm; ++i)
+    mortar_flux(subs, &m[i], cache, q, rhs);
+  free(cache);
+}
+
+void orc_lsrk_run_multi(const orc_problem_t *const *subs, int nsub,
+                        const orc_mortar_t *m, int32_t nm, int method,
+                        double t0, double dt, int nsteps, double *const *q,
+                        double *const *dq, double *const *work)
+{
+  double A[6], B[6], C[6];
+  const int ns = orc_lsrk_table(method, A, B, C);
+  size_t *n = malloc(sizeof(size_t) * (size_t)(nsub ? nsub : 1));
+  for (int s = 0; s < nsub; ++s)
+  {
+    const size_t n1 = (size_t)subs[s]->N + 1;
+    n[s] = (size_t)orc_nfields(subs[s]->physics) * subs[s]->K * n1 * n1 * n1;
+    for (size_t i = 0; i < n[s]; ++i)
+      dq[s][i] = 0;
+  }
+  double t = t0;
+  for (int step = 0; step < nsteps; ++step)
+  {
+    for (int st = 0; st < ns; ++st)
+    {
+      orc_rhs_multi(subs, nsub, m, nm, t + C[st] * dt, (const double *const *)q, work);
+      for (int s = 0; s < nsub; ++s)
+      {
+        double *qq = q[s], *d = dq[s];
+        const double *w = work[s];
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < n[s]; ++i)
+        {
+          d[i] = A[st] * d[i] + dt * w[i];
+          qq[i] = qq[i] + B[st] * d[i];
+        }
+      }
+    }
+    t += dt;
+  }
+  free(n);
 }

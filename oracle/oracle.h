@@ -140,6 +140,8 @@ typedef struct
   int mms;
   double mms_k, mms_omega;
   double mms_F[9];
+  /* K*6 flags (elastic): faces whose flux comes from mortars; NULL: none */
+  const int8_t *glued;
 } orc_problem_t;
 
 int orc_nfields(int physics);
@@ -148,6 +150,24 @@ void orc_rhs(const orc_problem_t *p, double t, const double *q, double *rhs);
 /* nsteps of the given LSRK method; dq and work are scratch of the same size */
 void orc_lsrk_run(const orc_problem_t *p, int method, double t0, double dt,
                   int nsteps, double *q, double *dq, double *work);
+/* ---- several elastic subdomains glued by mortars ---- */
+/* one interface piece: face s_face of (s_sub, s_elem) coincides with the
+ * mortar; face b_face of (b_sub, b_elem) is the same face (quad = -1) or the
+ * big face of a 2:1 hanging interface of which the mortar is quadrant quad
+ * (bit 0: upper half along the first tangential axis, bit 1: second) */
+typedef struct
+{
+  int32_t s_sub, s_elem, b_sub, b_elem;
+  int8_t s_face, b_face, quad, pad;
+} orc_mortar_t;
+/* q[s], rhs[s]: the field blocks of subdomain s */
+void orc_rhs_multi(const orc_problem_t *const *subs, int nsub,
+                   const orc_mortar_t *m, int32_t nm, double t,
+                   const double *const *q, double *const *rhs);
+void orc_lsrk_run_multi(const orc_problem_t *const *subs, int nsub,
+                        const orc_mortar_t *m, int32_t nm, int method,
+                        double t0, double dt, int nsteps, double *const *q,
+                        double *const *dq, double *const *work);
 /* 0.5 * sum J w (rho|v|^2 + sigma : C^{-1} : sigma), FP64 with Kahan sums */
 double orc_energy(const orc_problem_t *p, const double *q);
 /* manufactured solution at time t, and the forcing coefficients */

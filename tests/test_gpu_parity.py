@@ -214,3 +214,18 @@ def test_full_size_fp32_accuracy(dev):
         s.close()
     assert n2 > 0
     assert e32 < 1e-5 and e32r < 1e-5
+
+
+@pytest.mark.parametrize("phys", ["elastic", "acoustic"])
+def test_cfl_dt_reduction(dev, phys):
+    if phys == "elastic":
+        cfg, _ = random_elastic((3, 2, 4), 5, seed=2)
+    else:
+        cfg, _ = random_acoustic((4, 3), 4, seed=2)
+    m = cfg["mesh"]
+    sub = W.upload(dev, cfg, bx.FP32)
+    dt = sub.cfl_dt(0.5)
+    sub.close()
+    want = 0.5 * np.min(m.h.min(axis=1) / cfg["cp"]) / (m.N + 1) ** 2
+    assert abs(dt - want) <= 1e-15 * want
+    assert abs(dt - cfg["dt"]) <= 1e-15 * want
