@@ -144,6 +144,10 @@ SIGNATURES = {
     "bfamx_octree_sub": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int32),
                                    _vp, _vp, _vp, _vp, _vp, _vp]),
     "bfamx_octree_mortars": (C.c_int, [_vp, _vp]),
+    "bfamx_glue_upload": (C.c_int, [_vp, _PP, C.c_int, _vp, C.c_int32, _PP]),
+    "bfamx_glue_free": (C.c_int, [_vp]),
+    "bfamx_batch_set_glue": (C.c_int, [_vp, _vp]),
+    "bfamx_batch_rhs": (C.c_int, [_vp, C.c_double, _vp]),
     "bfamx_serve_generate": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(Request)]),
@@ -464,6 +468,36 @@ class Batch:
 
     def launch_graph(self):
         _check(lib().bfamx_batch_launch_graph(self.h))
+
+    def set_glue(self, glue):
+        _check(lib().bfamx_batch_set_glue(self.h, glue.h if glue is not None else None))
+        self.glue = glue
+
+    def rhs(self, t=0.0):
+        """L(q, t) of every member: list of (nfields, K*Np) arrays"""
+        outs = [np.zeros((s.nf, s.K * s.Np)) for s in self.subs]
+        rows = [_ptrs([o[f] for f in range(o.shape[0])]) for o in outs]
+        arr = (C.c_void_p * len(rows))(*[C.cast(r, C.c_void_p) for r in rows])
+        _check(lib().bfamx_batch_rhs(self.h, float(t), arr))
+        return outs
+
+
+class Glue:
+    """Mortars between subdomains on the device (bfamx_glue_upload)."""
+
+    def __init__(self, dev, subs, mortars):
+        m = np.ascontiguousarray(mortars, dtype=MORTAR_DTYPE)
+        self.subs = list(subs)
+        arr = (C.c_void_p * len(self.subs))(*[s.h for s in self.subs])
+        h = C.c_void_p()
+        _check(lib().bfamx_glue_upload(dev.h, arr, len(self.subs),
+                                       m.ctypes.data if len(m) else None, len(m), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(lib().bfamx_glue_free(self.h))
+            self.h = None
 
 
 # ------------------------------------------------------------- adapted mesh

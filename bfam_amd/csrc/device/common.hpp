@@ -119,12 +119,52 @@ struct SubDev
   const double *mms_tab; /* K x 3 axes x {sin, cos} x (N+1): sin/cos(k x) */
   const float *mms_tab_f;
   Clock *clock;
+  /* glued faces (nbr_code & NBR_GLUED): nbr_elem is the offset of the
+   * face's 6 x (N+1)^2 block in this buffer (arithmetic type), written by
+   * the mortar kernel before each stage: dv[3], dT[3] per face node */
+  void *glue_buf;
   double A[8], B[8], C[9];
   double src_amp[9];
   double src_f0, src_t0;
   double mms_k, mms_omega;
   double mms_F[9];
 };
+
+/* nbr_code flag of a face whose flux comes from the mortars */
+constexpr int NBR_GLUED = 0x80;
+
+/* glued faces are at most order 7: (N+1)^2 <= 64 nodes, one per lane */
+constexpr int GLUE_MAXFP = 64;
+
+/* one interface handled by one wavefront of the mortar kernel: the b face
+ * and its nm mortars (1: conforming, orders differ, quad0 = -1; 4: hanging,
+ * mortar m is quadrant m of the b face).  *_off: offsets of the faces'
+ * 6 x (N+1)^2 blocks in the glue buffer */
+struct MortarGroup
+{
+  int32_t b_sub, b_elem, b_off;
+  int32_t s_sub[4], s_elem[4], s_off[4];
+  int8_t b_face, nm, quad0, pad;
+  int8_t s_face[4];
+};
+
+struct GlueArgs
+{
+  const SubDev *subs; /* the batch's descriptors */
+  const MortarGroup *groups;
+  int ngroups;
+  const void *ops; /* glue_ops_index table, arithmetic type */
+  void *buf;       /* face fluctuation blocks */
+};
+
+/* 1-D mortar operators from order Na to Nb (1..8), column-major
+ * (Nb+1) x (Na+1) in blocks of 81: kind 0 = prj[k], kind 1 = wi_mass_prj[k],
+ * k = 0..4 as in create_interpolators */
+__host__ __device__ constexpr int glue_ops_index(int kind, int Na, int Nb, int k)
+{
+  return ((((Na - 1) * 8 + (Nb - 1)) * 2 + kind) * 5 + k) * 81;
+}
+constexpr int GLUE_OPS_SIZE = 64 * 2 * 5 * 81;
 
 /* one work item of a stage launch: elements [e0, e0+ne) of subdomain sub */
 struct Task

@@ -58,6 +58,7 @@ hipError_t launch_plain_update(const SubDev *sd_dev, size_t ndof,
                                hipStream_t s, int nblocks);
 /* advance the clocks of nsubs device descriptors by one stage */
 hipError_t launch_clock_advance(const SubDev *subs, int nsubs, hipStream_t s);
+hipError_t launch_mortar(int prec, const GlueArgs &a, hipStream_t s);
 
 /* reductions: per-element partial sums (FP64) of the energy density (K), or
  * interleaved pairs (2K) of |qa - qb|^2, |qb|^2 and of the error against the

@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../bfamx_internal.h"
@@ -53,6 +54,28 @@ struct bfamx_subdomain
   size_t bytes;
   int fields_set;
   bfamx_lsrk_t lsrk;
+  bfamx_glue_t *glue; /* set while faces of this subdomain are glued */
+};
+
+struct bfamx_glue
+{
+  bfamx_device_t *dev;
+  std::vector<bfamx_subdomain_t *> subs;
+  int prec;
+  MortarGroup *d_groups;
+  int ngroups;
+  void *d_ops;
+  void *d_buf;
+  std::vector<void *> allocs;
+  size_t bytes;
+  /* original face tables of the glued faces, restored by bfamx_glue_free */
+  struct saved_face
+  {
+    int sub;
+    int32_t idx, elem;
+    uint8_t code;
+  };
+  std::vector<saved_face> saved;
 };
 
 struct bfamx_batch
@@ -68,6 +91,7 @@ struct bfamx_batch
   hipGraph_t graph;
   hipGraphExec_t exec;
   int graph_steps;
+  bfamx_glue_t *glue;
 };
 
 #define HIPCHK(call)                                                           \
@@ -527,6 +551,8 @@ extern "C" int bfamx_subdomain_free(bfamx_subdomain_t *s)
 {
   if (!s)
     return BFAMX_OK;
+  if (s->glue)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_subdomain_free: free its glue first");
   (void)hipStreamSynchronize(s->dev->stream);
   dev_free_all(s->dev, s->allocs, s->bytes);
   delete s;
@@ -816,6 +842,8 @@ extern "C" int bfamx_stage(bfamx_subdomain_t *s)
     return bfamx_fail(BFAMX_EINVAL, "NULL subdomain");
   if (!s->fields_set)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_stage: no state set");
+  if (s->glue)
+    return bfamx_fail(BFAMX_ESTATE, "glued subdomains advance through their batch");
   return stage_once(s);
 }
 
@@ -825,6 +853,8 @@ extern "C" int bfamx_step(bfamx_subdomain_t *s, int nsteps)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_step: bad arguments");
   if (!s->fields_set)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_step: no state set");
+  if (s->glue)
+    return bfamx_fail(BFAMX_ESTATE, "glued subdomains advance through their batch");
   for (int i = 0; i < nsteps; ++i)
     for (int st = 0; st < s->h.nstages; ++st)
     {
@@ -835,12 +865,23 @@ extern "C" int bfamx_step(bfamx_subdomain_t *s, int nsteps)
   return BFAMX_OK;
 }
 
+static int rhs_one(bfamx_subdomain_t *s, double t, double *const *rhs);
+
 extern "C" int bfamx_rhs(bfamx_subdomain_t *s, double t, double *const *rhs)
 {
   if (!s || !rhs)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_rhs: NULL argument");
   if (!s->fields_set)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_rhs: no state set");
+  if (s->glue)
+    return bfamx_fail(BFAMX_ESTATE, "glued subdomains: use bfamx_batch_rhs");
+  return rhs_one(s, t, rhs);
+}
+
+/* RHS of one subdomain into host arrays; glued faces use the glue buffer as
+ * the last mortar launch left it */
+static int rhs_one(bfamx_subdomain_t *s, double t, double *const *rhs)
+{
   const size_t ndof = (size_t)s->K * s->nf * s->Np;
   int rc;
   if (!s->h.rhs)
@@ -1001,6 +1042,7 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
   b->graph = NULL;
   b->exec = NULL;
   b->graph_steps = 0;
+  b->glue = NULL;
   std::vector<SubDev> hs(nsubs);
   std::vector<Task> tasks;
   std::vector<int> per(nsubs);
@@ -1071,6 +1113,17 @@ extern "C" int bfamx_batch_free(bfamx_batch_t *b)
   return BFAMX_OK;
 }
 
+static GlueArgs glue_args(const bfamx_batch_t *b)
+{
+  GlueArgs g;
+  g.subs = b->d_subs;
+  g.groups = b->glue->d_groups;
+  g.ngroups = b->glue->ngroups;
+  g.ops = b->glue->d_ops;
+  g.buf = b->glue->d_buf;
+  return g;
+}
+
 static int batch_stage(bfamx_batch_t *b)
 {
   LaunchArgs a;
@@ -1082,9 +1135,13 @@ static int batch_stage(bfamx_batch_t *b)
   a.xcd_remap = b->dev->xcd_remap;
   a.nxcd = b->dev->num_xcd;
   const int grid = grid_for(b->dev, b->ntasks);
-  hipError_t e = b->physics == BFAMX_ELASTIC3D
-                     ? launch_elastic_line(b->prec, b->N, a, grid, b->dev->stream)
-                     : launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
+  hipError_t e = hipSuccess;
+  if (b->glue)
+    e = launch_mortar(b->prec, glue_args(b), b->dev->stream);
+  if (e == hipSuccess)
+    e = b->physics == BFAMX_ELASTIC3D
+            ? launch_elastic_line(b->prec, b->N, a, grid, b->dev->stream)
+            : launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
   if (e == hipSuccess)
     e = launch_clock_advance(b->d_subs, (int)b->subs.size(), b->dev->stream);
   if (e != hipSuccess)
@@ -1162,5 +1219,285 @@ extern "C" int bfamx_batch_launch_graph(bfamx_batch_t *b)
     if (!s->fields_set)
       return bfamx_fail(BFAMX_ESTATE, "batch member without state");
   HIPCHK(hipGraphLaunch(b->exec, b->dev->stream));
+  return BFAMX_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* glue: mortars between subdomains of one batch                           */
+/* ---------------------------------------------------------------------- */
+
+static int glue_release(bfamx_glue_t *g)
+{
+  /* restore the face tables of the glued faces */
+  for (const auto &f : g->saved)
+  {
+    bfamx_subdomain_t *s = g->subs[f.sub];
+    (void)hipMemcpy((void *)(s->h.nbr_elem + f.idx), &f.elem, 4, hipMemcpyHostToDevice);
+    (void)hipMemcpy((void *)(s->h.nbr_code + f.idx), &f.code, 1, hipMemcpyHostToDevice);
+  }
+  for (auto *s : g->subs)
+    if (s->glue == g)
+    {
+      s->glue = NULL;
+      s->h.glue_buf = NULL;
+      (void)push_sub(s);
+    }
+  dev_free_all(g->dev, g->allocs, g->bytes);
+  delete g;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_glue_free(bfamx_glue_t *g)
+{
+  if (!g)
+    return BFAMX_OK;
+  (void)hipStreamSynchronize(g->dev->stream);
+  return glue_release(g);
+}
+
+template <class T>
+static std::vector<T> glue_ops_table()
+{
+  std::vector<T> t(GLUE_OPS_SIZE, T(0));
+  double prj[5 * 81], mp[5 * 81], wmp[5 * 81];
+  for (int Na = 1; Na <= 8; ++Na)
+    for (int Nb = 1; Nb <= 8; ++Nb)
+    {
+      int ident = 0;
+      if (bfamx_mortar_ops(Na, Nb, prj, mp, wmp, &ident))
+        continue;
+      const int blk = (Na + 1) * (Nb + 1);
+      for (int k = 0; k < 5; ++k)
+        for (int i = 0; i < blk; ++i)
+        {
+          t[glue_ops_index(0, Na, Nb, k) + i] = (T)prj[k * blk + i];
+          t[glue_ops_index(1, Na, Nb, k) + i] = (T)wmp[k * blk + i];
+        }
+    }
+  return t;
+}
+
+extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *subs,
+                                 int nsubs, const bfamx_mortar_t *m, int32_t nm,
+                                 bfamx_glue_t **out)
+{
+  if (!dev || !subs || nsubs < 1 || nm < 0 || (nm && !m) || !out)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_glue_upload: bad arguments");
+  *out = NULL;
+  const int prec = subs[0]->prec;
+  for (int i = 0; i < nsubs; ++i)
+  {
+    const bfamx_subdomain_t *s = subs[i];
+    if (!s || s->dev != dev || s->physics != BFAMX_ELASTIC3D || s->prec != prec ||
+        prec == BFAMX_FP64_PLAIN || s->N < 3 || s->N > 7)
+      return bfamx_fail(BFAMX_EINVAL,
+                        "glue: elastic fused subdomains of one precision, orders 3..7");
+    if (s->glue)
+      return bfamx_fail(BFAMX_ESTATE, "glue: subdomain %d is already glued", i);
+  }
+  /* group the mortars by b face: one conforming mortar, or four quadrants */
+  std::vector<int32_t> ord(nm);
+  for (int32_t i = 0; i < nm; ++i)
+    ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+    const bfamx_mortar_t &a = m[x], &b = m[y];
+    if (a.b_sub != b.b_sub)
+      return a.b_sub < b.b_sub;
+    if (a.b_elem != b.b_elem)
+      return a.b_elem < b.b_elem;
+    if (a.b_face != b.b_face)
+      return a.b_face < b.b_face;
+    return a.quad < b.quad;
+  });
+  for (int32_t i = 0; i < nm; ++i)
+  {
+    const bfamx_mortar_t &r = m[i];
+    if (r.s_sub < 0 || r.s_sub >= nsubs || r.b_sub < 0 || r.b_sub >= nsubs ||
+        r.s_elem < 0 || r.s_elem >= subs[r.s_sub]->K || r.b_elem < 0 ||
+        r.b_elem >= subs[r.b_sub]->K || r.s_face < 0 || r.s_face > 5 ||
+        r.b_face != (r.s_face ^ 1) || r.quad < -1 || r.quad > 3)
+      return bfamx_fail(BFAMX_EINVAL, "glue: mortar %d is malformed", i);
+  }
+  std::vector<MortarGroup> groups;
+  /* face -> offset in the glue buffer; every glued face appears once */
+  std::vector<std::vector<int32_t>> off(nsubs);
+  for (int i = 0; i < nsubs; ++i)
+    off[i].assign((size_t)subs[i]->K * 6, -1);
+  size_t total = 0;
+  auto slot = [&](int sub, int32_t e, int f, int32_t *o) -> int {
+    int32_t &x = off[sub][(size_t)e * 6 + f];
+    if (x >= 0)
+      return bfamx_fail(BFAMX_EINVAL, "glue: face %d of element %d (subdomain %d) is "
+                        "named by more than one interface", f, e, sub);
+    const int n1 = subs[sub]->N + 1;
+    if (total + 6 * (size_t)n1 * n1 > (size_t)INT32_MAX)
+      return bfamx_fail(BFAMX_EINVAL, "glue: too many glued faces");
+    x = (int32_t)total;
+    total += 6 * (size_t)n1 * n1;
+    *o = x;
+    return BFAMX_OK;
+  };
+  int rc = BFAMX_OK;
+  for (int32_t i = 0; i < nm && !rc;)
+  {
+    const bfamx_mortar_t &r = m[ord[i]];
+    MortarGroup g;
+    memset(&g, 0, sizeof(g));
+    g.b_sub = r.b_sub;
+    g.b_elem = r.b_elem;
+    g.b_face = r.b_face;
+    int n = 1;
+    if (r.quad >= 0)
+    {
+      n = 4;
+      for (int k = 0; k < 4; ++k)
+      {
+        if (i + k >= nm)
+          return bfamx_fail(BFAMX_EINVAL, "glue: hanging face with fewer than 4 mortars");
+        const bfamx_mortar_t &q = m[ord[i + k]];
+        if (q.b_sub != r.b_sub || q.b_elem != r.b_elem || q.b_face != r.b_face ||
+            q.quad != k)
+          return bfamx_fail(BFAMX_EINVAL, "glue: hanging face without quadrants 0..3");
+      }
+    }
+    g.nm = (int8_t)n;
+    g.quad0 = n == 1 ? -1 : 0;
+    if ((rc = slot(r.b_sub, r.b_elem, r.b_face, &g.b_off)))
+      break;
+    for (int k = 0; k < n && !rc; ++k)
+    {
+      const bfamx_mortar_t &q = m[ord[i + k]];
+      g.s_sub[k] = q.s_sub;
+      g.s_elem[k] = q.s_elem;
+      g.s_face[k] = q.s_face;
+      rc = slot(q.s_sub, q.s_elem, q.s_face, &g.s_off[k]);
+    }
+    groups.push_back(g);
+    i += n;
+  }
+  if (rc)
+    return rc;
+  const int fp32 = prec != BFAMX_FP64;
+  bfamx_glue_t *gl = new bfamx_glue_t();
+  gl->dev = dev;
+  gl->subs.assign(subs, subs + nsubs);
+  gl->prec = prec;
+  gl->ngroups = (int)groups.size();
+  gl->bytes = 0;
+  void *p;
+  rc = dev_alloc(dev, gl->allocs, &gl->bytes, sizeof(MortarGroup) * (groups.size() + 1), &p);
+  if (!rc)
+  {
+    gl->d_groups = (MortarGroup *)p;
+    rc = dev_alloc(dev, gl->allocs, &gl->bytes, (fp32 ? 4 : 8) * (size_t)GLUE_OPS_SIZE, &p);
+  }
+  if (!rc)
+  {
+    gl->d_ops = p;
+    rc = dev_alloc(dev, gl->allocs, &gl->bytes, (fp32 ? 4 : 8) * (total + 1), &p);
+  }
+  if (rc)
+  {
+    dev_free_all(dev, gl->allocs, gl->bytes);
+    delete gl;
+    return rc;
+  }
+  gl->d_buf = p;
+  hipError_t he = hipMemcpy(gl->d_groups, groups.data(), sizeof(MortarGroup) * groups.size(),
+                            hipMemcpyHostToDevice);
+  if (he == hipSuccess)
+  {
+    if (fp32)
+    {
+      auto t = glue_ops_table<float>();
+      he = hipMemcpy(gl->d_ops, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
+    }
+    else
+    {
+      auto t = glue_ops_table<double>();
+      he = hipMemcpy(gl->d_ops, t.data(), 8 * t.size(), hipMemcpyHostToDevice);
+    }
+  }
+  if (he == hipSuccess)
+    he = hipMemset(gl->d_buf, 0, (fp32 ? 4 : 8) * (total + 1));
+  /* flag the glued faces in the subdomains' face tables */
+  for (int i = 0; i < nsubs && he == hipSuccess; ++i)
+  {
+    bfamx_subdomain_t *s = subs[i];
+    const size_t nfc = (size_t)s->K * 6;
+    std::vector<int32_t> ne(nfc);
+    std::vector<uint8_t> nc(nfc);
+    he = hipMemcpy(ne.data(), s->h.nbr_elem, 4 * nfc, hipMemcpyDeviceToHost);
+    if (he == hipSuccess)
+      he = hipMemcpy(nc.data(), s->h.nbr_code, nfc, hipMemcpyDeviceToHost);
+    if (he != hipSuccess)
+      break;
+    for (size_t k = 0; k < nfc; ++k)
+      if (off[i][k] >= 0)
+      {
+        gl->saved.push_back({i, (int32_t)k, ne[k], nc[k]});
+        ne[k] = off[i][k];
+        nc[k] = (uint8_t)NBR_GLUED;
+      }
+    he = hipMemcpy((void *)s->h.nbr_elem, ne.data(), 4 * nfc, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+      he = hipMemcpy((void *)s->h.nbr_code, nc.data(), nfc, hipMemcpyHostToDevice);
+    s->glue = gl;
+    s->h.glue_buf = gl->d_buf;
+    if (he == hipSuccess && push_sub(s))
+      he = hipErrorUnknown;
+  }
+  if (he != hipSuccess)
+  {
+    glue_release(gl);
+    return bfamx_fail(BFAMX_EHIP, "glue upload failed: %s", hipGetErrorString(he));
+  }
+  *out = gl;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_batch_set_glue(bfamx_batch_t *b, bfamx_glue_t *g)
+{
+  if (!b)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_set_glue: NULL batch");
+  if (g)
+  {
+    if (g->subs.size() != b->subs.size())
+      return bfamx_fail(BFAMX_EINVAL, "glue and batch must hold the same subdomains");
+    for (size_t i = 0; i < b->subs.size(); ++i)
+      if (g->subs[i] != b->subs[i])
+        return bfamx_fail(BFAMX_EINVAL, "glue and batch must list the subdomains in "
+                          "the same order");
+  }
+  if (b->exec)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_set_glue: batch already captured");
+  b->glue = g;
+  /* refresh the batch's descriptor copies (glue buffer, face flags) */
+  std::vector<SubDev> hs(b->subs.size());
+  for (size_t i = 0; i < b->subs.size(); ++i)
+    hs[i] = b->subs[i]->h;
+  HIPCHK(hipMemcpy(b->d_subs, hs.data(), sizeof(SubDev) * hs.size(), hipMemcpyHostToDevice));
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_batch_rhs(bfamx_batch_t *b, double t, double *const *const *rhs)
+{
+  if (!b || !rhs)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_rhs: bad arguments");
+  for (auto *s : b->subs)
+    if (!s->fields_set)
+      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  if (b->glue)
+  {
+    hipError_t e = launch_mortar(b->prec, glue_args(b), b->dev->stream);
+    if (e != hipSuccess)
+      return bfamx_fail(BFAMX_EHIP, "mortar launch: %s", hipGetErrorString(e));
+  }
+  for (size_t i = 0; i < b->subs.size(); ++i)
+  {
+    int rc = rhs_one(b->subs[i], t, rhs[i]);
+    if (rc)
+      return rc;
+  }
   return BFAMX_OK;
 }

@@ -34,6 +34,7 @@
  */
 #include <type_traits>
 
+#include "elastic_axis.hpp"
 #include "physics.hpp"
 
 namespace bfamx
@@ -61,55 +62,6 @@ template <int N, class TS> struct CoalVW
   static constexpr int w = 16 / (int)sizeof(TS);
   static constexpr int value = (np % w == 0) ? w : 1;
 };
-
-/* stress field index of sigma[c][axis] (field order v1 v2 v3 s11 s22 s33 s12
- * s13 s23) */
-__device__ __forceinline__ constexpr int sfield(int c, int a)
-{
-  return (c == a) ? 3 + c
-                  : ((c + a == 1) ? 6 : ((c + a == 2) ? 7 : 8));
-}
-
-/* the two velocity components tangential to axis a, and the shear rate they
- * feed: a=0: (v2 -> s12, v3 -> s13); a=1: (v1 -> s12, v3 -> s23);
- * a=2: (v1 -> s13, v2 -> s23) */
-__device__ __forceinline__ constexpr int tang(int a, int t)
-{
-  return a == 0 ? 1 + t : (a == 1 ? (t ? 2 : 0) : t);
-}
-__device__ __forceinline__ constexpr int shear_rate(int a, int t)
-{
-  return a == 0 ? (t ? 7 : 6) : (a == 1 ? (t ? 8 : 6) : (t ? 8 : 7));
-}
-
-/* upwind fluctuations for a face normal to axis A with sign s: from minus
- * and plus velocity v and stress column Sc (sigma[.][A]); izp = 1/(Zp- + Zp+),
- * izs = 1/(Zs- + Zs+) */
-template <int A, class TC>
-__device__ __forceinline__ void riemann_axis(TC s, const TC *vm, const TC *Sm,
-                                             const TC *vp, const TC *Sp,
-                                             TC Zpm, TC Zsm, TC Zpp, TC Zsp,
-                                             TC izp, TC izs, TC *dv, TC *dT)
-{
-  /* traction T = sigma n = s * column A; n.T = s*s*sigma_AA */
-  const TC dvn = (Zpp * (s * (vp[A] - vm[A])) + (Sp[A] - Sm[A])) * izp;
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-  {
-    if (c == A)
-    {
-      dv[c] = dvn * s;
-      dT[c] = Zpm * dvn * s;
-    }
-    else
-    {
-      /* tangential: T_c = s sigma[c][A] */
-      const TC dvt = (Zsp * (vp[c] - vm[c]) + s * (Sp[c] - Sm[c])) * izs;
-      dv[c] = dvt;
-      dT[c] = Zsm * dvt;
-    }
-  }
-}
 
 /*
  * The six per-axis raw quantities of a line, with face lifts folded in:
@@ -196,6 +148,15 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       const int f = 2 * A + side;
       const int32_t ep = nbe[f];
       const int code = nbc[f];
+      if (code & NBR_GLUED)
+      {
+        /* flux fluctuation already on this face node (mortar kernel) */
+        gptr<const TC> gb = glob(static_cast<const TC *>(sd.glue_buf)) + ep + pa + n1 * pb;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          nb[side][c] = gb[c * n1 * n1];
+        continue;
+      }
       const int oi = perm_apply(code >> 3, n1, pa, pb);
       const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
       gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
@@ -336,6 +297,13 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       {
         const int m = side ? N : 0;
         gptr<const TC> F = fmt + (2 * A + side) * FMAT_STRIDE;
+        if (nbc[2 * A + side] & NBR_GLUED)
+        {
+#pragma unroll
+          for (int c = 0; c < 6; ++c)
+            fl[side][c] = nb[side][c];
+          continue;
+        }
         TC vm[3], Sm[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c)

@@ -244,3 +244,18 @@ def upload(dev, cfg, precision):
     if cfg.get("F") is not None:
         sub.set_forcing_mms(cfg["k"], cfg["omega"], cfg["F"])
     return sub
+
+
+def upload_multi(dev, cfg, precision):
+    """One bfam_amd.Subdomain per subdomain of an adapted-mesh config, with
+    the source attached to the subdomain that holds it (no glue yet)."""
+    subs = []
+    for si, sm in enumerate(cfg["subs"]):
+        sub = bx.Subdomain(dev, bx.ELASTIC3D, precision, sm.N, sm.r, sm.w, sm.Dr, sm.vmapM,
+                           sm.vmapP, sm.h, cfg["rho"][si], cfg["cp"][si], cfg["cs"][si],
+                           x0=sm.x0)
+        s = cfg.get("src")
+        if s is not None and s["sub"] == si:
+            sub.set_source(s["elem"], s["weights"], s["amp"], s["f0"], s["t0"])
+        subs.append(sub)
+    return subs

@@ -260,6 +260,11 @@ int bfamx_batch_step(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_capture(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_launch_graph(bfamx_batch_t *b);
 
+/* RHS L(q, t) of every member into host arrays rhs[i][field] (parity
+ * checks); a glued batch evaluates its mortars first */
+int bfamx_batch_rhs(bfamx_batch_t *b, double t, double *const *const *rhs);
+
+
 /* ---------------------------------------------------------------------- */
 /* timing on the device stream (BFAMO_KERNEL_RUN's KERNEL_TIME analogue)   */
 /* ---------------------------------------------------------------------- */
@@ -312,6 +317,23 @@ int bfamx_octree_sub(const bfamx_octree_t *o, int s, int *N, int32_t *K,
                      double *h, int32_t *gid);
 /* the mortars, sorted by big side */
 int bfamx_octree_mortars(const bfamx_octree_t *o, bfamx_mortar_t *m);
+
+/*
+ * Glue: mortars between subdomains (bfamx_mortar_t, sub indices into subs),
+ * the device analogue of the glue subdomains and their operators
+ * (bfam_subdomain_dgx_glue_init, bfam.c:7328-7520): integer records, the
+ * 1-D interpolation / mass-projection operators of all order pairs, and one
+ * flux block per glued face.  The glued faces' entries of the subdomains'
+ * face tables are redirected to that block until bfamx_glue_free.  Elastic
+ * fused precisions, orders 3..7.  Glued subdomains advance only through a
+ * batch holding the same subdomains in the same order, with the glue set.
+ */
+typedef struct bfamx_glue bfamx_glue_t;
+int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *subs,
+                      int nsubs, const bfamx_mortar_t *mortars, int32_t nmortars,
+                      bfamx_glue_t **out);
+int bfamx_glue_free(bfamx_glue_t *g);
+int bfamx_batch_set_glue(bfamx_batch_t *b, bfamx_glue_t *g);
 
 /* ---------------------------------------------------------------------- */
 /* serving: independent elastic solves admitted under an HBM budget as they */

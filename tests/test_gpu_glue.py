@@ -1,0 +1,107 @@
+"""Glued subdomains on the device (mortar kernel + stage kernel through a
+batch) against the oracle's glued right-hand side and LSRK loop: 2:1
+hanging faces between orders 5, 4, 3 and conforming faces between orders 3
+and 5.  Tolerances as for conforming meshes: FP64 1e-12, FP32 2e-5 (max
+norm relative to the largest value)."""
+import numpy as np
+import pytest
+
+import bfam_amd as bx
+import bfam_amd.workloads as W
+from util import oracle_multi, two_order_brick
+
+pytestmark = pytest.mark.gpu
+
+TOL = {bx.FP64: 1e-12, bx.FP32_RHS: 2e-5, bx.FP32: 2e-5}
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = bx.Device(0)
+    yield d
+    d.close()
+
+
+def octree_cfg():
+    return W.elastic3d_octree(n_base=4, levels=2, radius=0.2, orders=(5, 4, 3), seed=5)
+
+
+def rel_max_multi(got, want):
+    num = max(np.abs(g - w).max() for g, w in zip(got, want))
+    return num / max(np.abs(w).max() for w in want)
+
+
+def setup(dev, cfg, prec, seed):
+    rng = np.random.default_rng(seed)
+    q0 = [rng.standard_normal((9, sm.K * sm.Np)) for sm in cfg["subs"]]
+    subs = W.upload_multi(dev, cfg, prec)
+    for s, q in zip(subs, q0):
+        s.set_fields(q, 0.0, cfg["dt"])
+    glue = bx.Glue(dev, subs, cfg["mortars"])
+    b = bx.Batch(dev, subs)
+    b.set_glue(glue)
+    return q0, subs, glue, b
+
+
+def teardown(subs, glue, b):
+    b.close()
+    glue.close()
+    for s in subs:
+        s.close()
+
+
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS, bx.FP32])
+def test_glued_rhs_octree(dev, prec):
+    print(f"glued rhs octree prec={prec}", flush=True)
+    cfg = octree_cfg()
+    q0, subs, glue, b = setup(dev, cfg, prec, 1)
+    t = 0.4 * cfg["src"]["t0"]
+    got = b.rhs(t)
+    teardown(subs, glue, b)
+    want = oracle_multi(cfg).rhs(q0, t)
+    assert rel_max_multi(got, want) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32])
+def test_glued_rhs_mixed_order_conforming(dev, prec):
+    print(f"glued rhs two-order brick prec={prec}", flush=True)
+    cfg = two_order_brick(4, (3, 5), seed=4)
+    q0, subs, glue, b = setup(dev, cfg, prec, 2)
+    got = b.rhs(0.0)
+    teardown(subs, glue, b)
+    want = oracle_multi(cfg).rhs(q0, 0.0)
+    assert rel_max_multi(got, want) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS])
+def test_glued_lsrk_steps(dev, prec):
+    print(f"glued lsrk 3 steps prec={prec}", flush=True)
+    cfg = octree_cfg()
+    q0, subs, glue, b = setup(dev, cfg, prec, 3)
+    b.step(3)
+    got = [s.download() for s in subs]
+    ts = [s.time() for s in subs]
+    teardown(subs, glue, b)
+    want = oracle_multi(cfg).run(q0, cfg["dt"], 3)
+    assert all(st == 15 for _, st in ts)
+    assert rel_max_multi(got, want) < TOL[prec]
+
+
+def test_glued_graph_replay(dev):
+    print("glued graph replay", flush=True)
+    cfg = octree_cfg()
+    q0, subs, glue, b = setup(dev, cfg, bx.FP32, 4)
+    b.step(2)
+    eager = [s.download() for s in subs]
+    for s, q in zip(subs, q0):
+        s.set_fields(q, 0.0, cfg["dt"])
+    b.capture(1)
+    for _ in range(2):
+        b.launch_graph()
+        dev.sync()
+    replay = [s.download() for s in subs]
+    with pytest.raises(bx.BfamxError):
+        subs[0].step(1)  # glued members advance only through their batch
+    teardown(subs, glue, b)
+    for e, r in zip(eager, replay):
+        assert np.array_equal(e, r)
