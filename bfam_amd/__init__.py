@@ -129,11 +129,15 @@ SIGNATURES = {
     "bfamx_reduce_cfl_dt": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_double)]),
     "bfamx_reduce_l2_diff": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bfamx_reduce_mms_error": (C.c_int, [_vp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bfamx_reduce_mms_energy_error": (C.c_int, [_vp, _dp, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double)]),
     "bfamx_batch_create": (C.c_int, [_vp, _PP, C.c_int, _PP]),
     "bfamx_batch_free": (C.c_int, [_vp]),
     "bfamx_batch_step": (C.c_int, [_vp, C.c_int]),
     "bfamx_batch_capture": (C.c_int, [_vp, C.c_int]),
     "bfamx_batch_launch_graph": (C.c_int, [_vp]),
+    "bfamx_batch_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "bfamx_batch_set_strategy": (C.c_int, [_vp, C.c_int]),
     "bfamx_timer_start": (C.c_int, [_vp]),
     "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
@@ -438,6 +442,13 @@ class Subdomain:
         _check(lib().bfamx_reduce_l2_diff(self.h, other.h, C.byref(d), C.byref(n)))
         return d.value, n.value
 
+    def mms_energy_error(self, A):
+        """(energy norm of q - exact, energy norm of exact)"""
+        d, n = C.c_double(), C.c_double()
+        _check(lib().bfamx_reduce_mms_energy_error(
+            self.h, np.ascontiguousarray(A, dtype=np.float64), C.byref(d), C.byref(n)))
+        return d.value, n.value
+
     def mms_error(self, A):
         d, n = C.c_double(), C.c_double()
         _check(lib().bfamx_reduce_mms_error(self.h, np.ascontiguousarray(A, dtype=np.float64),
@@ -468,6 +479,15 @@ class Batch:
 
     def launch_graph(self):
         _check(lib().bfamx_batch_launch_graph(self.h))
+
+    def autotune(self, trials=3, force=False):
+        """0: one mixed-order launch per stage, 1: one launch per order"""
+        c = C.c_int()
+        _check(lib().bfamx_batch_autotune(self.h, int(trials), int(force), C.byref(c)))
+        return c.value
+
+    def set_strategy(self, per_order):
+        _check(lib().bfamx_batch_set_strategy(self.h, int(per_order)))
 
     def set_glue(self, glue):
         _check(lib().bfamx_batch_set_glue(self.h, glue.h if glue is not None else None))

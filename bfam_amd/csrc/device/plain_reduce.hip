@@ -235,30 +235,50 @@ __global__ __launch_bounds__(256) void mms_error_kernel(SubDev sd, int parity,
   const int e = blockIdx.x;
   const int n1 = sd.N + 1, Np = n1 * n1 * n1;
   gptr<const double> T = glob(sd.mms_tab) + (size_t)e * 6 * n1;
-  double d2 = 0, n2 = 0;
+  const double *M = sd.mat + (size_t)e * MAT_STRIDE;
+  const double rho = 1.0 / M[MAT_RHOI], lam = M[MAT_LAM], mu = M[MAT_MU];
+  double d2 = 0, n2 = 0, ed = 0, en = 0;
   for (int n = threadIdx.x; n < Np; n += blockDim.x)
   {
     const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
     const double Sv[3] = {T[0 * n1 + i], T[2 * n1 + j], T[4 * n1 + k]};
     const double Cv[3] = {T[1 * n1 + i], T[3 * n1 + j], T[5 * n1 + k]};
     const double wq = quad_weight(sd, 3, n1, e, n);
-    double sd2 = 0, sn2 = 0;
+    double sd2 = 0, sn2 = 0, dv[9], xv[9];
     for (int f = 0; f < 9; ++f)
     {
       const double ex = amp.A[f] * mms_pat(f, Sv, Cv) * (f < 3 ? cwt : swt);
       const double x = ld<TS>(sd.q[parity], ((size_t)e * 9 + f) * Np + n);
+      dv[f] = x - ex;
+      xv[f] = ex;
       sd2 += (x - ex) * (x - ex);
       sn2 += ex * ex;
     }
     d2 += wq * sd2;
     n2 += wq * sn2;
+    /* energy norm 0.5 (rho |v|^2 + sigma : C^-1 : sigma) of the error and of
+     * the exact solution */
+    for (int w2 = 0; w2 < 2; ++w2)
+    {
+      const double *f = w2 ? xv : dv;
+      const double tr = f[3] + f[4] + f[5];
+      const double ss = f[3] * f[3] + f[4] * f[4] + f[5] * f[5] +
+                        2 * (f[6] * f[6] + f[7] * f[7] + f[8] * f[8]);
+      const double dens = rho * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) +
+                          (ss - lam / (3 * lam + 2 * mu) * tr * tr) / (2 * mu);
+      (w2 ? en : ed) += 0.5 * wq * dens;
+    }
   }
   const double r1 = block_sum<double>(d2, red);
   const double r2 = block_sum<double>(n2, red);
+  const double r3 = block_sum<double>(ed, red);
+  const double r4 = block_sum<double>(en, red);
   if (threadIdx.x == 0)
   {
-    partial[2 * (size_t)e] = r1;
-    partial[2 * (size_t)e + 1] = r2;
+    partial[4 * (size_t)e] = r1;
+    partial[4 * (size_t)e + 1] = r2;
+    partial[4 * (size_t)e + 2] = r3;
+    partial[4 * (size_t)e + 3] = r4;
   }
 }
 

@@ -34,6 +34,10 @@ struct bfamx_device
   int num_xcd;
   int xcd_remap;
   hipEvent_t ev0, ev1;
+  /* mixed-order launch strategy chosen by bfamx_batch_autotune, per
+   * precision and set of orders present (bit N): -1 unknown, 0 one mixed
+   * launch, 1 one launch per order */
+  signed char mixed_choice[4][256];
 };
 
 struct bfamx_subdomain
@@ -48,7 +52,7 @@ struct bfamx_subdomain
   int *d_ntasks;
   int ntasks;
   Clock *d_clock;
-  double *d_partial; /* 2K reduction partials */
+  double *d_partial; /* 4K reduction partials */
   double *d_rhs;     /* lazily allocated for bfamx_rhs on fused precisions */
   std::vector<void *> allocs;
   size_t bytes;
@@ -92,6 +96,10 @@ struct bfamx_batch
   hipGraphExec_t exec;
   int graph_steps;
   bfamx_glue_t *glue;
+  /* mixed orders: tasks sorted by order; [off, off+cnt) of order N */
+  int grouped;
+  int order_mask;
+  int grp_off[9], grp_cnt[9];
 };
 
 #define HIPCHK(call)                                                           \
@@ -135,6 +143,7 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
     nx = 1;
   d->num_xcd = nx;
   d->xcd_remap = 1;
+  memset(d->mixed_choice, -1, sizeof(d->mixed_choice));
   /* default priority only: streams of different priorities have hung on
    * this hardware */
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
@@ -487,7 +496,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     ALLOC(s->d_sub, sizeof(SubDev));
     ALLOC(s->d_tasks, sizeof(Task) * tasks.size());
     ALLOC(s->d_ntasks, sizeof(int));
-    ALLOC(s->d_partial, 16 * (size_t)K);
+    ALLOC(s->d_partial, 32 * (size_t)K);
     hipError_t he = hipSuccess;
 #define CP(dst, src, bytes)                                                    \
   if (he == hipSuccess)                                                        \
@@ -982,11 +991,10 @@ extern "C" int bfamx_reduce_l2_diff(bfamx_subdomain_t *a, bfamx_subdomain_t *b,
   return BFAMX_OK;
 }
 
-extern "C" int bfamx_reduce_mms_error(bfamx_subdomain_t *s, const double *A,
-                                      double *diff2, double *norm2)
+static int mms_partials(bfamx_subdomain_t *s, const double *A, double *out4)
 {
-  if (!s || !A || !diff2 || !norm2 || !s->h.mms_tab)
-    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_mms_error: no forcing set");
+  if (!s || !A || !s->h.mms_tab)
+    return bfamx_fail(BFAMX_EINVAL, "MMS error: no forcing set");
   Clock c;
   HIPCHK(hipStreamSynchronize(s->dev->stream));
   HIPCHK(hipMemcpy(&c, s->d_clock, sizeof(c), hipMemcpyDeviceToHost));
@@ -998,11 +1006,39 @@ extern "C" int bfamx_reduce_mms_error(bfamx_subdomain_t *s, const double *A,
                                   s->dev->stream);
   if (e != hipSuccess)
     return bfamx_fail(BFAMX_EHIP, "mms error launch: %s", hipGetErrorString(e));
-  std::vector<double> part(2 * (size_t)s->K);
+  std::vector<double> part(4 * (size_t)s->K);
   HIPCHK(hipStreamSynchronize(s->dev->stream));
   HIPCHK(hipMemcpy(part.data(), s->d_partial, 8 * part.size(), hipMemcpyDeviceToHost));
-  *diff2 = neumaier(part.data(), s->K, 2);
-  *norm2 = neumaier(part.data() + 1, s->K, 2);
+  for (int k = 0; k < 4; ++k)
+    out4[k] = neumaier(part.data() + k, s->K, 4);
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_reduce_mms_error(bfamx_subdomain_t *s, const double *A,
+                                      double *diff2, double *norm2)
+{
+  if (!diff2 || !norm2)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_mms_error: NULL output");
+  double o[4];
+  const int rc = mms_partials(s, A, o);
+  if (rc)
+    return rc;
+  *diff2 = o[0];
+  *norm2 = o[1];
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_reduce_mms_energy_error(bfamx_subdomain_t *s, const double *A,
+                                             double *err_energy, double *exact_energy)
+{
+  if (!err_energy || !exact_energy)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_reduce_mms_energy_error: NULL output");
+  double o[4];
+  const int rc = mms_partials(s, A, o);
+  if (rc)
+    return rc;
+  *err_energy = o[2];
+  *exact_energy = o[3];
   return BFAMX_OK;
 }
 
@@ -1063,7 +1099,25 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
     }
     per[i] = cnt;
   }
+  std::stable_sort(tasks.begin(), tasks.end(),
+                   [](const Task &x, const Task &y) { return x.N < y.N; });
   b->ntasks = (int)tasks.size();
+  b->grouped = 0;
+  b->order_mask = 0;
+  for (int n = 0; n < 9; ++n)
+    b->grp_off[n] = b->grp_cnt[n] = 0;
+  for (int t = 0; t < b->ntasks; ++t)
+  {
+    const int n = tasks[t].N;
+    if (b->grp_cnt[n]++ == 0)
+      b->grp_off[n] = t;
+    b->order_mask |= 1 << n;
+  }
+  if (N < 0)
+  {
+    const signed char c = dev->mixed_choice[prec][b->order_mask & 255];
+    b->grouped = c > 0;
+  }
   std::vector<void *> allocs;
   void *p;
   int rc = dev_alloc(dev, allocs, &b->bytes, sizeof(SubDev) * nsubs, &p);
@@ -1124,6 +1178,30 @@ static GlueArgs glue_args(const bfamx_batch_t *b)
   return g;
 }
 
+/* the stage kernels of a batch: one launch (one order, or all orders with
+ * the mixed kernel), or one launch per order over its slice of the tasks */
+static hipError_t launch_batch_kernels(const bfamx_batch_t *b, const LaunchArgs &a,
+                                       int grid)
+{
+  if (b->physics != BFAMX_ELASTIC3D)
+    return launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
+  if (b->N >= 0 || !b->grouped)
+    return launch_elastic_line(b->prec, b->N, a, grid, b->dev->stream);
+  for (int n = 0; n < 9; ++n)
+  {
+    if (!b->grp_cnt[n])
+      continue;
+    LaunchArgs g = a;
+    g.tasks = a.tasks + b->grp_off[n];
+    g.ntasks = b->grp_cnt[n];
+    hipError_t e = launch_elastic_line(b->prec, n, g, grid_for(b->dev, g.ntasks),
+                                       b->dev->stream);
+    if (e != hipSuccess)
+      return e;
+  }
+  return hipSuccess;
+}
+
 static int batch_stage(bfamx_batch_t *b)
 {
   LaunchArgs a;
@@ -1139,9 +1217,7 @@ static int batch_stage(bfamx_batch_t *b)
   if (b->glue)
     e = launch_mortar(b->prec, glue_args(b), b->dev->stream);
   if (e == hipSuccess)
-    e = b->physics == BFAMX_ELASTIC3D
-            ? launch_elastic_line(b->prec, b->N, a, grid, b->dev->stream)
-            : launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
+    e = launch_batch_kernels(b, a, grid);
   if (e == hipSuccess)
     e = launch_clock_advance(b->d_subs, (int)b->subs.size(), b->dev->stream);
   if (e != hipSuccess)
@@ -1499,5 +1575,118 @@ extern "C" int bfamx_batch_rhs(bfamx_batch_t *b, double t, double *const *const 
     if (rc)
       return rc;
   }
+  return BFAMX_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* autotuning of the mixed-order launch                                    */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Times `trials` stage launches of each strategy (one mixed-order launch,
+ * one launch per order) on the batch's own data and keeps the faster one,
+ * cached on the device per precision and set of orders.  The trial launches
+ * do not advance the clocks: they read the current state, write the free
+ * ping-pong buffer and dq; dq is saved beforehand and restored, so the
+ * batch's state and results are unchanged.
+ */
+extern "C" int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order)
+{
+  if (!b || per_order < 0 || per_order > 1)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_set_strategy: bad arguments");
+  if (b->exec)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_set_strategy: batch already captured");
+  b->grouped = b->N < 0 ? per_order : 0;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
+                                    int *choice)
+{
+  if (!b || trials < 1)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_autotune: bad arguments");
+  for (auto *s : b->subs)
+    if (!s->fields_set)
+      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  if (b->N >= 0 || b->physics != BFAMX_ELASTIC3D)
+  {
+    if (choice)
+      *choice = 0;
+    return BFAMX_OK;
+  }
+  if (b->exec)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_autotune: batch already captured");
+  signed char &cache = b->dev->mixed_choice[b->prec][b->order_mask & 255];
+  if (cache >= 0 && !force)
+  {
+    b->grouped = cache;
+    if (choice)
+      *choice = cache;
+    return BFAMX_OK;
+  }
+  bfamx_device_t *d = b->dev;
+  /* save dq */
+  std::vector<void *> saved(b->subs.size(), nullptr);
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  int rc = BFAMX_OK;
+  for (size_t i = 0; i < b->subs.size() && !rc; ++i)
+  {
+    bfamx_subdomain_t *s = b->subs[i];
+    const size_t nb = (size_t)s->K * s->nf * s->Np * (s->rate_fp32 ? 4 : 8);
+    rc = dev_alloc(d, allocs, &bytes, nb, &saved[i]);
+    if (!rc && hipMemcpyAsync(saved[i], s->h.dq, nb, hipMemcpyDeviceToDevice, d->stream) !=
+                   hipSuccess)
+      rc = bfamx_fail(BFAMX_EHIP, "autotune: dq save failed");
+  }
+  float ms[2] = {0, 0};
+  LaunchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.subs = b->d_subs;
+  a.tasks = b->d_tasks;
+  a.ntasks = b->ntasks;
+  a.mode = MODE_STAGE;
+  a.xcd_remap = d->xcd_remap;
+  a.nxcd = d->num_xcd;
+  const int grid = grid_for(d, b->ntasks);
+  const int keep = b->grouped;
+  for (int mode = 0; mode < 2 && !rc; ++mode)
+  {
+    b->grouped = mode;
+    hipError_t e = launch_batch_kernels(b, a, grid); /* warm-up */
+    if (e == hipSuccess)
+      e = hipEventRecord(d->ev0, d->stream);
+    for (int t = 0; t < trials && e == hipSuccess; ++t)
+      e = launch_batch_kernels(b, a, grid);
+    if (e == hipSuccess)
+      e = hipEventRecord(d->ev1, d->stream);
+    if (e == hipSuccess)
+      e = hipEventSynchronize(d->ev1);
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&ms[mode], d->ev0, d->ev1);
+    if (e != hipSuccess)
+      rc = bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
+  }
+  for (size_t i = 0; i < b->subs.size(); ++i)
+    if (saved[i])
+    {
+      bfamx_subdomain_t *s = b->subs[i];
+      const size_t nb = (size_t)s->K * s->nf * s->Np * (s->rate_fp32 ? 4 : 8);
+      if (hipMemcpyAsync(s->h.dq, saved[i], nb, hipMemcpyDeviceToDevice, d->stream) !=
+              hipSuccess &&
+          !rc)
+        rc = bfamx_fail(BFAMX_EHIP, "autotune: dq restore failed");
+    }
+  (void)hipStreamSynchronize(d->stream);
+  dev_free_all(d, allocs, bytes);
+  if (rc)
+  {
+    b->grouped = keep;
+    return rc;
+  }
+  b->grouped = ms[1] < ms[0] ? 1 : 0;
+  cache = (signed char)b->grouped;
+  if (choice)
+    *choice = b->grouped;
   return BFAMX_OK;
 }

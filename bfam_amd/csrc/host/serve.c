@@ -304,6 +304,10 @@ int bfamx_serve(bfamx_device_t *dev, const bfamx_request_t *reqs, int nreq,
         subs[i] = act[i].sub;
       if ((rc = bfamx_batch_create(dev, subs, nact, &batch)))
         goto out;
+      /* mixed orders: one launch or one per order, timed once per order
+       * set and cached on the device; a failure keeps the default */
+      if (bfamx_batch_autotune(batch, 2, 0, NULL))
+        bfamx_clear_error();
       dirty = 0;
       ++stats->nbatches;
     }

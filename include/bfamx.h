@@ -245,6 +245,10 @@ int bfamx_reduce_l2_diff(bfamx_subdomain_t *a, bfamx_subdomain_t *b,
 /* same against the manufactured solution with amplitudes A at the current t */
 int bfamx_reduce_mms_error(bfamx_subdomain_t *sub, const double *A,
                            double *diff2, double *norm2);
+/* energy norm 0.5 sum J w (rho|dv|^2 + ds : C^-1 : ds) of the difference to
+ * the manufactured solution, and of the manufactured solution itself */
+int bfamx_reduce_mms_energy_error(bfamx_subdomain_t *sub, const double *A,
+                                  double *err_energy, double *exact_energy);
 
 /* ---------------------------------------------------------------------- */
 /* batches: many subdomains (mixed N, independent problems) advanced by one */
@@ -259,6 +263,16 @@ int bfamx_batch_step(bfamx_batch_t *b, int nsteps);
 /* capture nsteps steps of the batch into a HIP graph; launch replays it */
 int bfamx_batch_capture(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_launch_graph(bfamx_batch_t *b);
+/*
+ * Mixed orders: choose between one launch per stage with the order-switching
+ * kernel and one launch per order, by timing `trials` stages of each on the
+ * batch's own data (the state and results are left unchanged).  The choice
+ * is cached on the device per precision and set of orders; force = 1
+ * re-times.  *choice: 0 mixed, 1 per order.  No-op for a single order.
+ */
+int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force, int *choice);
+/* set the mixed-order strategy by hand: 0 mixed launch, 1 one per order */
+int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order);
 
 /* RHS L(q, t) of every member into host arrays rhs[i][field] (parity
  * checks); a glued batch evaluates its mortars first */

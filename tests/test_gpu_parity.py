@@ -229,3 +229,60 @@ def test_cfl_dt_reduction(dev, phys):
     want = 0.5 * np.min(m.h.min(axis=1) / cfg["cp"]) / (m.N + 1) ** 2
     assert abs(dt - want) <= 1e-15 * want
     assert abs(dt - cfg["dt"]) <= 1e-15 * want
+
+
+@pytest.mark.parametrize("prec", [bx.FP32_RHS, bx.FP64])
+def test_mixed_order_launch_strategies(dev, prec):
+    """Orders 3..7 in one batch: the order-switching kernel and one launch
+    per order give the same bits, and autotuning (which times real launches
+    on the batch) leaves the state unchanged."""
+    print(f"mixed vs grouped prec={prec}", flush=True)
+    res = []
+    for strategy in (0, 1, "auto"):
+        subs = []
+        for i, N in enumerate([3, 4, 5, 6, 7]):
+            cfg, q0 = random_elastic((2, 2, 3), N, seed=60 + i)
+            s = W.upload(dev, cfg, prec)
+            s.set_fields(q0, 0.0, cfg["dt"])
+            subs.append(s)
+        b = bx.Batch(dev, subs)
+        if strategy == "auto":
+            assert b.autotune(trials=2, force=True) in (0, 1)
+        else:
+            b.set_strategy(strategy)
+        b.step(2)
+        res.append([s.download() for s in subs])
+        b.close()
+        for s in subs:
+            s.close()
+    for r in res[1:]:
+        for a, c in zip(res[0], r):
+            assert np.array_equal(a, c)
+
+
+def test_mms_energy_error_and_precisions(dev):
+    """The device energy-norm error against the manufactured solution equals
+    the oracle's energy of (q - exact); FP32_RHS (FP64 state) and FP32 stay
+    close to FP64 over 100 steps."""
+    print("mms energy error + precisions 5^3 N=5", flush=True)
+    cfg = W.elastic3d_mms(n=5, N=5)
+    m = cfg["mesh"]
+    q0 = W.mms_exact(m, cfg["A"], cfg["k"], cfg["omega"], 0.0)
+    P = oracle_problem(cfg)
+    out = {}
+    for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
+        s = W.upload(dev, cfg, prec)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(100)
+        q = s.download()
+        t = s.time()[0]
+        ed, en = s.mms_energy_error(cfg["A"])
+        ex = W.mms_exact(m, cfg["A"], cfg["k"], cfg["omega"], t)
+        assert abs(ed - P.energy(q - ex)) <= 1e-10 * P.energy(ex)
+        assert abs(en - P.energy(ex)) <= 1e-12 * P.energy(ex)
+        out[prec] = q
+        s.close()
+    ref = out[bx.FP64]
+    scale = np.sqrt((ref ** 2).sum())
+    assert np.sqrt(((out[bx.FP32_RHS] - ref) ** 2).sum()) / scale < 1e-5
+    assert np.sqrt(((out[bx.FP32] - ref) ** 2).sum()) / scale < 1e-5
