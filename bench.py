@@ -24,6 +24,11 @@ The JSON line also carries:
                      summary under profiles/);
   cpu_baseline       the FP64 CPU restatement (OpenMP) timed on this host's
                      cores over one LSRK step of the same brick;
+  adapted_mesh       the adapted configuration: a 16^3 forest refined two
+                     levels inside a seeded sphere, 2:1 balanced, orders
+                     7/5/3 by level, one subdomain per order glued by mortars
+                     (hanging faces), FP32, the mortar kernel and the stage
+                     kernels of all orders per stage (strategy autotuned);
   serving            a seeded Poisson stream of independent elastic solves
                      (periodic cube bricks of 8..32 elements a side, N in
                      3..7, 200..2000 steps, layered material, Ricker source)
@@ -120,6 +125,7 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adapted", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=0.6, help="arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -201,6 +207,36 @@ def main():
         b.close()
         log(f"accuracy: {accuracy}")
 
+    adapted = None
+    if not args.only and not args.no_adapted:
+        oc = W.elastic3d_octree()
+        subs_o = W.upload_multi(dev, oc, bx.FP32)
+        for s_, sm in zip(subs_o, oc["subs"]):
+            s_.set_fields(np.zeros((9, sm.K * sm.Np)), 0.0, oc["dt"])
+        glue = bx.Glue(dev, subs_o, oc["mortars"])
+        bo = bx.Batch(dev, subs_o)
+        bo.set_glue(glue)
+        strategy = bo.autotune(trials=3)
+        bo.capture(1)
+        for _ in range(3):
+            bo.launch_graph()
+        dev.sync()
+        ka = 40
+        dev.timer_start()
+        for _ in range(ka):
+            bo.launch_graph()
+        ms_a = dev.timer_stop()
+        adapted = {"elements_per_level": [int(x) for x in oc["K_level"][:3]],
+                   "orders": [7, 5, 3], "mortars": int(len(oc["mortars"])),
+                   "dof": int(oc["dof"]), "steps": ka, "ms_per_step": ms_a / ka,
+                   "dof_stage_per_s": oc["dof"] * 5 * ka / (ms_a * 1e-3),
+                   "strategy": "one launch per order" if strategy else "one mixed launch"}
+        bo.close()
+        glue.close()
+        for s_ in subs_o:
+            s_.close()
+        log(f"adapted mesh: {adapted}")
+
     serving = None
     if not args.only and args.serve_requests > 0:
         reqs = bx.serve_generate(seed=2024, nreq=args.serve_requests,
@@ -251,6 +287,7 @@ def main():
                      "kernel": f"elastic_line_kernel<{args.order}, fp32>",
                      "bytes_per_dof": BYTES_PER_DOF[head_name]},
         "cpu_baseline": cpu,
+        "adapted_mesh": adapted,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -141,6 +141,7 @@ SIGNATURES = {
     "bfamx_timer_start": (C.c_int, [_vp]),
     "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
+    "bfamx_set_persistent": (C.c_int, [_vp, C.c_int]),
     "bfamx_octree_build": (C.c_int, [C.POINTER(OctreeDesc), _PP]),
     "bfamx_octree_free": (C.c_int, [_vp]),
     "bfamx_octree_info": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int),
@@ -327,6 +328,9 @@ class Device:
 
     def set_tuning(self, xcd_remap=1):
         _check(lib().bfamx_set_tuning(self.h, int(xcd_remap)))
+
+    def set_persistent(self, on=1):
+        _check(lib().bfamx_set_persistent(self.h, int(on)))
 
 
 def subdomain_bytes(dim, N, K, physics, precision):

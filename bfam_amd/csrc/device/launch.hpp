@@ -22,6 +22,8 @@ inline int elastic_line_epw(int N)
 }
 /* workgroups of the line stage for ntasks tasks (one task per wave) */
 constexpr int LINE_TASKS_PER_BLOCK = 1;
+/* resident one-wave blocks per CU of the elastic stage kernel */
+hipError_t line_blocks_per_cu(int prec, int N, int *blocks);
 hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,
                                hipStream_t s);
 

@@ -51,8 +51,11 @@ template <int N, class TL, class TC> struct LineCfg
    * at N = 7 the dense layout fits 7 instead of 6 waves per CU and measured
    * 2.5 % faster (2.30 against 2.36 ms per 48^3 stage), so it is dense */
   static constexpr int S = NL;
-  static constexpr int FS = S * n1; /* words per field */
-  /* per wavefront: 9 staged state fields (type TL) + one transpose field */
+  static constexpr int FS = S * n1;    /* words per field of one element */
+  static constexpr int FST = EPW * FS; /* field stride: the wave's elements */
+  /* per wavefront: 9 staged state fields (type TL), field-major over the
+   * wave's elements so that a coalesced row lands at consecutive words,
+   * then one transpose field per element */
   static constexpr size_t bytes = EPW * FS * (9 * sizeof(TL) + sizeof(TC));
 };
 
@@ -64,6 +67,51 @@ template <int N, class TS> struct CoalVW
   static constexpr int w = 16 / (int)sizeof(TS);
   static constexpr int value = (np % w == 0) ? w : 1;
 };
+
+/* the next task of a persistent wavefront, whose rows it starts loading
+ * while the current task writes its output */
+template <class TS> struct NextRows
+{
+  gptr<const TS> q; /* state buffer the next task reads */
+  int e0, ne;
+  bool valid;
+};
+
+/*
+ * Field f of a task's state straight from HBM into LDS (global_load_lds, no
+ * register staging): lane l moves nodes VW*(l + 64 r) .. +VW-1 of the field
+ * row group, counted flat over the task's elements, to the same word offsets
+ * of the field's LDS region -- the LDS-DMA unit writes lane l's VW words at
+ * base + VW*l, which is exactly that offset.  Completion is tracked by
+ * vmcnt; readers wait for it.
+ */
+template <int N, class TS>
+__device__ __forceinline__ void rows_to_lds(gptr<const TS> q, int e0, int ne, int f,
+                                            TS *field_lds)
+{
+  constexpr int n1 = N + 1, Np = n1 * n1 * n1, EPW = 64 / (n1 * n1);
+  constexpr int VW = CoalVW<N, TS>::value;
+  constexpr int ROWS = (EPW * Np + 64 * VW - 1) / (64 * VW);
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
+  {
+    const int x = VW * (lane + 64 * r);
+    if (x < ne * Np)
+    {
+      const int lel = x / Np, n = x - lel * Np;
+      gptr<const TS> src = q + ((size_t)(e0 + lel) * 9 + f) * Np + n;
+      if constexpr (VW * sizeof(TS) == 16)
+        __builtin_amdgcn_global_load_lds(
+            src, (__attribute__((address_space(3))) void *)(field_lds + 64 * VW * r), 16,
+            0, 0);
+      else if constexpr (VW * sizeof(TS) == 4)
+        __builtin_amdgcn_global_load_lds(
+            src, (__attribute__((address_space(3))) void *)(field_lds + 64 * VW * r), 4,
+            0, 0);
+    }
+  }
+}
 
 /*
  * The six per-axis raw quantities of a line, with face lifts folded in:
@@ -96,11 +144,18 @@ __device__ __forceinline__ void fold(int c, TC q, TC g, TC glam, TC gl2m,
     rate[shear_rate(A, c - 4)] += gmu * q;
 }
 
+/*
+ * One task.  When the state type is the LDS type (FP32, FP64) its rows reach
+ * LDS by LDS-DMA; rows_ready says a previous task of this wavefront already
+ * issued them, and a valid nx gets its rows issued field by field as this
+ * task's output frees each field's LDS region (persistent wavefronts).
+ */
 template <int N, class P, int PART = PART_FUSED>
 __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
                                                   const Task &tk,
                                                   const StageCtx &cx, int mode,
-                                                  char *lds)
+                                                  char *lds, bool rows_ready,
+                                                  const NextRows<typename P::TS> &nx)
 {
   using TS = typename P::TS;
   using TR = typename P::TR;
@@ -111,6 +166,14 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   constexpr bool q_from_lds = std::is_same<TS, TL>::value;
   using CF = LineCfg<N, TL, TC>;
   constexpr int n1 = CF::n1, NL = CF::NL, Np = CF::Np, S = CF::S, FS = CF::FS;
+  constexpr int FST = CF::FST;
+  /* LDS-DMA moves 4 or 16 bytes per lane.  It is used for the FP32 state;
+   * the FP64 stage measured 10 % slower with it than with staging through
+   * registers (5.77 against 6.34 ms per 48^3 N=7 stage), and FP64 rows of
+   * odd-sized elements (one double per lane) cannot use it */
+  constexpr int dma_bytes = CoalVW<N, TS>::value * (int)sizeof(TS);
+  constexpr bool dma = std::is_same<TS, TL>::value && sizeof(TS) == 4 &&
+                       (dma_bytes == 4 || dma_bytes == 16);
   constexpr int NF = 9;
 
   const int lane = threadIdx.x & 63;
@@ -118,7 +181,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   const bool act = le < tk.ne;
   const int e = tk.e0 + (act ? le : 0);
   const int pa = L % n1, pb = L / n1; /* (a, b) of this lane in every pass */
-  TL *sq = reinterpret_cast<TL *>(lds) + le * NF * FS; /* staged state */
+  TL *sq = reinterpret_cast<TL *>(lds) + le * FS; /* staged state, field stride FST */
   TC *tb = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS) +
            le * FS; /* one-field transpose buffer */
 
@@ -192,6 +255,16 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   constexpr bool vec_lds = (n1 % VW) == 0;
   using VL = TL __attribute__((ext_vector_type(VW)));
   using VC = TC __attribute__((ext_vector_type(VW)));
+  if constexpr (dma)
+  {
+    if (!rows_ready)
+    {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        rows_to_lds<N, TS>(qin, tk.e0, tk.ne, f, reinterpret_cast<TS *>(lds) + f * FST);
+    }
+  }
+  else
   {
     TS v[NF][ROWS][VW];
 #pragma unroll
@@ -216,7 +289,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         int lel, n;
         if (coal_pos(r, &lel, &n))
         {
-          TL *dst = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
+          TL *dst = reinterpret_cast<TL *>(lds) + f * FST + lel * FS;
           if constexpr (vec_lds)
           {
             VL x;
@@ -275,6 +348,9 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         rate[f][i] = TC(0);
   }
 
+  /* LDS-DMA rows of this task have landed once vmcnt drains */
+  if constexpr (dma)
+    __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -310,8 +386,8 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
 #pragma unroll
         for (int c = 0; c < 3; ++c)
         {
-          vm[c] = (TC)sq[c * FS + at(m)];
-          Sm[c] = (TC)sq[sfield(c, A) * FS + at(m)];
+          vm[c] = (TC)sq[c * FST + at(m)];
+          Sm[c] = (TC)sq[sfield(c, A) * FST + at(m)];
         }
         riemann_axis<A, TC>(side ? TC(1) : TC(-1), vm, Sm, &nb[side][0],
                             &nb[side][3], Zpm, Zsm, F[FMAT_ZP], F[FMAT_ZS],
@@ -330,7 +406,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         TC w[n1];
 #pragma unroll
         for (int m = 0; m < n1; ++m)
-          w[m] = (TC)sq[f * FS + at(m)];
+          w[m] = (TC)sq[f * FST + at(m)];
 #pragma unroll
         for (int j = 0; j < n1; ++j)
         {
@@ -500,7 +576,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       if (!coal_pos(r, &lel, &n))
         continue;
       const TC *tbe = tb0 + lel * FS;
-      const TL *sqe = reinterpret_cast<TL *>(lds) + lel * NF * FS + f * FS;
+      const TL *sqe = reinterpret_cast<TL *>(lds) + f * FST + lel * FS;
       if constexpr (vec_lds)
       {
         const VC x = *reinterpret_cast<const VC *>(tbe + lds_node(n));
@@ -558,6 +634,14 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
     }
     if (!rhs_out && f + PD < NF)
       fetch(f + PD);
+    /* field f of this task is written: its LDS region takes the next
+     * task's field f (its reads above have completed, their results being
+     * stored) */
+    if constexpr (dma)
+    {
+      if (nx.valid)
+        rows_to_lds<N, TS>(nx.q, nx.e0, nx.ne, f, reinterpret_cast<TS *>(lds) + f * FST);
+    }
   }
 }
 
@@ -594,13 +678,28 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_kernel(LaunchArgs 
   using CF = LineCfg<N, typename P::TC, typename P::TC>;
   __shared__ __align__(16) char lds[LINE_WPB * CF::bytes];
   char *mine = lds + (threadIdx.x >> 6) * CF::bytes;
+  using TS = typename P::TS;
   int stride, end;
+  bool ready = false;
   for (int ti = first_wave_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
     const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
-    elastic_line_task<N, P, PART>(sd, tk, cx, a.mode, mine);
+    NextRows<TS> nx;
+    nx.valid = ti + stride < end;
+    nx.q = glob(static_cast<const TS *>(sd.q[cx.parity]));
+    nx.e0 = nx.ne = 0;
+    if (nx.valid)
+    {
+      const Task nt = load_task(a.tasks, ti + stride);
+      const SubDev &ns = *(const SubDev *)(glob(a.subs) + nt.sub);
+      nx.q = glob(static_cast<const TS *>(ns.q[stage_ctx(ns, a).parity]));
+      nx.e0 = nt.e0;
+      nx.ne = nt.ne;
+    }
+    elastic_line_task<N, P, PART>(sd, tk, cx, a.mode, mine, ready, nx);
+    ready = nx.valid;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -627,13 +726,17 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_mixed(LaunchArgs a
     const Task tk = load_task(a.tasks, ti);
     const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
+    NextRows<TS> nx; /* no cross-task prefetch: the next task's order differs */
+    nx.valid = false;
+    nx.q = glob(static_cast<const TS *>(sd.q[cx.parity]));
+    nx.e0 = nx.ne = 0;
     switch (tk.N)
     {
-    case 3: elastic_line_task<3, P>(sd, tk, cx, a.mode, mine); break;
-    case 4: elastic_line_task<4, P>(sd, tk, cx, a.mode, mine); break;
-    case 5: elastic_line_task<5, P>(sd, tk, cx, a.mode, mine); break;
-    case 6: elastic_line_task<6, P>(sd, tk, cx, a.mode, mine); break;
-    default: elastic_line_task<7, P>(sd, tk, cx, a.mode, mine); break;
+    case 3: elastic_line_task<3, P>(sd, tk, cx, a.mode, mine, false, nx); break;
+    case 4: elastic_line_task<4, P>(sd, tk, cx, a.mode, mine, false, nx); break;
+    case 5: elastic_line_task<5, P>(sd, tk, cx, a.mode, mine, false, nx); break;
+    case 6: elastic_line_task<6, P>(sd, tk, cx, a.mode, mine, false, nx); break;
+    default: elastic_line_task<7, P>(sd, tk, cx, a.mode, mine, false, nx); break;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -664,6 +767,37 @@ static hipError_t launch_line_n(int N, const LaunchArgs &a, int grid,
     break;
   }
   return hipGetLastError();
+}
+
+template <class P>
+static hipError_t occupancy_n(int N, int *blocks)
+{
+  switch (N)
+  {
+#define BFAMX_CASE(NN)                                                         \
+  case NN:                                                                     \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
+        blocks, elastic_line_kernel<NN, P, PART_FUSED>, 64 * LINE_WPB, 0);
+    BFAMX_CASE(3)
+    BFAMX_CASE(4)
+    BFAMX_CASE(5)
+    BFAMX_CASE(6)
+    BFAMX_CASE(7)
+#undef BFAMX_CASE
+  default:
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, elastic_line_mixed<P>,
+                                                        64 * LINE_WPB, 0);
+  }
+}
+
+hipError_t line_blocks_per_cu(int prec, int N, int *blocks)
+{
+  switch (prec)
+  {
+  case PREC_FP32: return occupancy_n<P32>(N, blocks);
+  case PREC_FP32_RHS: return occupancy_n<P32R>(N, blocks);
+  default: return occupancy_n<P64>(N, blocks);
+  }
 }
 
 hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,

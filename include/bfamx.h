@@ -288,6 +288,10 @@ int bfamx_timer_stop(bfamx_device_t *dev, float *ms);
 /* launch tuning: blocks remapped so each XCD walks a contiguous element
  * range (1, default) or hardware round-robin order (0) */
 int bfamx_set_tuning(bfamx_device_t *dev, int xcd_remap);
+/* elastic stage launches: 1 at most the resident wavefronts, each looping
+ * over tasks and streaming the next task's state into LDS while it writes
+ * the current one; 0 (default, measured faster) one wavefront per task */
+int bfamx_set_persistent(bfamx_device_t *dev, int on);
 
 /* ---------------------------------------------------------------------- */
 /* adapted meshes: brick forest refined inside a sphere, 2:1 face balanced, */

@@ -105,3 +105,17 @@ def test_glued_graph_replay(dev):
     teardown(subs, glue, b)
     for e, r in zip(eager, replay):
         assert np.array_equal(e, r)
+
+
+def test_full_size_octree_rhs(dev):
+    """The adapted configuration at full size (16^3 forest refined twice in
+    a sphere, orders 7/5/3, 5,104 mortars): one FP64 glued RHS against the
+    oracle on the host's cores."""
+    print("full size octree rhs vs oracle", flush=True)
+    cfg = W.elastic3d_octree()
+    q0, subs, glue, b = setup(dev, cfg, bx.FP64, 7)
+    t = 0.3 * cfg["src"]["t0"]
+    got = b.rhs(t)
+    teardown(subs, glue, b)
+    want = oracle_multi(cfg).rhs(q0, t)
+    assert rel_max_multi(got, want) < 1e-12

@@ -286,3 +286,26 @@ def test_mms_energy_error_and_precisions(dev):
     scale = np.sqrt((ref ** 2).sum())
     assert np.sqrt(((out[bx.FP32_RHS] - ref) ** 2).sum()) / scale < 1e-5
     assert np.sqrt(((out[bx.FP32] - ref) ** 2).sum()) / scale < 1e-5
+
+
+@pytest.mark.parametrize("prec", [bx.FP32, bx.FP64, bx.FP32_RHS])
+def test_persistent_launch_invariant(dev, prec):
+    """Persistent wavefronts (tasks looped, next state streamed into LDS
+    during the current output) give the same bits as one wavefront per
+    task, for an element count that leaves partial last tasks (N=4: 4
+    elements per wavefront)."""
+    res = []
+    for on in (1, 0):
+        dev.set_persistent(on)
+        for N in (7, 4):
+            cfg, q0 = random_elastic((5, 3, 3) if N == 7 else (5, 3, 5), N, seed=70 + N)
+            s = W.upload(dev, cfg, prec)
+            s.set_fields(q0, 0.0, cfg["dt"])
+            s.step(2)
+            res.append(s.download())
+            if on:
+                want = oracle_problem(cfg).run(q0, cfg["dt"], 2)
+                assert rel_max(res[-1], want) < TOL[prec]
+            s.close()
+    dev.set_persistent(0)
+    assert np.array_equal(res[0], res[2]) and np.array_equal(res[1], res[3])
