@@ -25,6 +25,17 @@
  *   host setup mirrors       bfam_jacobi_* (bfam.c:2681-3135), gmask/buildmaps
  *                            (bfam.c:6599-6877), create_interpolators
  *                            (bfam.c:5606-5691), bfam_pcg32_* (bfam.c:8328-8387)
+ *   bfamx_octree_*           p4est forest + split_dgx_subdomains + the face
+ *                            mapping of glue construction        bfam.c:4684-4796,
+ *                                                                 5083-5165
+ *   bfamx_glue_upload        bfam_subdomain_dgx_glue_init         bfam.c:7328-7520
+ *   bfamx_batch_*            one launch per stage over many subdomains: the
+ *                            stage loop bfamo_lsrk_t implies over the
+ *                            subdomains bfamo_lsrk_init collects bfam.c:9005-9010
+ * No counterpart in the reference (the kernels they serve are absent there):
+ * the FP64 reductions (energy, L2, manufactured-solution error, CFL time
+ * step), the Ricker source and manufactured forcing, the mixed-order launch
+ * autotuner and launch switches, and the request serving loop.
  */
 #ifndef BFAMX_H
 #define BFAMX_H
