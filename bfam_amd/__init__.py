@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbfamx.so")
 
 ACOUSTIC2D, ELASTIC3D = 0, 1
+BC_NONE, BC_FREE, BC_RIGID, BC_ABSORB = 0, 1, 2, 3
 FP64, FP64_PLAIN, FP32_RHS, FP32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp64_plain": FP64_PLAIN, "fp32_rhs": FP32_RHS, "fp32": FP32}
 LSRK_KC54, LSRK_FE, LSRK_HEUN, LSRK_W33, LSRK_NOOP = 0, 1, 2, 3, 4
@@ -117,6 +118,7 @@ SIGNATURES = {
     "bfamx_subdomain_free": (C.c_int, [_vp]),
     "bfamx_subdomain_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
     "bfamx_subdomain_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int32, C.c_int, C.c_int]),
+    "bfamx_set_boundary": (C.c_int, [_vp, _i8p]),
     "bfamx_set_source": (C.c_int, [_vp, C.c_int32, _dp, _dp, C.c_double, C.c_double]),
     "bfamx_set_forcing_mms": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
     "bfamx_set_fields": (C.c_int, [_vp, _PP, C.c_double, C.c_double]),
@@ -388,6 +390,11 @@ class Subdomain:
         nf, b = C.c_int(), C.c_size_t()
         _check(lib().bfamx_subdomain_info(self.h, C.byref(nf), C.byref(b)))
         return nf.value, b.value
+
+    def set_boundary(self, kind):
+        """kind: (K, 2*dim) int8: 0 none, 1 free surface, 2 rigid, 3 absorbing"""
+        k = np.ascontiguousarray(kind, dtype=np.int8).ravel()
+        _check(lib().bfamx_set_boundary(self.h, k))
 
     def set_source(self, elem, weights, amp, f0, t0):
         a = np.zeros(9)

@@ -133,6 +133,33 @@ struct SubDev
 /* nbr_code flag of a face whose flux comes from the mortars */
 constexpr int NBR_GLUED = 0x80;
 
+/* nbr_code of a physical boundary face: NBR_BOUNDARY | kind << 3 | own face.
+ * The plus state is the face's own trace, mirrored by kind:
+ *   1 free surface (traction-free / pressure-free): v+ = v-, sigma+ = -sigma-
+ *   2 rigid wall (no-slip / no normal flow):          v+ = -v-, sigma+ = sigma-
+ *   3 absorbing (characteristic):                    v+ = 0,  sigma+ = 0
+ * so that the exact Riemann solution at the face meets the condition. */
+constexpr int NBR_BOUNDARY = 0x40;
+
+/* orientation bits and mirror signs of a (non-glued) face code */
+__device__ __forceinline__ int nbr_perm(int code)
+{
+  return (code & NBR_BOUNDARY) ? 0 : (code >> 3) & 7;
+}
+template <class T>
+__device__ __forceinline__ void nbr_mirror(int code, T *sv, T *ss)
+{
+  if (!(code & NBR_BOUNDARY))
+  {
+    *sv = T(1);
+    *ss = T(1);
+    return;
+  }
+  const int kind = (code >> 3) & 7;
+  *sv = kind == 1 ? T(1) : (kind == 2 ? T(-1) : T(0));
+  *ss = kind == 1 ? T(-1) : (kind == 2 ? T(1) : T(0));
+}
+
 /* glued faces are at most order 7: (N+1)^2 <= 64 nodes, one per lane */
 constexpr int GLUE_MAXFP = 64;
 

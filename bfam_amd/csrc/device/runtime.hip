@@ -1735,3 +1735,42 @@ extern "C" int bfamx_set_persistent(bfamx_device_t *d, int on)
   d->persistent = on;
   return BFAMX_OK;
 }
+
+/* ---------------------------------------------------------------------- */
+/* physical boundaries                                                     */
+/* ---------------------------------------------------------------------- */
+
+extern "C" int bfamx_set_boundary(bfamx_subdomain_t *s, const int8_t *kind)
+{
+  if (!s || !kind)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_boundary: NULL argument");
+  if (s->glue)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_set_boundary: set boundaries before gluing");
+  const int nfaces = 2 * s->dim;
+  const size_t nfc = (size_t)s->K * nfaces;
+  std::vector<int32_t> ne(nfc);
+  std::vector<uint8_t> nc(nfc);
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  HIPCHK(hipMemcpy(ne.data(), s->h.nbr_elem, 4 * nfc, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(nc.data(), s->h.nbr_code, nfc, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < nfc; ++k)
+  {
+    const int e = (int)(k / nfaces), f = (int)(k % nfaces);
+    const int b = kind[k];
+    if (b < 0 || b > 3)
+      return bfamx_fail(BFAMX_EINVAL, "bfamx_set_boundary: kind %d at face %d of %d", b, f, e);
+    const int self = ne[k] == e && (nc[k] & 7) == f && !(nc[k] & NBR_GLUED);
+    if (b == 0)
+    {
+      if (nc[k] & NBR_BOUNDARY)
+        nc[k] = (uint8_t)f; /* back to the plain self face */
+      continue;
+    }
+    if (!self)
+      return bfamx_fail(BFAMX_EINVAL,
+                        "bfamx_set_boundary: face %d of element %d has a neighbour", f, e);
+    nc[k] = (uint8_t)(NBR_BOUNDARY | (b << 3) | f);
+  }
+  HIPCHK(hipMemcpy((void *)s->h.nbr_code, nc.data(), nfc, hipMemcpyHostToDevice));
+  return BFAMX_OK;
+}

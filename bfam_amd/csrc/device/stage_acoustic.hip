@@ -122,14 +122,16 @@ __device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
       const int nm = face_node<2>(f, n1, a, 0);
       const int32_t ep = glob(sd.nbr_elem)[(size_t)et * 4 + f];
       const int code = glob(sd.nbr_code)[(size_t)et * 4 + f];
-      const int oi = perm_apply(code >> 3, n1, a, 0);
+      const int oi = perm_apply(nbr_perm(code), n1, a, 0);
       const int np = face_node<2>(code & 7, n1, oi, 0);
       const TC *u = sq + lt * NF * Np;
       const int axis = f >> 1;
       const TC s = (f & 1) ? TC(1) : TC(-1);
+      TC sv, ss; /* pressure plays the stress: free surface p+ = -p- */
+      nbr_mirror<TC>(code, &sv, &ss);
       const TC pm = u[nm], vnm = s * u[(1 + axis) * Np + nm];
-      const TC pp = (TC)qin[qidx(ep, 0, np, NF, Np)];
-      const TC vnp = s * (TC)qin[qidx(ep, 1 + axis, np, NF, Np)];
+      const TC pp = ss * (TC)qin[qidx(ep, 0, np, NF, Np)];
+      const TC vnp = sv * s * (TC)qin[qidx(ep, 1 + axis, np, NF, Np)];
       gptr<const TC> Mm = mat_tab<TC>(sd) + (size_t)et * MAT_STRIDE;
       gptr<const TC> Mp = mat_tab<TC>(sd) + (size_t)ep * MAT_STRIDE;
       fl[r] = (Mp[MAT_ZP] * (vnp - vnm) + (pm - pp)) / (Mm[MAT_ZP] + Mp[MAT_ZP]);

@@ -104,22 +104,24 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
         const int nm = face_node<3>(f, n1, a, b);
         const int32_t ep = nbe[(size_t)et * 6 + f];
         const int code = nbc[(size_t)et * 6 + f];
-        const int oi = perm_apply(code >> 3, n1, a, b);
+        const int oi = perm_apply(nbr_perm(code), n1, a, b);
         const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
         gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
         const TC *u = sq + lt * NF * Np + nm;
+        TC sv, ss;
+        nbr_mirror<TC>(code, &sv, &ss);
         TC vm[3], sm[6], vp[3], spl[6];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
         {
           vm[c] = u[c * Np];
-          vp[c] = (TC)qp[c * Np];
+          vp[c] = sv * (TC)qp[c * Np];
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c)
         {
           sm[c] = u[(3 + c) * Np];
-          spl[c] = (TC)qp[(3 + c) * Np];
+          spl[c] = ss * (TC)qp[(3 + c) * Np];
         }
         gptr<const TC> Mm = mtab + (size_t)et * MAT_STRIDE;
         gptr<const TC> Mp = mtab + (size_t)ep * MAT_STRIDE;

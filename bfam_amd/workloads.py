@@ -233,11 +233,27 @@ def elastic3d_octree(n_base=16, levels=2, seed=2024, radius=0.2, orders=(7, 5, 3
             "dof": sum(9 * sm.K * sm.Np for sm in subs)}
 
 
+def boundary_kinds(mesh, kind_of_face):
+    """(K, 2 dim) int8 boundary kinds: kind_of_face[f] on every face f that
+    has no neighbour (maps to itself), 0 elsewhere."""
+    nf = 2 * mesh.dim
+    E = mesh.EToE.reshape(mesh.K, nf)
+    F = mesh.EToF.reshape(mesh.K, nf)
+    out = np.zeros((mesh.K, nf), np.int8)
+    for f in range(nf):
+        self_face = (E[:, f] == np.arange(mesh.K)) & (F[:, f] == f)
+        out[self_face, f] = kind_of_face[f]
+    return out
+
+
 def upload(dev, cfg, precision):
-    """bfam_amd.Subdomain for a config dict (source / forcing attached)."""
+    """bfam_amd.Subdomain for a config dict (boundaries, source and forcing
+    attached)."""
     m = cfg["mesh"]
     sub = bx.Subdomain(dev, cfg["physics"], precision, m.N, m.r, m.w, m.Dr, m.vmapM, m.vmapP,
                        m.h, cfg["rho"], cfg["cp"], cfg.get("cs"), x0=m.x0)
+    if cfg.get("bc") is not None:
+        sub.set_boundary(cfg["bc"])
     if cfg.get("src") is not None:
         s = cfg["src"]
         sub.set_source(s["elem"], s["weights"], s["amp"], s["f0"], s["t0"])

@@ -222,6 +222,17 @@ int bfamx_subdomain_info(const bfamx_subdomain_t *sub, int *nfields,
 size_t bfamx_subdomain_bytes(int dim, int N, int32_t K, bfamx_physics_t ph,
                              bfamx_precision_t prec);
 
+/*
+ * Physical boundary conditions on faces that have no neighbour (faces the
+ * maps send to themselves, e.g. non-periodic brick sides): kind[k*2dim + f]
+ * = 0 none, 1 free surface (traction / pressure free), 2 rigid wall
+ * (no-slip velocity / no normal flow), 3 absorbing (characteristic).  The
+ * plus state of the upwind flux is the own trace mirrored accordingly.  The
+ * reference routes such faces through _glue_boundary subdomains
+ * (bfam.c:7851-7899) and leaves the condition to the application.
+ */
+int bfamx_set_boundary(bfamx_subdomain_t *sub, const int8_t *kind);
+
 /* Ricker point source: rate[f][elem, n] += weights[n]*amp[f]*ricker(t) with
  * ricker = (1-2a)exp(-a), a = (pi f0 (t-t0))^2; weights = l_n(x_s)/(J w_n) */
 int bfamx_set_source(bfamx_subdomain_t *sub, int32_t elem,

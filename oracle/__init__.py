@@ -34,6 +34,7 @@ class Problem(C.Structure):
         ("mms", C.c_int), ("mms_k", C.c_double), ("mms_omega", C.c_double),
         ("mms_F", C.c_double * 9),
         ("glued", C.c_void_p),
+        ("bc", C.c_void_p),
     ]
 
 
@@ -243,7 +244,7 @@ class OracleProblem:
     """
 
     def __init__(self, physics, N, vmapM, vmapP, h, rho, lam, mu=None, x0=None,
-                 src=None, mms=None, ops=None, glued=None):
+                 src=None, mms=None, ops=None, glued=None, bc=None):
         dim = 2 if physics == ACOUSTIC2D else 3
         self.dim, self.N, self.physics = dim, N, physics
         self.Np = (N + 1) ** dim
@@ -284,6 +285,9 @@ class OracleProblem:
         if glued is not None:
             self._glued = np.ascontiguousarray(glued, dtype=np.int8).ravel()
             p.glued = self._glued.ctypes.data
+        if bc is not None:
+            self._bc = np.ascontiguousarray(bc, dtype=np.int8).ravel()
+            p.bc = self._bc.ctypes.data
         self.p = p
 
     def zeros(self):

@@ -1076,6 +1076,11 @@ static void rhs_elastic3d(const orc_problem_t *p, double t, const double *q,
         const double rhop = p->rho[ep], lamp = p->lam[ep], mup = p->mu[ep];
         const double Zpp = sqrt(rhop * (lamp + 2 * mup)), Zsp = sqrt(rhop * mup);
         double vm[3], vp[3], Tm[3], Tp[3], S[2][3][3];
+        /* physical boundary: plus state = own trace mirrored (free surface:
+         * sigma+ = -sigma-; rigid: v+ = -v-; absorbing: zero) */
+        const int kind = p->bc ? p->bc[(size_t)e * 6 + f] : 0;
+        const double msv = kind == 0 ? 1 : (kind == 1 ? 1 : (kind == 2 ? -1 : 0));
+        const double mss = kind == 0 ? 1 : (kind == 1 ? -1 : (kind == 2 ? 1 : 0));
         for (int side = 0; side < 2; ++side)
         {
           const size_t id = side ? (size_t)idP : (size_t)idM;
@@ -1087,10 +1092,11 @@ static void rhs_elastic3d(const orc_problem_t *p, double t, const double *q,
           Sg[0][2] = Sg[2][0] = q[7 * KNp + id];
           Sg[1][2] = Sg[2][1] = q[8 * KNp + id];
           double *v = side ? vp : vm, *T = side ? Tp : Tm;
+          const double fv = side ? msv : 1.0, fs = side ? mss : 1.0;
           for (int c = 0; c < 3; ++c)
           {
-            v[c] = q[c * KNp + id];
-            T[c] = Sg[c][0] * nrm[0] + Sg[c][1] * nrm[1] + Sg[c][2] * nrm[2];
+            v[c] = fv * q[c * KNp + id];
+            T[c] = fs * (Sg[c][0] * nrm[0] + Sg[c][1] * nrm[1] + Sg[c][2] * nrm[2]);
           }
         }
         double dvv[3], dT[3];
@@ -1168,11 +1174,15 @@ static void rhs_acoustic2d(const orc_problem_t *p, double t, const double *q,
         const int32_t ep = idP / Np;
         const int nm = idM - e * Np;
         const double Zp = sqrt(p->rho[ep] * p->lam[ep]);
+        const int kind = p->bc ? p->bc[(size_t)e * 4 + f] : 0;
+        const double msv = kind == 0 ? 1 : (kind == 1 ? 1 : (kind == 2 ? -1 : 0));
+        const double mss = kind == 0 ? 1 : (kind == 1 ? -1 : (kind == 2 ? 1 : 0));
         const double vnm = q[1 * KNp + idM] * nrm[0] + q[2 * KNp + idM] * nrm[1];
-        const double vnp = q[1 * KNp + idP] * nrm[0] + q[2 * KNp + idP] * nrm[1];
-        /* traction normal component T_n = -p */
+        const double vnp =
+            msv * (q[1 * KNp + idP] * nrm[0] + q[2 * KNp + idP] * nrm[1]);
+        /* traction normal component T_n = -p; a free surface mirrors p */
         const double dvn =
-            (Zp * (vnp - vnm) + (-q[idP] + q[idM])) / (Zm + Zp);
+            (Zp * (vnp - vnm) + (-mss * q[idP] + q[idM])) / (Zm + Zp);
         rU[nm] += lift * Zm * dvn * nrm[0] / rho;
         rV[nm] += lift * Zm * dvn * nrm[1] / rho;
         rP[nm] += -lift * K * dvn;

@@ -142,6 +142,10 @@ typedef struct
   double mms_F[9];
   /* K*6 flags (elastic): faces whose flux comes from mortars; NULL: none */
   const int8_t *glued;
+  /* K*2dim physical boundary kinds of self-mapped faces: 0 none, 1 free
+   * surface, 2 rigid wall, 3 absorbing (plus state = mirrored own trace);
+   * NULL: none */
+  const int8_t *bc;
 } orc_problem_t;
 
 int orc_nfields(int physics);

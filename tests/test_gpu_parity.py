@@ -309,3 +309,63 @@ def test_persistent_launch_invariant(dev, prec):
             s.close()
     dev.set_persistent(0)
     assert np.array_equal(res[0], res[2]) and np.array_equal(res[1], res[3])
+
+
+def bc_elastic(N, seed):
+    rng = np.random.default_rng(seed)
+    mesh = W.Mesh(3, [3, 2, 3], N, L=[1.0, 0.7, 1.2], periodic=[0, 0, 1])
+    K = mesh.K
+    rho = rng.uniform(1.5, 3.0, K)
+    cp = rng.uniform(3.0, 6.0, K)
+    cs = cp / rng.uniform(1.6, 2.0, K)
+    cfg = {"mesh": mesh, "rho": rho, "cp": cp, "cs": cs, "physics": bx.ELASTIC3D,
+           "dt": W.cfl_dt(float(mesh.h.min()), float(cp.max()), N, 0.5)}
+    cfg["bc"] = W.boundary_kinds(mesh, [bx.BC_RIGID, bx.BC_FREE, bx.BC_ABSORB, bx.BC_FREE, 0, 0])
+    return cfg, rng.standard_normal((9, K * mesh.Np))
+
+
+@pytest.mark.parametrize("N", [4, 7])
+@pytest.mark.parametrize("prec", PRECS)
+def test_elastic_boundaries_rhs(dev, N, prec):
+    """Free-surface, rigid and absorbing faces against the oracle."""
+    print(f"elastic boundaries N={N} prec={prec}", flush=True)
+    cfg, q0 = bc_elastic(N, 80 + N)
+    assert (cfg["bc"] > 0).sum() > 0
+    s = W.upload(dev, cfg, prec)
+    s.set_fields(q0, 0.0, cfg["dt"])
+    got = s.rhs(0.0)
+    s.close()
+    want = oracle_problem(cfg).rhs(q0, 0.0)
+    assert rel_max(got, want) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_acoustic_boundaries_rhs_and_steps(dev, prec):
+    print(f"acoustic boundaries prec={prec}", flush=True)
+    rng = np.random.default_rng(5)
+    mesh = W.Mesh(2, [4, 3], 5, L=[1.0, 0.8], periodic=[0, 0])
+    K = mesh.K
+    cfg = {"mesh": mesh, "rho": rng.uniform(1, 2, K), "cp": rng.uniform(1, 3, K), "cs": None,
+           "physics": bx.ACOUSTIC2D}
+    cfg["dt"] = W.cfl_dt(float(mesh.h.min()), float(cfg["cp"].max()), 5, 0.5)
+    cfg["bc"] = W.boundary_kinds(mesh, [bx.BC_FREE, bx.BC_RIGID, bx.BC_ABSORB, bx.BC_RIGID])
+    q0 = rng.standard_normal((3, K * mesh.Np))
+    s = W.upload(dev, cfg, prec)
+    s.set_fields(q0, 0.0, cfg["dt"])
+    got = s.rhs(0.0)
+    s.set_fields(q0, 0.0, cfg["dt"])
+    s.step(3)
+    q3 = s.download()
+    s.close()
+    P = oracle_problem(cfg)
+    assert rel_max(got, P.rhs(q0, 0.0)) < TOL[prec]
+    assert rel_max(q3, P.run(q0, cfg["dt"], 3)) < TOL[prec]
+
+
+def test_boundary_rejects_interior_face(dev):
+    cfg, q0 = random_elastic((2, 2, 2), 3, seed=1)
+    s = W.upload(dev, cfg, bx.FP64)
+    with pytest.raises(bx.BfamxError) as e:
+        s.set_boundary(np.ones((cfg["mesh"].K, 6), np.int8))
+    assert e.value.code == bx.EINVAL
+    s.close()

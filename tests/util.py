@@ -20,7 +20,7 @@ def oracle_problem(cfg, mms=None):
         mms = {"k": cfg["k"], "omega": cfg["omega"], "F": cfg["F"]}
     return oracle.OracleProblem(phys, m.N, m.vmapM, m.vmapP, m.h, cfg["rho"], lam, mu,
                                 x0=m.x0, src=cfg.get("src"), mms=mms,
-                                ops=(m.r, m.w, 1 / m.w, m.Dr))
+                                ops=(m.r, m.w, 1 / m.w, m.Dr), bc=cfg.get("bc"))
 
 
 def random_elastic(n, N, seed=0, L=(1.0, 1.3, 0.8), src=True):
