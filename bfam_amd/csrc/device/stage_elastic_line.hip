@@ -225,13 +225,22 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       const int oi = perm_apply(nbr_perm(code), n1, pa, pb);
       const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
       gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
-      TC sv, ss;
-      nbr_mirror<TC>(code, &sv, &ss);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
       {
-        nb[side][c] = sv * (TC)qp[c * Np];
-        nb[side][3 + c] = ss * (TC)qp[sfield(c, A) * Np];
+        nb[side][c] = (TC)qp[c * Np];
+        nb[side][3 + c] = (TC)qp[sfield(c, A) * Np];
+      }
+      if (code & NBR_BOUNDARY)
+      {
+        TC sv, ss;
+        nbr_mirror<TC>(code, &sv, &ss);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+          nb[side][c] *= sv;
+          nb[side][3 + c] *= ss;
+        }
       }
     }
   };
