@@ -155,6 +155,8 @@ SIGNATURES = {
     "bfamx_glue_free": (C.c_int, [_vp]),
     "bfamx_batch_set_glue": (C.c_int, [_vp, _vp]),
     "bfamx_batch_rhs": (C.c_int, [_vp, C.c_double, _vp]),
+    "bfamx_vtk_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int32, _dp, _dp, _dp,
+                                  C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_int]),
     "bfamx_serve_generate": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(Request)]),
@@ -529,6 +531,22 @@ class Glue:
         if self.h:
             _check(lib().bfamx_glue_free(self.h))
             self.h = None
+
+
+# ------------------------------------------------------------------- output
+
+def vtk_write(path, dim, N, x0, h, r, fields, time=0.0, binary=True, Np_write=0):
+    """fields: dict name -> (K*(N+1)^dim,) array in bfam node order."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).ravel()
+    h = np.ascontiguousarray(h, dtype=np.float64).ravel()
+    K = x0.size // dim
+    names = list(fields)
+    arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in names]
+    cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    _check(lib().bfamx_vtk_write(str(path).encode(), dim, N, K, x0, h,
+                                 np.ascontiguousarray(r, dtype=np.float64), len(names),
+                                 C.cast(cn, C.c_void_p), _ptrs(arrs) if arrs else None,
+                                 float(time), int(binary), int(Np_write)))
 
 
 # ------------------------------------------------------------- adapted mesh

@@ -1,0 +1,246 @@
+/*
+ * VTK XML unstructured-grid (.vtu) output of a subdomain's nodal fields.
+ *
+ * Follows what the reference writes per subdomain piece
+ * (bfam_subdomain_dgx_vtk_write_vtu_piece, bfam.c:6050-6572, and the
+ * resampling of bfam_subdomain_dgx_vtk_interp, bfam.c:5872-5929): the nodal
+ * values, optionally resampled to Np_write equispaced points per direction
+ * ("poor man's high-order viz"), every element split into N_vtk^dim linear
+ * cells (quads / hexahedra), a "time" cell array, and one point array per
+ * field.  Binary output is VTK's inline base64 with a 64-bit byte-count
+ * header; zlib compression (the reference's sc_vtk_write_compressed) is not
+ * offered.  Affine elements: node coordinates come from each element's lower
+ * corner and edge lengths.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bfamx_internal.h"
+
+static const char b64[] =
+    "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+
+/* base64 of [uint64 nbytes][data], as VTK reads "binary" inline arrays
+ * with header_type="UInt64" */
+static void write_base64(FILE *fp, const void *data, size_t nbytes)
+{
+  const uint64_t hdr = nbytes;
+  const size_t total = 8 + nbytes;
+  unsigned char *buf = malloc(total);
+  memcpy(buf, &hdr, 8);
+  memcpy(buf + 8, data, nbytes);
+  for (size_t i = 0; i < total; i += 3)
+  {
+    const unsigned a = buf[i];
+    const unsigned b = i + 1 < total ? buf[i + 1] : 0;
+    const unsigned c = i + 2 < total ? buf[i + 2] : 0;
+    const unsigned v = (a << 16) | (b << 8) | c;
+    fputc(b64[(v >> 18) & 63], fp);
+    fputc(b64[(v >> 12) & 63], fp);
+    fputc(i + 1 < total ? b64[(v >> 6) & 63] : '=', fp);
+    fputc(i + 2 < total ? b64[v & 63] : '=', fp);
+  }
+  fputc('\n', fp);
+  free(buf);
+}
+
+static void write_f64(FILE *fp, const char *name, int ncomp, const double *v, size_t n,
+                      int binary)
+{
+  fprintf(fp,
+          "        <DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"%d\" "
+          "format=\"%s\">\n",
+          name, ncomp, binary ? "binary" : "ascii");
+  if (binary)
+    write_base64(fp, v, sizeof(double) * n * (size_t)ncomp);
+  else
+    for (size_t i = 0; i < n * (size_t)ncomp; ++i)
+      fprintf(fp, "%.17g\n", v[i]);
+  fprintf(fp, "        </DataArray>\n");
+}
+
+static void write_i64(FILE *fp, const char *name, const int64_t *v, size_t n, int binary)
+{
+  fprintf(fp, "        <DataArray type=\"Int64\" Name=\"%s\" format=\"%s\">\n", name,
+          binary ? "binary" : "ascii");
+  if (binary)
+    write_base64(fp, v, sizeof(int64_t) * n);
+  else
+    for (size_t i = 0; i < n; ++i)
+      fprintf(fp, "%lld\n", (long long)v[i]);
+  fprintf(fp, "        </DataArray>\n");
+}
+
+static void write_u8(FILE *fp, const char *name, const uint8_t *v, size_t n, int binary)
+{
+  fprintf(fp, "        <DataArray type=\"UInt8\" Name=\"%s\" format=\"%s\">\n", name,
+          binary ? "binary" : "ascii");
+  if (binary)
+    write_base64(fp, v, n);
+  else
+    for (size_t i = 0; i < n; ++i)
+      fprintf(fp, "%u\n", (unsigned)v[i]);
+  fprintf(fp, "        </DataArray>\n");
+}
+
+int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *elem_x0,
+                    const double *elem_h, const double *r, int nfields,
+                    const char *const *names, const double *const *fields, double time,
+                    int binary, int Np_write)
+{
+  if (!path || (dim != 2 && dim != 3) || N < 1 || K < 0 || !elem_x0 || !elem_h || !r ||
+      nfields < 0 || (nfields && (!names || !fields)) || Np_write == 1 || Np_write < 0)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: bad arguments");
+  const int n1 = N + 1;
+  const int nv = Np_write > 0 ? Np_write : n1; /* points per direction */
+  const int Npw = dim == 2 ? nv * nv : nv * nv * nv;
+  const int Nps = dim == 2 ? n1 * n1 : n1 * n1 * n1;
+  /* 1-D interpolation from the LGL nodes to the output points */
+  double *I = malloc(sizeof(double) * (size_t)nv * n1);
+  double *ref = malloc(sizeof(double) * nv);
+  for (int p = 0; p < nv; ++p)
+  {
+    ref[p] = Np_write > 0 ? -1.0 + 2.0 * p / (nv - 1) : r[p];
+    bfamx_lagrange_eval(N, r, ref[p], I + (size_t)p * n1);
+  }
+  const size_t npts = (size_t)K * Npw;
+  const int cpe = dim == 2 ? (nv - 1) * (nv - 1) : (nv - 1) * (nv - 1) * (nv - 1);
+  const size_t ncells = (size_t)K * cpe;
+  const int nvert = dim == 2 ? 4 : 8;
+  double *xyz = malloc(sizeof(double) * 3 * npts);
+  int64_t *conn = malloc(sizeof(int64_t) * nvert * ncells);
+  int64_t *offs = malloc(sizeof(int64_t) * ncells);
+  uint8_t *type = malloc(ncells);
+  double *tcell = malloc(sizeof(double) * (ncells ? ncells : 1));
+  double *val = malloc(sizeof(double) * (npts ? npts : 1));
+  FILE *fp = fopen(path, "w");
+  int rc = BFAMX_OK;
+  if (!I || !ref || !xyz || !conn || !offs || !type || !tcell || !val || !fp)
+  {
+    rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: cannot write %s", path);
+    goto done;
+  }
+  for (int32_t e = 0; e < K; ++e)
+    for (int p = 0; p < Npw; ++p)
+    {
+      const int idx[3] = {p % nv, (p / nv) % nv, p / (nv * nv)};
+      double *X = xyz + 3 * ((size_t)e * Npw + p);
+      X[2] = 0;
+      for (int d = 0; d < dim; ++d)
+        X[d] = elem_x0[(size_t)e * dim + d] +
+               0.5 * (ref[idx[d]] + 1.0) * elem_h[(size_t)e * dim + d];
+    }
+  {
+    size_t c = 0;
+    for (int32_t e = 0; e < K; ++e)
+    {
+      const int64_t base = (int64_t)e * Npw;
+      const int nk = dim == 3 ? nv - 1 : 1;
+      for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nv - 1; ++j)
+          for (int i = 0; i < nv - 1; ++i, ++c)
+          {
+            int64_t *v = conn + (size_t)nvert * c;
+            const int64_t o = base + i + (int64_t)nv * (j + (int64_t)nv * k);
+            const int64_t di = 1, dj = nv, dk = (int64_t)nv * nv;
+            v[0] = o;
+            v[1] = o + di;
+            v[2] = o + di + dj;
+            v[3] = o + dj;
+            if (dim == 3)
+            {
+              v[4] = o + dk;
+              v[5] = o + di + dk;
+              v[6] = o + di + dj + dk;
+              v[7] = o + dj + dk;
+            }
+            offs[c] = (int64_t)nvert * (int64_t)(c + 1);
+            type[c] = dim == 2 ? 9 : 12; /* VTK_QUAD, VTK_HEXAHEDRON */
+            tcell[c] = time;
+          }
+    }
+  }
+  fprintf(fp, "<?xml version=\"1.0\"?>\n");
+  fprintf(fp,
+          "<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+          "header_type=\"UInt64\">\n");
+  fprintf(fp, "  <UnstructuredGrid>\n");
+  fprintf(fp, "    <Piece NumberOfPoints=\"%zu\" NumberOfCells=\"%zu\">\n", npts, ncells);
+  fprintf(fp, "      <Points>\n");
+  write_f64(fp, "Position", 3, xyz, npts, binary);
+  fprintf(fp, "      </Points>\n");
+  fprintf(fp, "      <Cells>\n");
+  write_i64(fp, "connectivity", conn, (size_t)nvert * ncells, binary);
+  write_i64(fp, "offsets", offs, ncells, binary);
+  write_u8(fp, "types", type, ncells, binary);
+  fprintf(fp, "      </Cells>\n");
+  fprintf(fp, "      <CellData Scalars=\"time\">\n");
+  write_f64(fp, "time", 1, tcell, ncells, binary);
+  fprintf(fp, "      </CellData>\n");
+  fprintf(fp, "      <PointData>\n");
+  {
+    /* sum factorisation: one 1-D interpolation per direction */
+    const int nk = dim == 3 ? n1 : 1, mk = dim == 3 ? nv : 1;
+    double *t1 = malloc(sizeof(double) * (size_t)nv * n1 * n1);
+    double *t2 = malloc(sizeof(double) * (size_t)nv * nv * n1);
+    for (int f = 0; f < nfields; ++f)
+    {
+      const double *s = fields[f];
+      for (int32_t e = 0; e < K; ++e)
+      {
+        const double *u = s + (size_t)e * Nps;
+        for (int k = 0; k < nk; ++k)
+          for (int j = 0; j < n1; ++j)
+            for (int a = 0; a < nv; ++a)
+            {
+              double acc = 0;
+              for (int i = 0; i < n1; ++i)
+                acc += I[(size_t)a * n1 + i] * u[i + n1 * (j + n1 * k)];
+              t1[a + (size_t)nv * (j + (size_t)n1 * k)] = acc;
+            }
+        for (int k = 0; k < nk; ++k)
+          for (int b = 0; b < nv; ++b)
+            for (int a = 0; a < nv; ++a)
+            {
+              double acc = 0;
+              for (int j = 0; j < n1; ++j)
+                acc += I[(size_t)b * n1 + j] * t1[a + (size_t)nv * (j + (size_t)n1 * k)];
+              t2[a + (size_t)nv * (b + (size_t)nv * k)] = acc;
+            }
+        double *o = val + (size_t)e * Npw;
+        for (int c3 = 0; c3 < mk; ++c3)
+          for (int b = 0; b < nv; ++b)
+            for (int a = 0; a < nv; ++a)
+            {
+              double acc = 0;
+              if (dim == 3)
+                for (int k = 0; k < n1; ++k)
+                  acc += I[(size_t)c3 * n1 + k] * t2[a + (size_t)nv * (b + (size_t)nv * k)];
+              else
+                acc = t2[a + (size_t)nv * b];
+              o[a + (size_t)nv * (b + (size_t)nv * c3)] = acc;
+            }
+      }
+      write_f64(fp, names[f], 1, val, npts, binary);
+    }
+    free(t1);
+    free(t2);
+  }
+  fprintf(fp, "      </PointData>\n");
+  fprintf(fp, "    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n");
+  if (ferror(fp))
+    rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: write error on %s", path);
+done:
+  if (fp && fclose(fp) && !rc)
+    rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: cannot close %s", path);
+  free(I);
+  free(ref);
+  free(xyz);
+  free(conn);
+  free(offs);
+  free(type);
+  free(tcell);
+  free(val);
+  return rc;
+}

@@ -376,6 +376,23 @@ int bfamx_glue_free(bfamx_glue_t *g);
 int bfamx_batch_set_glue(bfamx_batch_t *b, bfamx_glue_t *g);
 
 /* ---------------------------------------------------------------------- */
+/* output                                                                  */
+/* ---------------------------------------------------------------------- */
+/*
+ * One subdomain's nodal fields (host arrays of K*(N+1)^dim doubles, bfam
+ * node order) to a VTK XML unstructured-grid file: every element split into
+ * linear cells through Np_write equispaced points per direction (0: the LGL
+ * nodes), a "time" cell array and one point array per field; ascii or
+ * base64 binary.  The per-subdomain piece of bfam_vtk_write_file
+ * (bfam.c:6050-6572, 8234-8326) for affine elements.
+ */
+int bfamx_vtk_write(const char *path, int dim, int N, int32_t K,
+                    const double *elem_x0, const double *elem_h, const double *r,
+                    int nfields, const char *const *names,
+                    const double *const *fields, double time, int binary,
+                    int Np_write);
+
+/* ---------------------------------------------------------------------- */
 /* serving: independent elastic solves admitted under an HBM budget as they */
 /* arrive and advanced together by one batched launch per LSRK stage.  No   */
 /* counterpart in the reference (the budget check follows                   */
