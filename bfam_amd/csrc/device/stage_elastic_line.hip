@@ -54,9 +54,11 @@ template <int N, class TL, class TC> struct LineCfg
   static constexpr int FS = S * n1;    /* words per field of one element */
   static constexpr int FST = EPW * FS; /* field stride: the wave's elements */
   /* per wavefront: 9 staged state fields (type TL), field-major over the
-   * wave's elements so that a coalesced row lands at consecutive words,
-   * then one transpose field per element */
-  static constexpr size_t bytes = EPW * FS * (9 * sizeof(TL) + sizeof(TC));
+   * wave's elements so that a coalesced row lands at consecutive words.
+   * The transposes of the y and z passes and of the output reuse the
+   * region of sigma_11, which only the x pass reads (its row values are
+   * kept in registers for the update), so no separate buffer is needed */
+  static constexpr size_t bytes = EPW * FS * 9 * sizeof(TL);
 };
 
 /* vector width of the coalesced row accesses: 16-byte loads when every
@@ -182,7 +184,9 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   const int e = tk.e0 + (act ? le : 0);
   const int pa = L % n1, pb = L / n1; /* (a, b) of this lane in every pass */
   TL *sq = reinterpret_cast<TL *>(lds) + le * FS; /* staged state, field stride FST */
-  TC *tb = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS) +
+  static_assert(std::is_same<TL, TC>::value, "the transposes share the LDS state");
+  constexpr int TBF = 3; /* sigma_11: its region is the transpose buffer after x */
+  TC *tb = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + TBF * FST) +
            le * FS; /* one-field transpose buffer */
 
   gptr<const TS> qin = glob(static_cast<const TS *>(sd.q[cx.parity]));
@@ -485,6 +489,27 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   };
 
   pass(std::integral_constant<int, 0>{}, nbx);
+  /* sigma_11 is read by nothing after the x pass but the update: keep this
+   * lane's row values of it and free its region for the transposes */
+  const bool keep_s11 = q_from_lds && !(mode == MODE_RHS || PART != PART_FUSED);
+  TL s11[ROWS][VW];
+  if (keep_s11)
+  {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+    {
+      int lel, n;
+      if (!coal_pos(r, &lel, &n))
+        continue;
+      const TL *src = reinterpret_cast<TL *>(lds) + TBF * FST + lel * FS;
+#pragma unroll
+      for (int t = 0; t < VW; ++t)
+        s11[r][t] = src[lds_node(n + t)];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   TC nbz[2][6];
   if (act && PART != PART_VOLUME)
     gather(2, nbz);
@@ -567,7 +592,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
     for (int f = 0; f < PD; ++f)
       fetch(f);
   }
-  const TC *tb0 = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + CF::EPW * NF * FS);
+  const TC *tb0 = reinterpret_cast<TC *>(reinterpret_cast<TL *>(lds) + TBF * FST);
 #pragma unroll
   for (int f = 0; f < NF; ++f)
   {
@@ -596,10 +621,19 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
           rv[r][t] = x[t];
         if constexpr (q_from_lds)
         {
-          const VL y = *reinterpret_cast<const VL *>(sqe + lds_node(n));
+          if (f == TBF)
+          {
 #pragma unroll
-          for (int t = 0; t < VW; ++t)
-            qo_l[r][t] = y[t];
+            for (int t = 0; t < VW; ++t)
+              qo_l[r][t] = s11[r][t];
+          }
+          else
+          {
+            const VL y = *reinterpret_cast<const VL *>(sqe + lds_node(n));
+#pragma unroll
+            for (int t = 0; t < VW; ++t)
+              qo_l[r][t] = y[t];
+          }
         }
       }
       else
@@ -609,7 +643,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         {
           rv[r][t] = tbe[lds_node(n + t)];
           if constexpr (q_from_lds)
-            qo_l[r][t] = sqe[lds_node(n + t)];
+            qo_l[r][t] = f == TBF ? s11[r][t] : sqe[lds_node(n + t)];
         }
       }
     }
