@@ -140,6 +140,8 @@ SIGNATURES = {
     "bfamx_batch_launch_graph": (C.c_int, [_vp]),
     "bfamx_batch_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "bfamx_batch_set_strategy": (C.c_int, [_vp, C.c_int]),
+    "bfamx_set_kernel": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "bfamx_get_kernel": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "bfamx_timer_start": (C.c_int, [_vp]),
     "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
@@ -332,6 +334,15 @@ class Device:
 
     def set_tuning(self, xcd_remap=1):
         _check(lib().bfamx_set_tuning(self.h, int(xcd_remap)))
+
+    def set_kernel(self, precision, N, kind):
+        """elastic stage kernel for (precision, N): 0 line, 1 node, -1 unknown"""
+        _check(lib().bfamx_set_kernel(self.h, int(precision), int(N), int(kind)))
+
+    def get_kernel(self, precision, N):
+        k = C.c_int()
+        _check(lib().bfamx_get_kernel(self.h, int(precision), int(N), C.byref(k)))
+        return k.value
 
     def set_persistent(self, on=1):
         _check(lib().bfamx_set_persistent(self.h, int(on)))

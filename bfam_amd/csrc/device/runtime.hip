@@ -46,6 +46,11 @@ struct bfamx_device
    * 2.34 ms with one wavefront per task */
   int persistent;
   int line_bpc[4][9]; /* cached blocks per CU, 0 unknown */
+  /* elastic stage kernel per precision and order, chosen by
+   * bfamx_batch_autotune: -1 unknown (line), 0 line (one wavefront per
+   * element, lanes own lines), 1 node-parallel (512-thread blocks, threads
+   * own nodes) */
+  signed char kchoice[4][9];
 };
 
 struct bfamx_subdomain
@@ -56,6 +61,8 @@ struct bfamx_subdomain
   int state_fp32, rate_fp32;
   SubDev h;        /* host image of the descriptor */
   SubDev *d_sub;   /* device copy */
+  Task *d_tasks_node; /* elastic fused: tasks of the node-parallel kernel */
+  int ntasks_node;
   Task *d_tasks;
   int *d_ntasks;
   int ntasks;
@@ -108,6 +115,10 @@ struct bfamx_batch
   int grouped;
   int order_mask;
   int grp_off[9], grp_cnt[9];
+  /* the same for the node-parallel kernel's tasks (elastic fused) */
+  Task *d_tasks_node;
+  int ntasks_node;
+  int grpn_off[9], grpn_cnt[9];
 };
 
 #define HIPCHK(call)                                                           \
@@ -154,6 +165,7 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
   memset(d->mixed_choice, -1, sizeof(d->mixed_choice));
   d->persistent = 0;
   memset(d->line_bpc, 0, sizeof(d->line_bpc));
+  memset(d->kchoice, -1, sizeof(d->kchoice));
   /* default priority only: streams of different priorities have hung on
    * this hardware */
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
@@ -466,6 +478,19 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     tasks[t].ne = (t * epb + epb <= K) ? epb : K - t * epb;
     tasks[t].N = N;
   }
+  /* the fused elastic stage has two kernels; the node-parallel one takes
+   * elastic_epb(N) elements per task */
+  const int has_node = physics == BFAMX_ELASTIC3D && prec != BFAMX_FP64_PLAIN;
+  const int epn = elastic_epb(N);
+  s->ntasks_node = has_node ? (K + epn - 1) / epn : 0;
+  std::vector<Task> tasks_node(s->ntasks_node > 0 ? s->ntasks_node : 1);
+  for (int t = 0; t < s->ntasks_node; ++t)
+  {
+    tasks_node[t].sub = 0;
+    tasks_node[t].e0 = t * epn;
+    tasks_node[t].ne = (t * epn + epn <= K) ? epn : K - t * epn;
+    tasks_node[t].N = N;
+  }
 
   void *p;
 #define ALLOC(ptr, nb_)                                                        \
@@ -505,6 +530,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     ALLOC(s->d_clock, sizeof(Clock));
     ALLOC(s->d_sub, sizeof(SubDev));
     ALLOC(s->d_tasks, sizeof(Task) * tasks.size());
+    ALLOC(s->d_tasks_node, sizeof(Task) * tasks_node.size());
     ALLOC(s->d_ntasks, sizeof(int));
     ALLOC(s->d_partial, 32 * (size_t)K);
     hipError_t he = hipSuccess;
@@ -524,6 +550,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     CP(r, sub->r, 8 * n1);
     CP(w, sub->w, 8 * n1);
     CP(s->d_tasks, tasks.data(), sizeof(Task) * tasks.size());
+    CP(s->d_tasks_node, tasks_node.data(), sizeof(Task) * tasks_node.size());
     CP(s->d_ntasks, &s->ntasks, sizeof(int));
 #undef CP
     if (he != hipSuccess)
@@ -814,13 +841,26 @@ static int grid_for(const bfamx_device_t *d, int ntasks)
   return d->num_xcd * ((ntasks + d->num_xcd - 1) / d->num_xcd);
 }
 
+static int node_kernel(const bfamx_device_t *d, int prec, int N)
+{
+  return N >= 0 && N <= 8 && d->kchoice[prec & 3][N] == 1;
+}
+
 static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
 {
-  const int grid = s->physics == BFAMX_ELASTIC3D ? grid_line(s->dev, s->prec, s->N, a.ntasks)
-                                                 : grid_for(s->dev, a.ntasks);
-  hipError_t e = s->physics == BFAMX_ELASTIC3D
-                     ? launch_elastic_line(s->prec, s->N, a, grid, s->dev->stream)
-                     : launch_acoustic_stage(s->prec, s->N, a, grid, s->dev->stream);
+  hipError_t e;
+  if (s->physics != BFAMX_ELASTIC3D)
+    e = launch_acoustic_stage(s->prec, s->N, a, grid_for(s->dev, a.ntasks), s->dev->stream);
+  else if (node_kernel(s->dev, s->prec, s->N))
+  {
+    LaunchArgs b = a;
+    b.tasks = s->d_tasks_node;
+    b.ntasks = s->ntasks_node;
+    e = launch_elastic_stage(s->prec, s->N, b, grid_for(s->dev, b.ntasks), s->dev->stream);
+  }
+  else
+    e = launch_elastic_line(s->prec, s->N, a, grid_line(s->dev, s->prec, s->N, a.ntasks),
+                            s->dev->stream);
   if (e == hipSuccess && a.mode == MODE_STAGE)
     e = launch_clock_advance(s->d_sub, 1, s->dev->stream);
   if (e != hipSuccess)
@@ -1116,6 +1156,34 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
   }
   std::stable_sort(tasks.begin(), tasks.end(),
                    [](const Task &x, const Task &y) { return x.N < y.N; });
+  std::vector<Task> tasks_node;
+  if (physics == BFAMX_ELASTIC3D)
+    for (int i = 0; i < nsubs; ++i)
+    {
+      const int epn = elastic_epb(subs[i]->N);
+      for (int32_t e0 = 0; e0 < subs[i]->K; e0 += epn)
+      {
+        Task t;
+        t.sub = i;
+        t.e0 = e0;
+        t.ne = e0 + epn <= subs[i]->K ? epn : subs[i]->K - e0;
+        t.N = subs[i]->N;
+        tasks_node.push_back(t);
+      }
+    }
+  std::stable_sort(tasks_node.begin(), tasks_node.end(),
+                   [](const Task &x, const Task &y) { return x.N < y.N; });
+  b->ntasks_node = (int)tasks_node.size();
+  for (int n = 0; n < 9; ++n)
+    b->grpn_off[n] = b->grpn_cnt[n] = 0;
+  for (int t = 0; t < b->ntasks_node; ++t)
+  {
+    const int n = tasks_node[t].N;
+    if (b->grpn_cnt[n]++ == 0)
+      b->grpn_off[n] = t;
+  }
+  if (tasks_node.empty())
+    tasks_node.resize(1);
   b->ntasks = (int)tasks.size();
   b->grouped = 0;
   b->order_mask = 0;
@@ -1144,6 +1212,11 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
   if (!rc)
   {
     b->d_tasks = (Task *)p;
+    rc = dev_alloc(dev, allocs, &b->bytes, sizeof(Task) * tasks_node.size(), &p);
+  }
+  if (!rc)
+  {
+    b->d_tasks_node = (Task *)p;
     rc = dev_alloc(dev, allocs, &b->bytes, sizeof(int) * nsubs, &p);
   }
   if (rc)
@@ -1155,6 +1228,8 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
   b->d_ntasks = (int *)p;
   if (hipMemcpy(b->d_subs, hs.data(), sizeof(SubDev) * nsubs, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_tasks, tasks.data(), sizeof(Task) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_tasks_node, tasks_node.data(), sizeof(Task) * tasks_node.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_ntasks, per.data(), sizeof(int) * nsubs, hipMemcpyHostToDevice) != hipSuccess)
   {
     dev_free_all(dev, allocs, b->bytes);
@@ -1176,6 +1251,7 @@ extern "C" int bfamx_batch_free(bfamx_batch_t *b)
     (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->d_subs);
   (void)hipFree(b->d_tasks);
+  (void)hipFree(b->d_tasks_node);
   (void)hipFree(b->d_ntasks);
   b->dev->used -= b->bytes;
   delete b;
@@ -1195,23 +1271,42 @@ static GlueArgs glue_args(const bfamx_batch_t *b)
 
 /* the stage kernels of a batch: one launch (one order, or all orders with
  * the mixed kernel), or one launch per order over its slice of the tasks */
+/* the stage kernels of the orders [n] of a batch with their chosen kernel:
+ * the line kernel over the line tasks, or the node-parallel kernel over the
+ * node tasks */
+static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n)
+{
+  LaunchArgs g = a;
+  if (node_kernel(b->dev, b->prec, n))
+  {
+    g.tasks = b->d_tasks_node + b->grpn_off[n];
+    g.ntasks = b->grpn_cnt[n];
+    return launch_elastic_stage(b->prec, n, g, grid_for(b->dev, g.ntasks), b->dev->stream);
+  }
+  g.tasks = a.tasks + b->grp_off[n];
+  g.ntasks = b->grp_cnt[n];
+  return launch_elastic_line(b->prec, n, g, grid_line(b->dev, b->prec, n, g.ntasks),
+                             b->dev->stream);
+}
+
+/* the stage kernels of a batch: one launch (one order, or all orders with
+ * the order-switching kernel), or one launch per order over its slice of
+ * the tasks */
 static hipError_t launch_batch_kernels(bfamx_batch_t *b, const LaunchArgs &a,
                                        int grid)
 {
   if (b->physics != BFAMX_ELASTIC3D)
     return launch_acoustic_stage(b->prec, b->N, a, grid, b->dev->stream);
-  if (b->N >= 0 || !b->grouped)
+  if (b->N >= 0)
+    return launch_order(b, a, b->N);
+  if (!b->grouped)
     return launch_elastic_line(b->prec, b->N, a, grid_line(b->dev, b->prec, b->N, a.ntasks),
                                b->dev->stream);
   for (int n = 0; n < 9; ++n)
   {
     if (!b->grp_cnt[n])
       continue;
-    LaunchArgs g = a;
-    g.tasks = a.tasks + b->grp_off[n];
-    g.ntasks = b->grp_cnt[n];
-    hipError_t e = launch_elastic_line(b->prec, n, g, grid_line(b->dev, b->prec, n, g.ntasks),
-                                       b->dev->stream);
+    hipError_t e = launch_order(b, a, n);
     if (e != hipSuccess)
       return e;
   }
@@ -1598,14 +1693,6 @@ extern "C" int bfamx_batch_rhs(bfamx_batch_t *b, double t, double *const *const 
 /* autotuning of the mixed-order launch                                    */
 /* ---------------------------------------------------------------------- */
 
-/*
- * Times `trials` stage launches of each strategy (one mixed-order launch,
- * one launch per order) on the batch's own data and keeps the faster one,
- * cached on the device per precision and set of orders.  The trial launches
- * do not advance the clocks: they read the current state, write the free
- * ping-pong buffer and dq; dq is saved beforehand and restored, so the
- * batch's state and results are unchanged.
- */
 extern "C" int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order)
 {
   if (!b || per_order < 0 || per_order > 1)
@@ -1616,6 +1703,16 @@ extern "C" int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order)
   return BFAMX_OK;
 }
 
+/*
+ * Autotuning on the batch's own data.  For every order present it times
+ * `trials` stage launches of the line kernel and of the node-parallel
+ * kernel and keeps the faster; with mixed orders it then times one
+ * order-switching launch against one launch per order.  The trial launches
+ * do not advance the clocks: they read the current state, write the free
+ * ping-pong buffer and dq, and dq is saved beforehand and restored, so the
+ * batch's state and results are unchanged.  Choices are cached on the
+ * device (per precision and order, and per precision and set of orders).
+ */
 extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
                                     int *choice)
 {
@@ -1624,7 +1721,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   for (auto *s : b->subs)
     if (!s->fields_set)
       return bfamx_fail(BFAMX_ESTATE, "batch member without state");
-  if (b->N >= 0 || b->physics != BFAMX_ELASTIC3D)
+  if (b->physics != BFAMX_ELASTIC3D)
   {
     if (choice)
       *choice = 0;
@@ -1632,15 +1729,22 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   }
   if (b->exec)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_autotune: batch already captured");
-  signed char &cache = b->dev->mixed_choice[b->prec][b->order_mask & 255];
-  if (cache >= 0 && !force)
+  bfamx_device_t *d = b->dev;
+  signed char *kc = d->kchoice[b->prec & 3];
+  signed char &mc = d->mixed_choice[b->prec & 3][b->order_mask & 255];
+  int need = 0;
+  for (int n = 0; n < 9; ++n)
+    if (b->grp_cnt[n] && (kc[n] < 0 || force))
+      need = 1;
+  if (b->N < 0 && (mc < 0 || force))
+    need = 1;
+  if (!need)
   {
-    b->grouped = cache;
+    b->grouped = b->N < 0 ? mc > 0 : 0;
     if (choice)
-      *choice = cache;
+      *choice = b->grouped;
     return BFAMX_OK;
   }
-  bfamx_device_t *d = b->dev;
   /* save dq */
   std::vector<void *> saved(b->subs.size(), nullptr);
   std::vector<void *> allocs;
@@ -1655,7 +1759,6 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
                    hipSuccess)
       rc = bfamx_fail(BFAMX_EHIP, "autotune: dq save failed");
   }
-  float ms[2] = {0, 0};
   LaunchArgs a;
   memset(&a, 0, sizeof(a));
   a.subs = b->d_subs;
@@ -1664,24 +1767,48 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   a.mode = MODE_STAGE;
   a.xcd_remap = d->xcd_remap;
   a.nxcd = d->num_xcd;
-  const int grid = grid_for(d, b->ntasks);
-  const int keep = b->grouped;
-  for (int mode = 0; mode < 2 && !rc; ++mode)
-  {
-    b->grouped = mode;
-    hipError_t e = launch_batch_kernels(b, a, grid); /* warm-up */
+  /* mean time of `trials` launches of fn() after one warm-up */
+  auto time_it = [&](auto fn, float *ms) -> int {
+    hipError_t e = fn();
     if (e == hipSuccess)
       e = hipEventRecord(d->ev0, d->stream);
     for (int t = 0; t < trials && e == hipSuccess; ++t)
-      e = launch_batch_kernels(b, a, grid);
+      e = fn();
     if (e == hipSuccess)
       e = hipEventRecord(d->ev1, d->stream);
     if (e == hipSuccess)
       e = hipEventSynchronize(d->ev1);
     if (e == hipSuccess)
-      e = hipEventElapsedTime(&ms[mode], d->ev0, d->ev1);
-    if (e != hipSuccess)
-      rc = bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
+      e = hipEventElapsedTime(ms, d->ev0, d->ev1);
+    return e == hipSuccess ? BFAMX_OK
+                           : bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
+  };
+  /* (a) kernel per order */
+  for (int n = 0; n < 9 && !rc; ++n)
+  {
+    if (!b->grp_cnt[n] || (kc[n] >= 0 && !force))
+      continue;
+    float ms[2] = {0, 0};
+    for (int k = 0; k < 2 && !rc; ++k)
+    {
+      kc[n] = (signed char)k;
+      rc = time_it([&]() { return launch_order(b, a, n); }, &ms[k]);
+    }
+    kc[n] = rc ? -1 : (ms[1] < ms[0] ? 1 : 0);
+  }
+  /* (b) one order-switching launch or one launch per order */
+  const int keep = b->grouped;
+  if (!rc && b->N < 0 && (mc < 0 || force))
+  {
+    float ms[2] = {0, 0};
+    for (int mode = 0; mode < 2 && !rc; ++mode)
+    {
+      b->grouped = mode;
+      rc = time_it([&]() { return launch_batch_kernels(b, a, grid_for(d, b->ntasks)); },
+                   &ms[mode]);
+    }
+    if (!rc)
+      mc = (signed char)(ms[1] < ms[0] ? 1 : 0);
   }
   for (size_t i = 0; i < b->subs.size(); ++i)
     if (saved[i])
@@ -1700,10 +1827,25 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     b->grouped = keep;
     return rc;
   }
-  b->grouped = ms[1] < ms[0] ? 1 : 0;
-  cache = (signed char)b->grouped;
+  b->grouped = b->N < 0 ? mc > 0 : 0;
   if (choice)
     *choice = b->grouped;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_set_kernel(bfamx_device_t *d, int prec, int N, int kind)
+{
+  if (!d || prec < 0 || prec > 3 || N < 0 || N > 8 || kind < -1 || kind > 1)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_kernel: bad arguments");
+  d->kchoice[prec][N] = (signed char)kind;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_get_kernel(const bfamx_device_t *d, int prec, int N, int *kind)
+{
+  if (!d || prec < 0 || prec > 3 || N < 0 || N > 8 || !kind)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_get_kernel: bad arguments");
+  *kind = d->kchoice[prec][N] == 1 ? 1 : 0;
   return BFAMX_OK;
 }
 

@@ -104,6 +104,16 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
         const int nm = face_node<3>(f, n1, a, b);
         const int32_t ep = nbe[(size_t)et * 6 + f];
         const int code = nbc[(size_t)et * 6 + f];
+        TC *o = sfl + (lt * 6 + f) * 6 * Nfp + fn;
+        if (code & NBR_GLUED)
+        {
+          /* fluctuations already on this face node (mortar kernel) */
+          gptr<const TC> gb = glob(static_cast<const TC *>(sd.glue_buf)) + ep + fn;
+#pragma unroll
+          for (int c = 0; c < 6; ++c)
+            o[c * Nfp] = gb[c * Nfp];
+          continue;
+        }
         const int oi = perm_apply(nbr_perm(code), n1, a, b);
         const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
         gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
@@ -128,7 +138,6 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
         TC dv[3], dT[3];
         elastic_riemann<TC>(f >> 1, (f & 1) ? TC(1) : TC(-1), vm, sm, Mm[MAT_ZP],
                             Mm[MAT_ZS], vp, spl, Mp[MAT_ZP], Mp[MAT_ZS], dv, dT);
-        TC *o = sfl + (lt * 6 + f) * 6 * Nfp + fn;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
         {

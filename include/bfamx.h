@@ -286,15 +286,23 @@ int bfamx_batch_step(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_capture(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_launch_graph(bfamx_batch_t *b);
 /*
- * Mixed orders: choose between one launch per stage with the order-switching
- * kernel and one launch per order, by timing `trials` stages of each on the
- * batch's own data (the state and results are left unchanged).  The choice
- * is cached on the device per precision and set of orders; force = 1
- * re-times.  *choice: 0 mixed, 1 per order.  No-op for a single order.
+ * Autotuning on the batch's own data, for elastic batches.  For every order
+ * present it times `trials` stages of two stage kernels and keeps the
+ * faster: the line kernel (one wavefront per element, lanes own node lines)
+ * and the node-parallel kernel (512-thread blocks, threads own nodes).  With
+ * mixed orders it then chooses between one launch of the order-switching
+ * kernel and one launch per order.  The state and results are left
+ * unchanged.  Choices are cached on the device, per precision and order and
+ * per precision and set of orders, and also used by single subdomains;
+ * force = 1 re-times.  *choice: 0 mixed launch, 1 per order.
  */
 int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force, int *choice);
 /* set the mixed-order strategy by hand: 0 mixed launch, 1 one per order */
 int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order);
+/* the elastic stage kernel for (precision, order) by hand: 0 line,
+ * 1 node-parallel, -1 back to unknown (line until autotuned) */
+int bfamx_set_kernel(bfamx_device_t *dev, int precision, int N, int kind);
+int bfamx_get_kernel(const bfamx_device_t *dev, int precision, int N, int *kind);
 
 /* RHS L(q, t) of every member into host arrays rhs[i][field] (parity
  * checks); a glued batch evaluates its mortars first */

@@ -50,9 +50,20 @@ def teardown(subs, glue, b):
         s.close()
 
 
+@pytest.fixture(params=[0, 1], ids=["line", "node"])
+def kernel(request, dev):
+    for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
+        for N in range(3, 8):
+            dev.set_kernel(prec, N, request.param)
+    yield request.param
+    for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
+        for N in range(3, 8):
+            dev.set_kernel(prec, N, -1)
+
+
 @pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS, bx.FP32])
-def test_glued_rhs_octree(dev, prec):
-    print(f"glued rhs octree prec={prec}", flush=True)
+def test_glued_rhs_octree(dev, kernel, prec):
+    print(f"glued rhs octree prec={prec} kernel={kernel}", flush=True)
     cfg = octree_cfg()
     q0, subs, glue, b = setup(dev, cfg, prec, 1)
     t = 0.4 * cfg["src"]["t0"]
@@ -74,7 +85,7 @@ def test_glued_rhs_mixed_order_conforming(dev, prec):
 
 
 @pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS])
-def test_glued_lsrk_steps(dev, prec):
+def test_glued_lsrk_steps(dev, kernel, prec):
     print(f"glued lsrk 3 steps prec={prec}", flush=True)
     cfg = octree_cfg()
     q0, subs, glue, b = setup(dev, cfg, prec, 3)

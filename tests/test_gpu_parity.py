@@ -22,10 +22,22 @@ def dev():
     d.close()
 
 
+@pytest.fixture(params=[0, 1], ids=["line", "node"])
+def kernel(request, dev):
+    """run with the line (0) or the node-parallel (1) elastic stage kernel"""
+    for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
+        for N in range(3, 8):
+            dev.set_kernel(prec, N, request.param)
+    yield request.param
+    for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
+        for N in range(3, 8):
+            dev.set_kernel(prec, N, -1)
+
+
 @pytest.mark.parametrize("N", [3, 4, 5, 6, 7])
 @pytest.mark.parametrize("prec", PRECS)
-def test_elastic_rhs(dev, N, prec):
-    print(f"elastic rhs N={N} prec={prec}", flush=True)
+def test_elastic_rhs(dev, kernel, N, prec):
+    print(f"elastic rhs N={N} prec={prec} kernel={kernel}", flush=True)
     cfg, q0 = random_elastic((3, 2, 4), N, seed=N)
     sub = W.upload(dev, cfg, prec)
     sub.set_fields(q0, 0.0, cfg["dt"])
@@ -49,8 +61,8 @@ def test_acoustic_rhs(dev, N, prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_elastic_lsrk_steps(dev, prec):
-    print(f"elastic lsrk 4 steps prec={prec}", flush=True)
+def test_elastic_lsrk_steps(dev, kernel, prec):
+    print(f"elastic lsrk 4 steps prec={prec} kernel={kernel}", flush=True)
     cfg, q0 = random_elastic((4, 4, 3), 5, seed=3)
     sub = W.upload(dev, cfg, prec)
     sub.set_fields(q0, 0.0, cfg["dt"])
@@ -248,6 +260,10 @@ def test_mixed_order_launch_strategies(dev, prec):
         b = bx.Batch(dev, subs)
         if strategy == "auto":
             assert b.autotune(trials=2, force=True) in (0, 1)
+            # compare on the line kernels: the autotuner must not have
+            # touched the state
+            for N in range(3, 8):
+                dev.set_kernel(prec, N, -1)
         else:
             b.set_strategy(strategy)
         b.step(2)
@@ -326,7 +342,7 @@ def bc_elastic(N, seed):
 
 @pytest.mark.parametrize("N", [4, 7])
 @pytest.mark.parametrize("prec", PRECS)
-def test_elastic_boundaries_rhs(dev, N, prec):
+def test_elastic_boundaries_rhs(dev, kernel, N, prec):
     """Free-surface, rigid and absorbing faces against the oracle."""
     print(f"elastic boundaries N={N} prec={prec}", flush=True)
     cfg, q0 = bc_elastic(N, 80 + N)
