@@ -30,6 +30,31 @@ template <class T> __device__ __forceinline__ gptr<T> glob(T *p)
   return (gptr<T>)(p);
 }
 
+/* Read-only tables that no kernel which reads them writes (task lists,
+ * subdomain descriptors, clocks inside a stage launch, operators, element
+ * material and connectivity) are read through the constant address space:
+ * a wave-uniform load from it becomes a scalar load through the scalar data
+ * cache, instead of a vector load queued behind the kernel's HBM streams
+ * whose result must then be moved to a scalar register. */
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <class T> __device__ __forceinline__ cptr<T> cglob(const T *p)
+{
+  return (cptr<T>)(p);
+}
+
+/* pointer to an element table: constant (scalar loads when the element is
+ * uniform in the wave) or global address space */
+template <bool SCALAR, class T> struct ElemPtr
+{
+  using type = gptr<const T>;
+};
+template <class T> struct ElemPtr<true, T>
+{
+  using type = cptr<T>;
+};
+
 enum
 {
   GEO_GX = 0, /* 2/h_x, 2/h_y, 2/h_z : reference -> physical derivative */
@@ -218,7 +243,7 @@ __device__ __forceinline__ double uniform(double v)
 /* field-wise load of a task from the global task table */
 __device__ __forceinline__ Task load_task(const Task *tasks, int i)
 {
-  gptr<const Task> t = glob(tasks) + i;
+  cptr<Task> t = cglob(tasks) + i;
   Task r;
   r.sub = t->sub;
   r.e0 = t->e0;
@@ -322,15 +347,19 @@ struct StageCtx
   double g, sw, cw;
 };
 
-__device__ __forceinline__ StageCtx stage_ctx(const SubDev &sd,
-                                              const LaunchArgs &a)
+/* SD: SubDev, possibly qualified with the constant address space.  The
+ * clock is constant within a stage launch: only clock_advance_kernel, a
+ * separate launch, writes it. */
+template <class SD>
+__device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
 {
   StageCtx c;
   const int mode = a.mode;
   const double t_rhs = a.t_rhs;
+  cptr<Clock> clk = cglob((const Clock *)sd.clock);
   if (mode == MODE_RHS)
   {
-    c.parity = (int)(glob(sd.clock)->stage & 1);
+    c.parity = (int)(clk->stage & 1);
     c.a = 0;
     c.b = 0;
     c.dt = 0;
@@ -340,16 +369,16 @@ __device__ __forceinline__ StageCtx stage_ctx(const SubDev &sd,
     c.cw = a.cw_rhs;
     return c;
   }
-  c.g = sd.clock->g;
-  c.sw = sd.clock->sw;
-  c.cw = sd.clock->cw;
-  const long long st = sd.clock->stage;
+  c.g = clk->g;
+  c.sw = clk->sw;
+  c.cw = clk->cw;
+  const long long st = clk->stage;
   const int s = (int)(st % sd.nstages);
   c.parity = (int)(st & 1);
   c.a = sd.A[s];
   c.b = sd.B[s];
-  c.dt = sd.clock->dt;
-  c.t = sd.clock->t + sd.C[s] * c.dt;
+  c.dt = clk->dt;
+  c.t = clk->t + sd.C[s] * c.dt;
   return c;
 }
 

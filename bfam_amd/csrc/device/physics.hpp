@@ -18,18 +18,16 @@ using P64 = Prec<double, double, double>;
 using P32R = Prec<double, float, float>;
 using P32 = Prec<float, float, float>;
 
-/* per-precision copies of the element tables, as global pointers */
+/* per-precision copies of the element tables, as global pointers; SD is
+ * SubDev, possibly qualified with the constant address space */
 #define BFAMX_TAB(fn, dfield, ffield)                                          \
-  template <class T> __device__ __forceinline__ gptr<const T> fn(const SubDev &s); \
-  template <>                                                                  \
-  __device__ __forceinline__ gptr<const double> fn<double>(const SubDev &s)    \
+  template <class T, class SD>                                                 \
+  __device__ __forceinline__ gptr<const T> fn(const SD &s)                     \
   {                                                                            \
-    return glob(s.dfield);                                                     \
-  }                                                                            \
-  template <>                                                                  \
-  __device__ __forceinline__ gptr<const float> fn<float>(const SubDev &s)      \
-  {                                                                            \
-    return glob(s.ffield);                                                     \
+    if constexpr (sizeof(T) == sizeof(double))                                 \
+      return glob(s.dfield);                                                   \
+    else                                                                       \
+      return glob(s.ffield);                                                   \
   }
 BFAMX_TAB(geo_tab, geo, geo_f)
 BFAMX_TAB(mat_tab, mat, mat_f)

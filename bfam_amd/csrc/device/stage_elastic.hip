@@ -28,8 +28,8 @@
 namespace bfamx
 {
 
-template <int N, int EPB, class P, int PART = PART_FUSED>
-__device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
+template <int N, int EPB, class P, int PART = PART_FUSED, class SD>
+__device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
                                              const StageCtx &cx, int mode,
                                              char *smem)
 {
@@ -45,7 +45,10 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
   using SM = ElasticSmem<N, EPB, TC>;
 
   const int tid = threadIdx.x;
-  const int le = tid / Np, n = tid - (tid / Np) * Np;
+  /* one element per block (N >= 6): the element index is uniform, and its
+   * tables are read with scalar loads */
+  using EP = typename ElemPtr<EPB == 1, TC>::type;
+  const int le = EPB == 1 ? 0 : tid / Np, n = EPB == 1 ? tid : tid - (tid / Np) * Np;
   const bool act = tid < BS && le < tk.ne;
   const int e = tk.e0 + le;
   const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
@@ -94,7 +97,7 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
     for (int r = 0; r < TPT; ++r)
     {
       const int t = tid + r * BS;
-      const int lt = t / (6 * Nfp);
+      const int lt = EPB == 1 ? 0 : t / (6 * Nfp);
       if (tid < BS && t < NT && lt < tk.ne)
       {
         const int rem = t - lt * 6 * Nfp;
@@ -133,7 +136,7 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
           sm[c] = u[(3 + c) * Np];
           spl[c] = ss * (TC)qp[(3 + c) * Np];
         }
-        gptr<const TC> Mm = mtab + (size_t)et * MAT_STRIDE;
+        EP Mm = (EP)mtab + (size_t)et * MAT_STRIDE;
         gptr<const TC> Mp = mtab + (size_t)ep * MAT_STRIDE;
         TC dv[3], dT[3];
         elastic_riemann<TC>(f >> 1, (f & 1) ? TC(1) : TC(-1), vm, sm, Mm[MAT_ZP],
@@ -152,9 +155,9 @@ __device__ __forceinline__ void elastic_task(const SubDev &sd, const Task &tk,
     return;
 
   TC rate[NF];
-  gptr<const TC> M = mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
+  EP M = (EP)mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
   const TC rhoi = M[MAT_RHOI], lam = M[MAT_LAM], mu = M[MAT_MU];
-  gptr<const TC> G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+  EP G = (EP)geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
   if (PART == PART_SURFACE)
   {
     gptr<const double> rh = glob((const double *)sd.rhs) + (size_t)e * NF * Np + n;
@@ -327,7 +330,7 @@ elastic_stage_kernel(LaunchArgs a)
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     elastic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
     __syncthreads();
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(512) BFAMX_WAVES void elastic_stage_mixed(LaunchArg
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     switch (tk.N)
     {
@@ -410,7 +413,7 @@ elastic_part_kernel(LaunchArgs a)
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     elastic_task<N, EPB, P64, PART>(sd, tk, cx, a.mode, smem);
     __syncthreads();

@@ -51,6 +51,13 @@ template <int N, class TL, class TC> struct LineCfg
    * at N = 7 the dense layout fits 7 instead of 6 waves per CU and measured
    * 2.5 % faster (2.30 against 2.36 ms per 48^3 stage), so it is dense */
   static constexpr int S = NL;
+  /* At N = 7 the words of a field are swizzled: node (i, j, k) sits at
+   * 64 k + 8 (j ^ k) + (i ^ 4 (k & 1)).  Each k-plane keeps its 64 words and
+   * each half x-line (i = 0..3 or 4..7) its 4 consecutive words, so rows
+   * still move as 16-byte vectors, but the lanes of every pass -- (j, k) for
+   * x-lines, (i, k) for y-lines, (i, j) for z-lines -- now fall on distinct
+   * banks (the dense layout puts the y-line reads of a wave on 8 banks) */
+  static constexpr bool swz = (n1 == 8);
   static constexpr int FS = S * n1;    /* words per field of one element */
   static constexpr int FST = EPW * FS; /* field stride: the wave's elements */
   /* per wavefront: 9 staged state fields (type TL), field-major over the
@@ -102,7 +109,14 @@ __device__ __forceinline__ void rows_to_lds(gptr<const TS> q, int e0, int ne, in
     if (x < ne * Np)
     {
       const int lel = x / Np, n = x - lel * Np;
-      gptr<const TS> src = q + ((size_t)(e0 + lel) * 9 + f) * Np + n;
+      /* the node whose words land at LDS word n (inverse of the swizzle) */
+      int sn = n;
+      if constexpr (LineCfg<N, TS, TS>::swz)
+      {
+        const int k = n >> 6;
+        sn = 64 * k + 8 * (((n >> 3) & 7) ^ k) + ((n & 4) ^ ((k & 1) << 2));
+      }
+      gptr<const TS> src = q + ((size_t)(e0 + lel) * 9 + f) * Np + sn;
       if constexpr (VW * sizeof(TS) == 16)
         __builtin_amdgcn_global_load_lds(
             src, (__attribute__((address_space(3))) void *)(field_lds + 64 * VW * r), 16,
@@ -152,8 +166,8 @@ __device__ __forceinline__ void fold(int c, TC q, TC g, TC glam, TC gl2m,
  * issued them, and a valid nx gets its rows issued field by field as this
  * task's output frees each field's LDS region (persistent wavefronts).
  */
-template <int N, class P, int PART = PART_FUSED>
-__device__ __forceinline__ void elastic_line_task(const SubDev &sd,
+template <int N, class P, int PART = PART_FUSED, class SD>
+__device__ __forceinline__ void elastic_line_task(const SD &sd,
                                                   const Task &tk,
                                                   const StageCtx &cx, int mode,
                                                   char *lds, bool rows_ready,
@@ -190,23 +204,81 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
            le * FS; /* one-field transpose buffer */
 
   gptr<const TS> qin = glob(static_cast<const TS *>(sd.q[cx.parity]));
-  gptr<const TC> mtab = mat_tab<TC>(sd);
-  gptr<const TC> fmt = fmat_tab<TC>(sd) + (size_t)e * 6 * FMAT_STRIDE;
+  /* element tables and D: through the constant address space in FP32,
+   * where one element per wave (N = 7) makes them scalar loads.  FP64 reads
+   * them with vector loads: D alone would take 128 scalar registers, and the
+   * scalar spills measured 20 % slower (6.5 against 5.4 ms per 48^3 stage) */
+  using EP = typename ElemPtr<sizeof(TC) == 4, TC>::type;
+  EP mtab = (EP)mat_tab<TC>(sd);
+  EP fmt = (EP)fmat_tab<TC>(sd) + (size_t)e * 6 * FMAT_STRIDE;
   /* D is uniform across the wavefront: keep it in scalar registers */
   TC Dr[n1 * n1];
   {
-    gptr<const TC> Dg = dr_tab<TC>(sd);
+    EP Dg = (EP)dr_tab<TC>(sd);
 #pragma unroll
     for (int t = 0; t < n1 * n1; ++t)
       Dr[t] = uniform(Dg[t]);
   }
   const TC wi0 = (TC)sd.wi0;
-  gptr<const TC> G = geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
-  gptr<const int32_t> nbe = glob(sd.nbr_elem) + (size_t)e * 6;
+  EP G = (EP)geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
+  cptr<int32_t> nbe = cglob(sd.nbr_elem) + (size_t)e * 6;
+  /* face codes as vector byte loads: reading them as scalar words measured
+   * slower (2.21 against 2.14 ms per 48^3 N=7 FP32 stage) */
   gptr<const uint8_t> nbc = glob(sd.nbr_code) + (size_t)e * 6;
+  auto ncode = [&](int f) -> int { return (int)nbc[f]; };
 
   /* LDS word of node (i, j, k) */
-  auto at3 = [&](int i, int j, int k) -> int { return i + n1 * j + S * k; };
+  auto at3 = [&](int i, int j, int k) -> int {
+    if constexpr (CF::swz)
+      return S * k + n1 * (j ^ k) + (i ^ ((k & 1) << 2));
+    else
+      return i + n1 * j + S * k;
+  };
+  /* x-line (j, k) of an LDS field as whole vectors: with the swizzle its
+   * halves i = 0..3 and 4..7 are the aligned 4-word groups at at3(0, j, k)
+   * and at3(0, j, k) ^ 4 */
+  using V4 = TL __attribute__((ext_vector_type(4)));
+  auto xline_load = [&](const TL *base, int j, int k, TC *out) {
+    if constexpr (CF::swz)
+    {
+      const int x = at3(0, j, k);
+      const V4 lo = *reinterpret_cast<const V4 *>(base + x);
+      const V4 hi = *reinterpret_cast<const V4 *>(base + (x ^ 4));
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+      {
+        out[t] = lo[t];
+        out[4 + t] = hi[t];
+      }
+    }
+    else
+    {
+#pragma unroll
+      for (int i = 0; i < n1; ++i)
+        out[i] = base[at3(i, j, k)];
+    }
+  };
+  auto xline_store = [&](TL *base, int j, int k, const TC *in) {
+    if constexpr (CF::swz)
+    {
+      const int x = at3(0, j, k);
+      V4 lo, hi;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+      {
+        lo[t] = in[t];
+        hi[t] = in[4 + t];
+      }
+      *reinterpret_cast<V4 *>(base + x) = lo;
+      *reinterpret_cast<V4 *>(base + (x ^ 4)) = hi;
+    }
+    else
+    {
+#pragma unroll
+      for (int i = 0; i < n1; ++i)
+        base[at3(i, j, k)] = in[i];
+    }
+  };
 
   /* ---- neighbour traces at face node (pa, pb) of both faces of axis A:
    * v1 v2 v3 and the stress column of A ---- */
@@ -216,7 +288,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
     {
       const int f = 2 * A + side;
       const int32_t ep = nbe[f];
-      const int code = nbc[f];
+      const int code = ncode(f);
       if (code & NBR_GLUED)
       {
         /* flux fluctuation already on this face node (mortar kernel) */
@@ -322,7 +394,36 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         }
       }
   }
-  TC nbx[2][6], nby[2][6];
+  const bool rhs_out = (mode == MODE_RHS || PART != PART_FUSED);
+  const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
+  const TS cb = (TS)cx.b;
+  constexpr int PD = sizeof(TS) == 4 ? NF : 3;
+  TR dqv[PD][ROWS][VW];
+  TS qv[q_from_lds ? 1 : PD][ROWS][VW];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+    {
+      int lel, n;
+      if (!coal_pos(r, &lel, &n))
+        continue;
+      const size_t off = ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
+      gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + off;
+#pragma unroll
+      for (int t = 0; t < VW; ++t)
+        dqv[f % PD][r][t] = dqe[t];
+      if constexpr (!q_from_lds)
+      {
+#pragma unroll
+        for (int t = 0; t < VW; ++t)
+          qv[f % PD][r][t] = qin[off + t];
+      }
+    }
+  };
+  /* the x- and y-face traces are requested with the state; each face
+   * code's byte load makes them wait for the state rows to land, which
+   * measured faster than issuing them alongside */
+  TC nbx[2][6], nby[2][6], nbz[2][6];
   if (act && PART != PART_VOLUME)
   {
     gather(0, nbx);
@@ -332,7 +433,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   TC rhoi = 0, lam = 0, mu = 0, Zpm = 0, Zsm = 0;
   if (act)
   {
-    gptr<const TC> M = mtab + (size_t)e * MAT_STRIDE;
+    EP M = mtab + (size_t)e * MAT_STRIDE;
     rhoi = M[MAT_RHOI];
     lam = M[MAT_LAM];
     mu = M[MAT_MU];
@@ -389,8 +490,8 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       for (int side = 0; side < 2; ++side)
       {
         const int m = side ? N : 0;
-        gptr<const TC> F = fmt + (2 * A + side) * FMAT_STRIDE;
-        if (nbc[2 * A + side] & NBR_GLUED)
+        EP F = fmt + (2 * A + side) * FMAT_STRIDE;
+        if (ncode(2 * A + side) & NBR_GLUED)
         {
 #pragma unroll
           for (int c = 0; c < 6; ++c)
@@ -419,9 +520,12 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
       if (act)
       {
         TC w[n1];
+        if constexpr (A == 0)
+          xline_load(sq + f * FST, pa, pb, w);
+        else
 #pragma unroll
-        for (int m = 0; m < n1; ++m)
-          w[m] = (TC)sq[f * FST + at(m)];
+          for (int m = 0; m < n1; ++m)
+            w[m] = (TC)sq[f * FST + at(m)];
 #pragma unroll
         for (int j = 0; j < n1; ++j)
         {
@@ -467,11 +571,14 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        TC qx[n1];
+        if (act)
+          xline_load(tb, pa, pb, qx);
         if (act)
 #pragma unroll
           for (int i = 0; i < n1; ++i)
           {
-            const TC qv = tb[at3(i, pa, pb)];
+            const TC qv = qx[i];
             TC r9[NF];
 #pragma unroll
             for (int ff = 0; ff < NF; ++ff)
@@ -510,7 +617,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  TC nbz[2][6];
+  /* the z-face traces load under the y pass */
   if (act && PART != PART_VOLUME)
     gather(2, nbz);
   pass(std::integral_constant<int, 1>{}, nby);
@@ -560,32 +667,8 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
    * rows (same lane -> node assignment as the load).  The dq (and, when the
    * LDS copy is rounded, q_old) rows are fetched PD fields ahead so that the
    * per-field LDS round trips do not wait on global memory ---- */
-  const bool rhs_out = (mode == MODE_RHS || PART != PART_FUSED);
-  const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
-  const TS cb = (TS)cx.b;
-  constexpr int PD = sizeof(TS) == 4 ? NF : 3;
-  TR dqv[PD][ROWS][VW];
-  TS qv[q_from_lds ? 1 : PD][ROWS][VW];
-  auto fetch = [&](int f) {
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r)
-    {
-      int lel, n;
-      if (!coal_pos(r, &lel, &n))
-        continue;
-      const size_t off = ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
-      gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + off;
-#pragma unroll
-      for (int t = 0; t < VW; ++t)
-        dqv[f % PD][r][t] = dqe[t];
-      if constexpr (!q_from_lds)
-      {
-#pragma unroll
-        for (int t = 0; t < VW; ++t)
-          qv[f % PD][r][t] = qin[off + t];
-      }
-    }
-  };
+  /* fetching the dq rows before the passes instead (FP32: all nine fit in
+   * registers) measured slower, 2.28 against 2.14 ms per 48^3 N=7 stage */
   if (!rhs_out)
   {
 #pragma unroll
@@ -597,9 +680,7 @@ __device__ __forceinline__ void elastic_line_task(const SubDev &sd,
   for (int f = 0; f < NF; ++f)
   {
     if (act)
-#pragma unroll
-      for (int i = 0; i < n1; ++i)
-        tb[at3(i, pa, pb)] = rate[f][i];
+      xline_store(tb, pa, pb, rate[f]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -697,7 +778,9 @@ constexpr int LINE_WPB = 1;
 __device__ __forceinline__ int first_wave_task(const LaunchArgs &a, int *stride,
                                                int *end)
 {
-  const int w = threadIdx.x >> 6;
+  /* the wave index is uniform in the wave; saying so keeps the task walk,
+   * and every table read that follows from it, in scalar registers */
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.x, G = gridDim.x;
   if (!a.xcd_remap || G % a.nxcd)
   {
@@ -729,7 +812,7 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_kernel(LaunchArgs 
   for (int ti = first_wave_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     NextRows<TS> nx;
     nx.valid = ti + stride < end;
@@ -738,7 +821,7 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_kernel(LaunchArgs 
     if (nx.valid)
     {
       const Task nt = load_task(a.tasks, ti + stride);
-      const SubDev &ns = *(const SubDev *)(glob(a.subs) + nt.sub);
+      const auto &ns = *(cglob(a.subs) + nt.sub);
       nx.q = glob(static_cast<const TS *>(ns.q[stage_ctx(ns, a).parity]));
       nx.e0 = nt.e0;
       nx.ne = nt.ne;
@@ -769,7 +852,7 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_mixed(LaunchArgs a
   for (int ti = first_wave_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     NextRows<TS> nx; /* no cross-task prefetch: the next task's order differs */
     nx.valid = false;
