@@ -19,8 +19,7 @@ INCS = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 CFLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in generated code
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-DBFAMX_WAVES="]
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def _sources():

@@ -45,13 +45,14 @@ enum
   PART_SURFACE = 2  /* rhs += lifted face fluctuations (unfused path) */
 };
 
-template <int N, int EPB, class T> struct ElasticSmem
+template <int N, int EPB, class T, bool DQ = false> struct ElasticSmem
 {
   static constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
   static constexpr int QW = EPB * 9 * Np;   /* staged fields */
+  static constexpr int DW = DQ ? QW : 0;    /* staged dq rows */
   static constexpr int FW = EPB * 36 * Nfp; /* face fluctuations */
-  /* LDS: fields, face fluctuations, D, sin/cos tables */
-  static constexpr size_t bytes = sizeof(T) * (QW + FW + n1 * n1 + EPB * 6 * n1);
+  /* LDS: fields, dq, face fluctuations, D, sin/cos tables */
+  static constexpr size_t bytes = sizeof(T) * (QW + DW + FW + n1 * n1 + EPB * 6 * n1);
 };
 
 /*

@@ -28,6 +28,45 @@
 namespace bfamx
 {
 
+/* the dq rows of a task are staged in LDS by LDS-DMA (no registers held
+ * while they load) for FP32 rates, where the larger LDS footprint still
+ * lets three 512-thread blocks share a CU */
+template <int N, int EPB, class P> struct NodeDq
+{
+  static constexpr bool value =
+      sizeof(typename P::TR) == 4 &&
+      3 * ElasticSmem<N, EPB, typename P::TC, true>::bytes <= 160 * 1024;
+};
+template <int N, int EPB, class P>
+using NodeSmem = ElasticSmem<N, EPB, typename P::TC, NodeDq<N, EPB, P>::value>;
+
+/*
+ * Block-wide copy of `count` contiguous 32-bit words from global memory to
+ * LDS with LDS-DMA (global_load_lds), W bytes per lane: the LDS image is the
+ * global one.  Each wave's destination is uniform (lane l lands at
+ * base + W*l); completion is tracked by vmcnt.  W = 16 needs count and both
+ * addresses 16-byte aligned.
+ */
+template <int W, int BLOCK>
+__device__ __forceinline__ void block_to_lds(gptr<const float> src, float *dst,
+                                             int count)
+{
+  constexpr int per = W / 4;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  for (int base = 0; base < count; base += BLOCK * per)
+  {
+    const int x = base + per * tid;
+    auto *d = (__attribute__((address_space(3))) void *)(dst + base + per * 64 * wave);
+    if (x < count)
+    {
+      if constexpr (W == 16)
+        __builtin_amdgcn_global_load_lds(src + x, d, 16, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds(src + x, d, 4, 0, 0);
+    }
+  }
+}
+
 template <int N, int EPB, class P, int PART = PART_FUSED, class SD>
 __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
                                              const StageCtx &cx, int mode,
@@ -42,7 +81,11 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   constexpr int BS = EPB * Np;
   constexpr int NT = EPB * 6 * Nfp;
   constexpr int TPT = (NT + BS - 1) / BS;
-  using SM = ElasticSmem<N, EPB, TC>;
+  using SM = NodeSmem<N, EPB, P>;
+  /* FP32 state: LDS-DMA straight into the staged fields */
+  constexpr bool DMA_Q = sizeof(TS) == 4 && sizeof(TC) == 4;
+  constexpr bool DMA_DQ = SM::DW > 0;
+  constexpr int DMA_W = Np % 4 == 0 ? 16 : 4;
 
   const int tid = threadIdx.x;
   /* one element per block (N >= 6): the element index is uniform, and its
@@ -54,7 +97,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
 
   TC *sq = reinterpret_cast<TC *>(smem); /* [EPB][9][Np] */
-  TC *sfl = sq + SM::QW;                 /* [EPB][6 faces][6 comps][Nfp] */
+  TC *sdq = sq + SM::QW;                 /* [EPB][9][Np] when DMA_DQ */
+  TC *sfl = sdq + SM::DW;                /* [EPB][6 faces][6 comps][Nfp] */
   TC *sD = sfl + SM::FW;
   TC *smms = sD + n1 * n1;
 
@@ -73,7 +117,12 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   /* ---- 1. stream the element's state into LDS ---- */
   gptr<const TS> qin = glob(static_cast<const TS *>(sd.q[cx.parity]));
   TS qs[KEEP_Q ? NF : 1];
-  if (act)
+  if constexpr (DMA_Q)
+  {
+    block_to_lds<DMA_W, BS>((gptr<const float>)(qin + (size_t)tk.e0 * NF * Np),
+                            reinterpret_cast<float *>(sq), tk.ne * NF * Np);
+  }
+  else if (act)
   {
     gptr<const TS> qe = qin + (size_t)e * NF * Np + n;
 #pragma unroll
@@ -85,6 +134,19 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       sq[(le * NF + f) * Np + n] = (TC)v;
     }
   }
+  /* the dq rows load into LDS under the face and volume work */
+  const bool upd = !(mode == MODE_RHS || PART != PART_FUSED);
+  if constexpr (DMA_DQ)
+  {
+    if (upd)
+      block_to_lds<DMA_W, BS>(
+          (gptr<const float>)glob(static_cast<const float *>(sd.dq)) + (size_t)tk.e0 * NF * Np,
+          reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
+  }
+  /* LDS-DMA completes by vmcnt: every wave drains its own before the
+   * barrier that publishes the staged state */
+  if constexpr (DMA_Q || DMA_DQ)
+    __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
   /* ---- 2. faces: one task per face node, fluctuations straight to LDS ---- */
@@ -295,7 +357,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   }
 
   /* ---- 5. output ---- */
-  if (mode == MODE_RHS || PART != PART_FUSED)
+  if (!upd)
   {
     gptr<double> rh = glob(sd.rhs) + (size_t)e * NF * Np + n;
 #pragma unroll
@@ -315,17 +377,33 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       qv = qs[f];
     else
       qv = (TS)sq[(le * NF + f) * Np + n];
-    const TR d = ca * dqe[f * Np] + cdt * (TR)rate[f];
+    TR dqo;
+    if constexpr (DMA_DQ)
+      dqo = (TR)sdq[(le * NF + f) * Np + n];
+    else
+      dqo = dqe[f * Np];
+    const TR d = ca * dqo + cdt * (TR)rate[f];
     dqe[f * Np] = d;
     qo[f * Np] = qv + cb * (TS)d;
   }
 }
 
+/* waves per SIMD the register allocation must leave room for, for the
+ * one-element blocks of N = 6, 7 (two waves per SIMD each): 6 keeps three
+ * blocks per CU for the FP32 stage; FP32_RHS and FP64 keep two (forcing
+ * FP32_RHS to three spilled registers and measured slower, 3.01 against
+ * 2.88 ms per 48^3 stage).  Smaller orders are left to the compiler. */
+template <int N, int EPB, class P> struct NodeWaves
+{
+  static constexpr int value = EPB != 1 ? 1 : (sizeof(typename P::TS) == 4 ? 6 : 4);
+};
+
 template <int N, int EPB, class P>
-__global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1)) BFAMX_WAVES void
+__global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1))
+__attribute__((amdgpu_waves_per_eu(NodeWaves<N, EPB, P>::value))) void
 elastic_stage_kernel(LaunchArgs a)
 {
-  __shared__ __align__(16) char smem[ElasticSmem<N, EPB, typename P::TC>::bytes];
+  __shared__ __align__(16) char smem[NodeSmem<N, EPB, P>::bytes];
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
@@ -341,13 +419,13 @@ template <class T> constexpr size_t cmax(T a, T b) { return a > b ? a : b; }
 
 /* every order 3..7 in one launch; 512 threads, elements per block by N */
 template <class P>
-__global__ __launch_bounds__(512) BFAMX_WAVES void elastic_stage_mixed(LaunchArgs a)
+__global__ __launch_bounds__(512) void elastic_stage_mixed(LaunchArgs a)
 {
   using TC = typename P::TC;
   constexpr size_t B = cmax(
-      cmax(cmax(ElasticSmem<3, 8, TC>::bytes, ElasticSmem<4, 4, TC>::bytes),
-           cmax(ElasticSmem<5, 2, TC>::bytes, ElasticSmem<6, 1, TC>::bytes)),
-      ElasticSmem<7, 1, TC>::bytes);
+      cmax(cmax(NodeSmem<3, 8, P>::bytes, NodeSmem<4, 4, P>::bytes),
+           cmax(NodeSmem<5, 2, P>::bytes, NodeSmem<6, 1, P>::bytes)),
+      NodeSmem<7, 1, P>::bytes);
   __shared__ __align__(16) char smem[B];
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
