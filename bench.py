@@ -51,7 +51,8 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_DOF = {"fp32": 16, "fp32_rhs": 24, "fp64": 32, "fp64_plain": 80}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_elastic_line_fp32_48cube_N7.txt")
+TRAFFIC_FILES = {"line": "traffic_elastic_line_fp32_48cube_N7.txt",
+                 "node-parallel": "traffic_elastic_stage_fp32_48cube_N7.txt"}
 
 
 def log(msg):
@@ -69,14 +70,14 @@ def copy_calibration():
     log("copy calibration: 3 x 1 GiB device copies done")
 
 
-def read_traffic():
-    """HBM bytes per launch of the FP32 stage kernel from the committed
-    counter summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the
-    kernel, and of a 1 GiB copy kernel, which reads and writes exactly
-    2^30 bytes."""
+def read_traffic(kernel):
+    """HBM bytes per launch of the FP32 stage kernel (`kernel`: "line" or
+    "node-parallel") from the committed counter summary: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) of the kernel, and of a 1 GiB copy kernel,
+    which reads and writes exactly 2^30 bytes."""
     try:
         kv = {}
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", TRAFFIC_FILES[kernel])) as f:
             for line in f:
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
@@ -159,11 +160,14 @@ def main():
     dev = bx.Device(0)
     K, Wm = args.steps, args.warmup
 
-    def timed(sub, batched):
+    def timed(sub, batched, fused):
         sub.set_fields(q0, 0.0, cfg["dt"])
-        if batched:
+        if fused:
             b = bx.Batch(dev, [sub])
             b.autotune(trials=3)  # stage kernel for this precision and order
+            if not batched:
+                b.close()  # the single-subdomain path keeps the tuned kernel
+        if batched:
             b.capture(1)
             for _ in range(Wm):
                 b.launch_graph()
@@ -188,11 +192,11 @@ def main():
     res = {}
     for name, prec, batched in forms:
         sub = W.upload(dev, cfg, prec)
-        ms = timed(sub, batched and not args.eager)
+        ms = timed(sub, batched and not args.eager, batched)
         sub.close()
         res[name] = {"ms_per_step": ms / K, "ms_per_stage": ms / (5 * K),
                      "dof_stage_per_s": ndof * 5 * K / (ms * 1e-3)}
-        if batched and not args.eager:
+        if batched:
             res[name]["kernel"] = "node-parallel" if dev.get_kernel(prec, args.order) else "line"
         log(f"{name}: {ms / K:.3f} ms/step, {res[name]['dof_stage_per_s']:.4e} DOF-stage/s")
 
@@ -259,7 +263,8 @@ def main():
     head_name = "fp32" if "fp32" in res else next(iter(res))
     launch_s = head["ms_per_stage"] * 1e-3
     achieved = BYTES_PER_DOF[head_name] * ndof / launch_s / 1e9
-    traffic = read_traffic() if head_name == "fp32" else None
+    head_kernel = head.get("kernel", "line")
+    traffic = read_traffic(head_kernel) if head_name == "fp32" else None
 
     cpu = None
     if not args.no_cpu and not args.only:
@@ -287,9 +292,9 @@ def main():
         "accuracy": accuracy,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"elastic_line_kernel<{args.order}, fp32>"
-                                if res.get("fp32", {}).get("kernel", "line") == "line"
-                                else f"elastic_stage_kernel<{args.order}, fp32>"),
+                     "kernel": (f"elastic_line_kernel<{args.order}, {head_name}>"
+                                if head_kernel == "line"
+                                else f"elastic_stage_kernel<{args.order}, {head_name}>"),
                      "bytes_per_dof": BYTES_PER_DOF[head_name]},
         "cpu_baseline": cpu,
         "adapted_mesh": adapted,

@@ -62,6 +62,22 @@ __device__ __forceinline__ void riemann_axis(TC s, const TC *vm, const TC *Sm,
   }
 }
 
+/* riemann_axis for an axis known only at run time */
+template <class TC>
+__device__ __forceinline__ void riemann_dispatch(int A, TC s, const TC *vm,
+                                                 const TC *Sm, const TC *vp,
+                                                 const TC *Sp, TC Zpm, TC Zsm,
+                                                 TC Zpp, TC Zsp, TC izp, TC izs,
+                                                 TC *dv, TC *dT)
+{
+  if (A == 0)
+    riemann_axis<0, TC>(s, vm, Sm, vp, Sp, Zpm, Zsm, Zpp, Zsp, izp, izs, dv, dT);
+  else if (A == 1)
+    riemann_axis<1, TC>(s, vm, Sm, vp, Sp, Zpm, Zsm, Zpp, Zsp, izp, izs, dv, dT);
+  else
+    riemann_axis<2, TC>(s, vm, Sm, vp, Sp, Zpm, Zsm, Zpp, Zsp, izp, izs, dv, dT);
+}
+
 } // namespace bfamx
 
 #endif

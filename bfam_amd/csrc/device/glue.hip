@@ -24,21 +24,6 @@
 namespace bfamx
 {
 
-template <class TC>
-__device__ __forceinline__ void riemann_dispatch(int A, TC s, const TC *vm,
-                                                 const TC *Sm, const TC *vp,
-                                                 const TC *Sp, TC Zpm, TC Zsm,
-                                                 TC Zpp, TC Zsp, TC izp, TC izs,
-                                                 TC *dv, TC *dT)
-{
-  if (A == 0)
-    riemann_axis<0, TC>(s, vm, Sm, vp, Sp, Zpm, Zsm, Zpp, Zsp, izp, izs, dv, dT);
-  else if (A == 1)
-    riemann_axis<1, TC>(s, vm, Sm, vp, Sp, Zpm, Zsm, Zpp, Zsp, izp, izs, dv, dT);
-  else
-    riemann_axis<2, TC>(s, vm, Sm, vp, Sp, Zpm, Zsm, Zpp, Zsp, izp, izs, dv, dT);
-}
-
 /* face trace (v, sigma[.][A]) of one element face into LDS, lane = node */
 template <class TS, class TC>
 __device__ __forceinline__ void load_trace(const SubDev &sd, int32_t e, int f,

@@ -55,40 +55,6 @@ template <int N, int EPB, class T, bool DQ = false> struct ElasticSmem
   static constexpr size_t bytes = sizeof(T) * (QW + DW + FW + n1 * n1 + EPB * 6 * n1);
 };
 
-/*
- * Upwind fluctuations on a face with normal s*e_axis: from the minus and plus
- * velocities and stresses (stress order s11 s22 s33 s12 s13 s23) returns
- * dv = v* - v-  and  dT = T* - T-  with T = sigma n.
- */
-template <class T>
-__device__ __forceinline__ void elastic_riemann(int axis, T s, const T *vm,
-                                                const T *sm, T Zpm, T Zsm,
-                                                const T *vp, const T *sp,
-                                                T Zpp, T Zsp, T *dv, T *dT)
-{
-  /* column `axis` of the symmetric stress: sigma[c][axis] */
-  const int col0 = axis == 0 ? 0 : (axis == 1 ? 3 : 4);
-  const int col1 = axis == 0 ? 3 : (axis == 1 ? 1 : 5);
-  const int col2 = axis == 0 ? 4 : (axis == 1 ? 5 : 2);
-  const T Tm[3] = {s * sm[col0], s * sm[col1], s * sm[col2]};
-  const T Tp[3] = {s * sp[col0], s * sp[col1], s * sp[col2]};
-  const T vnm = s * vm[axis], vnp = s * vp[axis];
-  const T Tnm = s * Tm[axis], Tnp = s * Tp[axis];
-  const T dvn = (Zpp * (vnp - vnm) + (Tnp - Tnm)) / (Zpm + Zpp);
-  const T zs = Zsm + Zsp;
-  const T izs = zs > T(0) ? T(1) / zs : T(0);
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-  {
-    const T nc = (c == axis) ? s : T(0);
-    const T vtm = vm[c] - vnm * nc, vtp = vp[c] - vnp * nc;
-    const T Ttm = Tm[c] - Tnm * nc, Ttp = Tp[c] - Tnp * nc;
-    const T dvt = (Zsp * (vtp - vtm) + (Ttp - Ttm)) * izs;
-    dv[c] = dvn * nc + dvt;
-    dT[c] = Zpm * dvn * nc + Zsm * dvt;
-  }
-}
-
 /* rate += lift * (dT/rho, lambda dvn I + mu (dv n^T + n dv^T)) */
 template <class T>
 __device__ __forceinline__ void elastic_lift(int axis, T s, T lift, T rhoi,

@@ -23,6 +23,7 @@
  *   5. dq read, dq and q_out written (q ping-pongs so neighbours read the
  *      stage input); nothing but the loop state is held across a barrier
  */
+#include "elastic_axis.hpp"
 #include "physics.hpp"
 
 namespace bfamx
@@ -185,24 +186,25 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
         const TC *u = sq + lt * NF * Np + nm;
         TC sv, ss;
         nbr_mirror<TC>(code, &sv, &ss);
-        TC vm[3], sm[6], vp[3], spl[6];
+        /* only the velocity and the stress column of the face's axis enter
+         * the flux: six of the nine neighbour values are gathered */
+        const int A = f >> 1;
+        TC vm[3], Sm[3], vp[3], Sp[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
         {
+          const int col = sfield(c, A);
           vm[c] = u[c * Np];
+          Sm[c] = u[col * Np];
           vp[c] = sv * (TC)qp[c * Np];
-        }
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-        {
-          sm[c] = u[(3 + c) * Np];
-          spl[c] = ss * (TC)qp[(3 + c) * Np];
+          Sp[c] = ss * (TC)qp[col * Np];
         }
         EP Mm = (EP)mtab + (size_t)et * MAT_STRIDE;
-        gptr<const TC> Mp = mtab + (size_t)ep * MAT_STRIDE;
+        gptr<const TC> F = fmat_tab<TC>(sd) + ((size_t)et * 6 + f) * FMAT_STRIDE;
         TC dv[3], dT[3];
-        elastic_riemann<TC>(f >> 1, (f & 1) ? TC(1) : TC(-1), vm, sm, Mm[MAT_ZP],
-                            Mm[MAT_ZS], vp, spl, Mp[MAT_ZP], Mp[MAT_ZS], dv, dT);
+        riemann_dispatch<TC>(A, (f & 1) ? TC(1) : TC(-1), vm, Sm, vp, Sp, Mm[MAT_ZP],
+                             Mm[MAT_ZS], F[FMAT_ZP], F[FMAT_ZS], F[FMAT_IZP],
+                             F[FMAT_IZS], dv, dT);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
         {
