@@ -25,14 +25,14 @@ namespace bfamx
 {
 
 /* face trace (v, sigma[.][A]) of one element face into LDS, lane = node */
-template <class TS, class TC>
-__device__ __forceinline__ void load_trace(const SubDev &sd, int32_t e, int f,
+template <class TS, class TC, class SD>
+__device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
                                            TC (*tr)[GLUE_MAXFP], int lane)
 {
   const int n1 = sd.N + 1, nfp = n1 * n1, np = nfp * n1, A = f >> 1;
   if (lane >= nfp)
     return;
-  const int parity = (int)(glob((const Clock *)sd.clock)->stage & 1);
+  const int parity = (int)(cglob((const Clock *)sd.clock)->stage & 1);
   gptr<const TS> q = glob(static_cast<const TS *>(sd.q[parity])) + (size_t)e * 9 * np +
                      face_node<3>(f, n1, lane % n1, lane / n1);
 #pragma unroll
@@ -52,17 +52,19 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
   if (gi >= a.ngroups)
     return;
   const int lane = threadIdx.x;
-  gptr<const MortarGroup> gp = glob(a.groups) + gi;
+  /* group, descriptors, clocks and materials are read-only here: constant
+   * address space, scalar loads */
+  cptr<MortarGroup> gp = cglob(a.groups) + gi;
   const int32_t b_sub = gp->b_sub, b_elem = gp->b_elem, b_off = gp->b_off;
   const int b_face = gp->b_face, nm = gp->nm, quad0 = gp->quad0;
-  const SubDev &B = *(const SubDev *)(glob(a.subs) + b_sub);
+  const auto &B = *(cglob(a.subs) + b_sub);
   const int Nb = B.N, nb1 = Nb + 1, nfb = nb1 * nb1;
   const int A = b_face >> 1;
   const TC sgs = (b_face & 1) ? TC(-1) : TC(1); /* normal sign of the s side */
   gptr<const TC> ops = glob(static_cast<const TC *>(a.ops));
   gptr<TC> buf = glob(static_cast<TC *>(a.buf));
   load_trace<TS, TC>(B, b_elem, b_face, tb, lane);
-  gptr<const TC> Mb = mat_tab<TC>(B) + (size_t)b_elem * MAT_STRIDE;
+  cptr<TC> Mb = (cptr<TC>)mat_tab<TC>(B) + (size_t)b_elem * MAT_STRIDE;
   const TC Zpb = Mb[MAT_ZP], Zsb = Mb[MAT_ZS];
   TC acc[6] = {0, 0, 0, 0, 0, 0};
   for (int m = 0; m < nm; ++m)
@@ -70,11 +72,11 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
     const int32_t s_sub = gp->s_sub[m], s_elem = gp->s_elem[m], s_off = gp->s_off[m];
     const int s_face = gp->s_face[m];
     const int quad = nm == 1 ? quad0 : m;
-    const SubDev &S = *(const SubDev *)(glob(a.subs) + s_sub);
+    const auto &S = *(cglob(a.subs) + s_sub);
     const int Ns = S.N, ns1 = Ns + 1, nfs = ns1 * ns1;
     const int Ng = Ns > Nb ? Ns : Nb, ng1 = Ng + 1;
     load_trace<TS, TC>(S, s_elem, s_face, ts, lane);
-    gptr<const TC> Ms = mat_tab<TC>(S) + (size_t)s_elem * MAT_STRIDE;
+    cptr<TC> Ms = (cptr<TC>)mat_tab<TC>(S) + (size_t)s_elem * MAT_STRIDE;
     const TC Zps = Ms[MAT_ZP], Zss = Ms[MAT_ZS];
     const TC izp = TC(1) / (Zps + Zpb), izs = TC(1) / (Zss + Zsb);
     __syncthreads();

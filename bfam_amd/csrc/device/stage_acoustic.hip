@@ -22,8 +22,8 @@ template <int N, int EPB, class T> struct AcousticSmem
   static constexpr size_t bytes = sizeof(T) * (W + n1 * n1);
 };
 
-template <int N, int EPB, class P, int PART = PART_FUSED>
-__device__ __forceinline__ void acoustic_task(const SubDev &sd, const Task &tk,
+template <int N, int EPB, class P, int PART = PART_FUSED, class SD>
+__device__ __forceinline__ void acoustic_task(const SD &sd, const Task &tk,
                                               const StageCtx &cx, int mode,
                                               char *smem)
 {
@@ -211,7 +211,7 @@ acoustic_stage_kernel(LaunchArgs a)
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     acoustic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
     __syncthreads();
@@ -265,7 +265,7 @@ acoustic_part_kernel(LaunchArgs a)
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
-    const SubDev &sd = *(const SubDev *)(glob(a.subs) + tk.sub);
+    const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
     acoustic_task<N, EPB, P64, PART>(sd, tk, cx, a.mode, smem);
     __syncthreads();

@@ -83,10 +83,11 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   constexpr int NT = EPB * 6 * Nfp;
   constexpr int TPT = (NT + BS - 1) / BS;
   using SM = NodeSmem<N, EPB, P>;
-  /* FP32 state: LDS-DMA straight into the staged fields */
-  constexpr bool DMA_Q = sizeof(TS) == 4 && sizeof(TC) == 4;
+  /* state staged exactly (FP32, FP64): LDS-DMA straight into the staged
+   * fields, 16 bytes per lane where every element block is 16-byte aligned */
+  constexpr bool DMA_Q = sizeof(TS) == sizeof(TC);
   constexpr bool DMA_DQ = SM::DW > 0;
-  constexpr int DMA_W = Np % 4 == 0 ? 16 : 4;
+  constexpr int DMA_W = (9 * Np * (int)sizeof(TS) / 4) % 4 == 0 ? 16 : 4;
 
   const int tid = threadIdx.x;
   /* one element per block (N >= 6): the element index is uniform, and its
@@ -121,7 +122,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   if constexpr (DMA_Q)
   {
     block_to_lds<DMA_W, BS>((gptr<const float>)(qin + (size_t)tk.e0 * NF * Np),
-                            reinterpret_cast<float *>(sq), tk.ne * NF * Np);
+                            reinterpret_cast<float *>(sq),
+                            tk.ne * NF * Np * (int)(sizeof(TS) / 4));
   }
   else if (act)
   {
@@ -143,6 +145,19 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       block_to_lds<DMA_W, BS>(
           (gptr<const float>)glob(static_cast<const float *>(sd.dq)) + (size_t)tk.e0 * NF * Np,
           reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
+  }
+  /* FP64 rates have no room in LDS: this node's dq values load into
+   * registers instead, also under the face and volume work */
+  TR dqs[DMA_DQ ? 1 : NF];
+  if constexpr (!DMA_DQ)
+  {
+    if (act && upd)
+    {
+      gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + (size_t)e * NF * Np + n;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        dqs[f] = dqe[f * Np];
+    }
   }
   /* LDS-DMA completes by vmcnt: every wave drains its own before the
    * barrier that publishes the staged state */
@@ -383,7 +398,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     if constexpr (DMA_DQ)
       dqo = (TR)sdq[(le * NF + f) * Np + n];
     else
-      dqo = dqe[f * Np];
+      dqo = dqs[f];
     const TR d = ca * dqo + cdt * (TR)rate[f];
     dqe[f * Np] = d;
     qo[f * Np] = qv + cb * (TS)d;
