@@ -18,7 +18,9 @@ The JSON line also carries:
                      nine fields);
   roofline           the FP32 stage kernel against HBM: algorithmic bytes
                      (q read + write, dq read + write, 4 B each = 16 B per
-                     DOF) over the mean stage duration, and the HBM traffic
+                     DOF; 12 B on the first stage of a step, which does not
+                     read dq: 15.2 B on average) over the mean stage
+                     duration, and the HBM traffic
                      per launch measured with FETCH_SIZE / WRITE_SIZE,
                      calibrated on a plain copy kernel (read from the text
                      summary under profiles/);
@@ -50,7 +52,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_DOF = {"fp32": 16, "fp32_rhs": 24, "fp64": 32, "fp64_plain": 80}
+# algorithmic bytes per DOF-stage, averaged over the five stages of LSRK4(5):
+# q read + written and dq read + written, except that the first stage of a
+# step (A = 0) does not read dq: FP32 (4 x 16 + 12) / 5, FP32_RHS
+# (4 x 24 + 20) / 5, FP64 (4 x 32 + 24) / 5
+BYTES_PER_DOF = {"fp32": 15.2, "fp32_rhs": 23.2, "fp64": 30.4, "fp64_plain": 80}
+# the line kernel reads dq on every stage
+BYTES_PER_DOF_LINE = {"fp32": 16, "fp32_rhs": 24, "fp64": 32}
 TRAFFIC_FILES = {"line": "traffic_elastic_line_fp32_48cube_N7.txt",
                  "node-parallel": "traffic_elastic_stage_fp32_48cube_N7.txt"}
 
@@ -262,8 +270,9 @@ def main():
     head = res.get("fp32") or next(iter(res.values()))
     head_name = "fp32" if "fp32" in res else next(iter(res))
     launch_s = head["ms_per_stage"] * 1e-3
-    achieved = BYTES_PER_DOF[head_name] * ndof / launch_s / 1e9
-    head_kernel = head.get("kernel", "line")
+    head_kernel = head.get("kernel")  # None for the unfused FP64 form
+    bpd = (BYTES_PER_DOF_LINE if head_kernel == "line" else BYTES_PER_DOF)[head_name]
+    achieved = bpd * ndof / launch_s / 1e9
     traffic = read_traffic(head_kernel) if head_name == "fp32" else None
 
     cpu = None
@@ -294,8 +303,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": (f"elastic_line_kernel<{args.order}, {head_name}>"
                                 if head_kernel == "line"
-                                else f"elastic_stage_kernel<{args.order}, {head_name}>"),
-                     "bytes_per_dof": BYTES_PER_DOF[head_name]},
+                                else f"elastic_stage_kernel<{args.order}, {head_name}>"
+                                if head_kernel else "elastic_part_kernel + plain_update_kernel"),
+                     "bytes_per_dof": bpd},
         "cpu_baseline": cpu,
         "adapted_mesh": adapted,
         "serving": serving,

@@ -139,9 +139,12 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   }
   /* the dq rows load into LDS under the face and volume work */
   const bool upd = !(mode == MODE_RHS || PART != PART_FUSED);
+  /* a 2N-storage scheme starts every step with A = 0 (dq = dt L): the old
+   * dq is not read then */
+  const bool rd_dq = upd && cx.a != 0.0;
   if constexpr (DMA_DQ)
   {
-    if (upd)
+    if (rd_dq)
       block_to_lds<DMA_W, BS>(
           (gptr<const float>)glob(static_cast<const float *>(sd.dq)) + (size_t)tk.e0 * NF * Np,
           reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
@@ -151,7 +154,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   TR dqs[DMA_DQ ? 1 : NF];
   if constexpr (!DMA_DQ)
   {
-    if (act && upd)
+    if (act && rd_dq)
     {
       gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + (size_t)e * NF * Np + n;
 #pragma unroll
@@ -394,11 +397,14 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       qv = qs[f];
     else
       qv = (TS)sq[(le * NF + f) * Np + n];
-    TR dqo;
-    if constexpr (DMA_DQ)
-      dqo = (TR)sdq[(le * NF + f) * Np + n];
-    else
-      dqo = dqs[f];
+    TR dqo = 0;
+    if (rd_dq)
+    {
+      if constexpr (DMA_DQ)
+        dqo = (TR)sdq[(le * NF + f) * Np + n];
+      else
+        dqo = dqs[f];
+    }
     const TR d = ca * dqo + cdt * (TR)rate[f];
     dqe[f * Np] = d;
     qo[f * Np] = qv + cb * (TS)d;
