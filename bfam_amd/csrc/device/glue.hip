@@ -43,11 +43,37 @@ __device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
   }
 }
 
+/* the six 1-D operators of one mortar, staged in LDS: interpolation of the
+ * s face (Is) and of the b face along each axis (Ia, Ib) to the mortar grid,
+ * and the mass projections back (Ws; Wa, Wb) */
+enum
+{
+  OP_IS = 0,
+  OP_IA = 1,
+  OP_IB = 2,
+  OP_WS = 3,
+  OP_WA = 4,
+  OP_WB = 5,
+  OP_BLOCK = 81
+};
+
+/*
+ * One wavefront per interface group.  Every global read of the group (the b
+ * trace, the traces of its 1 or 4 mortars' s faces and their operators) is
+ * issued up front into LDS, so the group waits on memory once.  The tensor
+ * products run sum-factorised, one axis per step with the partial result in
+ * LDS: lane (g, j) first contracts along i, then lane (g, h) along j.  Each
+ * step sums in the same order as the nested form, so the results are those
+ * of the direct double sum.
+ */
 template <class TS, class TC>
 __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
 {
-  __shared__ TC tb[6][GLUE_MAXFP], ts[6][GLUE_MAXFP], fs[6][GLUE_MAXFP],
-      fb[6][GLUE_MAXFP];
+  __shared__ TC tb[6][GLUE_MAXFP];    /* b trace */
+  __shared__ TC ts[4][6][GLUE_MAXFP]; /* s traces of the mortars */
+  __shared__ TC sop[4][6][OP_BLOCK];  /* operators of the mortars */
+  __shared__ TC t1[6][GLUE_MAXFP], t2[6][GLUE_MAXFP]; /* partial contractions */
+  __shared__ TC fs[6][GLUE_MAXFP], fb[6][GLUE_MAXFP]; /* mortar fluctuations */
   const int gi = blockIdx.x;
   if (gi >= a.ngroups)
     return;
@@ -63,52 +89,81 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
   const TC sgs = (b_face & 1) ? TC(-1) : TC(1); /* normal sign of the s side */
   gptr<const TC> ops = glob(static_cast<const TC *>(a.ops));
   gptr<TC> buf = glob(static_cast<TC *>(a.buf));
+
+  /* ---- every load of the group ---- */
   load_trace<TS, TC>(B, b_elem, b_face, tb, lane);
+  for (int m = 0; m < nm; ++m)
+  {
+    const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
+    load_trace<TS, TC>(S, gp->s_elem[m], gp->s_face[m], ts[m], lane);
+    const int Ns = S.N, Ng = Ns > Nb ? Ns : Nb;
+    const int quad = nm == 1 ? quad0 : m;
+    const int kI = quad < 0 ? 0 : 3 + (quad & 1), kJ = quad < 0 ? 0 : 3 + ((quad >> 1) & 1);
+    const int kA = quad < 0 ? 0 : 1 + (quad & 1), kB = quad < 0 ? 0 : 1 + ((quad >> 1) & 1);
+    const int base[6] = {glue_ops_index(0, Ns, Ng, 0), glue_ops_index(0, Nb, Ng, kI),
+                         glue_ops_index(0, Nb, Ng, kJ), glue_ops_index(1, Ng, Ns, 0),
+                         glue_ops_index(1, Ng, Nb, kA), glue_ops_index(1, Ng, Nb, kB)};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      for (int t = lane; t < OP_BLOCK; t += 64)
+        sop[m][k][t] = ops[base[k] + t];
+  }
   cptr<TC> Mb = (cptr<TC>)mat_tab<TC>(B) + (size_t)b_elem * MAT_STRIDE;
   const TC Zpb = Mb[MAT_ZP], Zsb = Mb[MAT_ZS];
+  __syncthreads();
+
   TC acc[6] = {0, 0, 0, 0, 0, 0};
   for (int m = 0; m < nm; ++m)
   {
-    const int32_t s_sub = gp->s_sub[m], s_elem = gp->s_elem[m], s_off = gp->s_off[m];
-    const int s_face = gp->s_face[m];
-    const int quad = nm == 1 ? quad0 : m;
-    const auto &S = *(cglob(a.subs) + s_sub);
+    const int s_off = gp->s_off[m];
+    const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
     const int Ns = S.N, ns1 = Ns + 1, nfs = ns1 * ns1;
     const int Ng = Ns > Nb ? Ns : Nb, ng1 = Ng + 1;
-    load_trace<TS, TC>(S, s_elem, s_face, ts, lane);
-    cptr<TC> Ms = (cptr<TC>)mat_tab<TC>(S) + (size_t)s_elem * MAT_STRIDE;
+    cptr<TC> Ms = (cptr<TC>)mat_tab<TC>(S) + (size_t)gp->s_elem[m] * MAT_STRIDE;
     const TC Zps = Ms[MAT_ZP], Zss = Ms[MAT_ZS];
     const TC izp = TC(1) / (Zps + Zpb), izs = TC(1) / (Zss + Zsb);
+    const TC *Is = sop[m][OP_IS], *Ia = sop[m][OP_IA], *Ib = sop[m][OP_IB];
+    const TC *Ws = sop[m][OP_WS], *Wa = sop[m][OP_WA], *Wb = sop[m][OP_WB];
+    const TC(*tsm)[GLUE_MAXFP] = ts[m];
+
+    /* interpolation to the mortar grid, first axis: lane (g, j) */
+    {
+      const int g = lane % ng1, j = lane / ng1;
+      if (j < ns1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+        {
+          TC row = 0;
+          for (int i = 0; i < ns1; ++i)
+            row += Is[g + ng1 * i] * tsm[c][i + ns1 * j];
+          t1[c][lane] = row;
+        }
+      if (j < nb1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+        {
+          TC row = 0;
+          for (int i = 0; i < nb1; ++i)
+            row += Ia[g + ng1 * i] * tb[c][i + nb1 * j];
+          t2[c][lane] = row;
+        }
+    }
     __syncthreads();
+    /* second axis and the Riemann solves: lane (g, h) */
     if (lane < ng1 * ng1)
     {
       const int g = lane % ng1, h = lane / ng1;
-      const int kI = quad < 0 ? 0 : 3 + (quad & 1);
-      const int kJ = quad < 0 ? 0 : 3 + ((quad >> 1) & 1);
-      gptr<const TC> Is = ops + glue_ops_index(0, Ns, Ng, 0);
-      gptr<const TC> Ia = ops + glue_ops_index(0, Nb, Ng, kI);
-      gptr<const TC> Ib = ops + glue_ops_index(0, Nb, Ng, kJ);
       TC us[6], ub[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c)
       {
         TC x = 0;
         for (int j = 0; j < ns1; ++j)
-        {
-          TC row = 0;
-          for (int i = 0; i < ns1; ++i)
-            row += Is[g + ng1 * i] * ts[c][i + ns1 * j];
-          x += Is[h + ng1 * j] * row;
-        }
+          x += Is[h + ng1 * j] * t1[c][g + ng1 * j];
         us[c] = x;
         TC y = 0;
         for (int j = 0; j < nb1; ++j)
-        {
-          TC row = 0;
-          for (int i = 0; i < nb1; ++i)
-            row += Ia[g + ng1 * i] * tb[c][i + nb1 * j];
-          y += Ib[h + ng1 * j] * row;
-        }
+          y += Ib[h + ng1 * j] * t2[c][g + ng1 * j];
         ub[c] = y;
       }
       TC dv[3], dT[3];
@@ -130,42 +185,52 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
       }
     }
     __syncthreads();
+    /* mass projection back to each face, first axis: lane (i, h) */
+    {
+      const int h = lane / ns1, i = lane % ns1;
+      if (h < ng1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+        {
+          TC row = 0;
+          for (int g = 0; g < ng1; ++g)
+            row += Ws[i + ns1 * g] * fs[c][g + ng1 * h];
+          t1[c][lane] = row;
+        }
+      const int hb = lane / nb1, ib = lane % nb1;
+      if (hb < ng1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+        {
+          TC row = 0;
+          for (int g = 0; g < ng1; ++g)
+            row += Wa[ib + nb1 * g] * fb[c][g + ng1 * hb];
+          t2[c][lane] = row;
+        }
+    }
+    __syncthreads();
+    /* second axis: lane (i, j) of each face */
     if (lane < nfs)
     {
       const int i = lane % ns1, j = lane / ns1;
-      gptr<const TC> Ws = ops + glue_ops_index(1, Ng, Ns, 0);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
       {
         TC x = 0;
         for (int h = 0; h < ng1; ++h)
-        {
-          TC row = 0;
-          for (int g = 0; g < ng1; ++g)
-            row += Ws[i + ns1 * g] * fs[c][g + ng1 * h];
-          x += Ws[j + ns1 * h] * row;
-        }
+          x += Ws[j + ns1 * h] * t1[c][i + ns1 * h];
         buf[s_off + c * nfs + lane] = x;
       }
     }
     if (lane < nfb)
     {
       const int i = lane % nb1, j = lane / nb1;
-      const int kA = quad < 0 ? 0 : 1 + (quad & 1);
-      const int kB = quad < 0 ? 0 : 1 + ((quad >> 1) & 1);
-      gptr<const TC> Wa = ops + glue_ops_index(1, Ng, Nb, kA);
-      gptr<const TC> Wb = ops + glue_ops_index(1, Ng, Nb, kB);
 #pragma unroll
       for (int c = 0; c < 6; ++c)
       {
         TC x = 0;
         for (int h = 0; h < ng1; ++h)
-        {
-          TC row = 0;
-          for (int g = 0; g < ng1; ++g)
-            row += Wa[i + nb1 * g] * fb[c][g + ng1 * h];
-          x += Wb[j + nb1 * h] * row;
-        }
+          x += Wb[j + nb1 * h] * t2[c][i + nb1 * h];
         acc[c] += x;
       }
     }
