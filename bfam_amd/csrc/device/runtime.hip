@@ -51,6 +51,10 @@ struct bfamx_device
    * element, lanes own lines), 1 node-parallel (512-thread blocks, threads
    * own nodes) */
   signed char kchoice[4][9];
+  /* the per-order launches of a grouped batch run on side streams forked
+   * from and joined back to `stream` (parallel branches when captured) */
+  hipStream_t side[8];
+  hipEvent_t fork_ev, join_ev[8];
 };
 
 struct bfamx_subdomain
@@ -171,6 +175,12 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&d->ev0));
   HIPCHK(hipEventCreate(&d->ev1));
+  HIPCHK(hipEventCreateWithFlags(&d->fork_ev, hipEventDisableTiming));
+  for (int i = 0; i < 8; ++i)
+  {
+    HIPCHK(hipStreamCreateWithFlags(&d->side[i], hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&d->join_ev[i], hipEventDisableTiming));
+  }
   *out = d;
   return BFAMX_OK;
 }
@@ -183,6 +193,13 @@ extern "C" int bfamx_device_destroy(bfamx_device_t *d)
   (void)hipStreamSynchronize(d->stream);
   (void)hipEventDestroy(d->ev0);
   (void)hipEventDestroy(d->ev1);
+  (void)hipEventDestroy(d->fork_ev);
+  for (int i = 0; i < 8; ++i)
+  {
+    (void)hipStreamSynchronize(d->side[i]);
+    (void)hipStreamDestroy(d->side[i]);
+    (void)hipEventDestroy(d->join_ev[i]);
+  }
   (void)hipStreamDestroy(d->stream);
   delete d;
   return BFAMX_OK;
@@ -1274,19 +1291,23 @@ static GlueArgs glue_args(const bfamx_batch_t *b)
 /* the stage kernels of the orders [n] of a batch with their chosen kernel:
  * the line kernel over the line tasks, or the node-parallel kernel over the
  * node tasks */
-static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n)
+static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n,
+                               hipStream_t st)
 {
   LaunchArgs g = a;
   if (node_kernel(b->dev, b->prec, n))
   {
     g.tasks = b->d_tasks_node + b->grpn_off[n];
     g.ntasks = b->grpn_cnt[n];
-    return launch_elastic_stage(b->prec, n, g, grid_for(b->dev, g.ntasks), b->dev->stream);
+    return launch_elastic_stage(b->prec, n, g, grid_for(b->dev, g.ntasks), st);
   }
   g.tasks = a.tasks + b->grp_off[n];
   g.ntasks = b->grp_cnt[n];
-  return launch_elastic_line(b->prec, n, g, grid_line(b->dev, b->prec, n, g.ntasks),
-                             b->dev->stream);
+  return launch_elastic_line(b->prec, n, g, grid_line(b->dev, b->prec, n, g.ntasks), st);
+}
+static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n)
+{
+  return launch_order(b, a, n, b->dev->stream);
 }
 
 /* the stage kernels of a batch: one launch (one order, or all orders with
@@ -1302,15 +1323,39 @@ static hipError_t launch_batch_kernels(bfamx_batch_t *b, const LaunchArgs &a,
   if (!b->grouped)
     return launch_elastic_line(b->prec, b->N, a, grid_line(b->dev, b->prec, b->N, a.ntasks),
                                b->dev->stream);
+  /* one launch per order: the largest on the batch's stream, the others on
+   * side streams forked after and joined before the next launch, so small
+   * orders do not wait for the tail of a large one */
+  int ord[9], no = 0, big = -1;
   for (int n = 0; n < 9; ++n)
+    if (b->grp_cnt[n])
+    {
+      ord[no++] = n;
+      if (big < 0 || (size_t)b->grp_cnt[n] * (n + 1) * (n + 1) * (n + 1) >
+                         (size_t)b->grp_cnt[big] * (big + 1) * (big + 1) * (big + 1))
+        big = n;
+    }
+  bfamx_device_t *d = b->dev;
+  hipError_t e = hipSuccess;
+  if (no > 1)
+    e = hipEventRecord(d->fork_ev, d->stream);
+  int k = 0;
+  for (int i = 0; i < no && e == hipSuccess; ++i)
   {
-    if (!b->grp_cnt[n])
+    if (ord[i] == big)
       continue;
-    hipError_t e = launch_order(b, a, n);
-    if (e != hipSuccess)
-      return e;
+    e = hipStreamWaitEvent(d->side[k], d->fork_ev, 0);
+    if (e == hipSuccess)
+      e = launch_order(b, a, ord[i], d->side[k]);
+    if (e == hipSuccess)
+      e = hipEventRecord(d->join_ev[k], d->side[k]);
+    ++k;
   }
-  return hipSuccess;
+  if (e == hipSuccess && big >= 0)
+    e = launch_order(b, a, big, d->stream);
+  for (int i = 0; i < k && e == hipSuccess; ++i)
+    e = hipStreamWaitEvent(d->stream, d->join_ev[i], 0);
+  return e;
 }
 
 static int batch_stage(bfamx_batch_t *b)
