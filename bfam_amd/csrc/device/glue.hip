@@ -24,17 +24,31 @@
 namespace bfamx
 {
 
-/* face trace (v, sigma[.][A]) of one element face into LDS, lane = node */
+/* Face data and 1-D operators of the mortar kernel live in LDS padded to
+ * order 7: node (a, b) of a face at a + 8 b, operator entry (row, col) at
+ * row + 8 col, zero outside the actual size.  Every contraction then runs a
+ * fixed eight terms, fully unrolled; the padded terms add exact zeros after
+ * the actual ones, so the sums are those of the actual size. */
+constexpr int GP = 8;
+
+/* face trace (v, sigma[.][A]) of one element face into LDS, lane = padded
+ * node */
 template <class TS, class TC, class SD>
 __device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
                                            TC (*tr)[GLUE_MAXFP], int lane)
 {
-  const int n1 = sd.N + 1, nfp = n1 * n1, np = nfp * n1, A = f >> 1;
-  if (lane >= nfp)
+  const int n1 = sd.N + 1, np = n1 * n1 * n1, A = f >> 1;
+  const int a = lane % GP, b = lane / GP;
+  if (a >= n1 || b >= n1)
+  {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      tr[c][lane] = TC(0);
     return;
+  }
   const int parity = (int)(cglob((const Clock *)sd.clock)->stage & 1);
   gptr<const TS> q = glob(static_cast<const TS *>(sd.q[parity])) + (size_t)e * 9 * np +
-                     face_node<3>(f, n1, lane % n1, lane / n1);
+                     face_node<3>(f, n1, a, b);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
   {
@@ -43,9 +57,9 @@ __device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
   }
 }
 
-/* the six 1-D operators of one mortar, staged in LDS: interpolation of the
- * s face (Is) and of the b face along each axis (Ia, Ib) to the mortar grid,
- * and the mass projections back (Ws; Wa, Wb) */
+/* the six 1-D operators of one mortar: interpolation of the s face (Is) and
+ * of the b face along each axis (Ia, Ib) to the mortar grid, and the mass
+ * projections back (Ws; Wa, Wb) */
 enum
 {
   OP_IS = 0,
@@ -53,9 +67,19 @@ enum
   OP_IB = 2,
   OP_WS = 3,
   OP_WA = 4,
-  OP_WB = 5,
-  OP_BLOCK = 81
+  OP_WB = 5
 };
+
+/* sum_{t < 8} M[r + 8 t] x[t * sx + o], fully unrolled */
+template <class TC>
+__device__ __forceinline__ TC contract(const TC *M, int r, const TC *x, int o, int sx)
+{
+  TC acc = 0;
+#pragma unroll
+  for (int t = 0; t < GP; ++t)
+    acc += M[r + GP * t] * x[o + sx * t];
+  return acc;
+}
 
 /*
  * One wavefront per interface group.  Every global read of the group (the b
@@ -71,13 +95,14 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
 {
   __shared__ TC tb[6][GLUE_MAXFP];    /* b trace */
   __shared__ TC ts[4][6][GLUE_MAXFP]; /* s traces of the mortars */
-  __shared__ TC sop[4][6][OP_BLOCK];  /* operators of the mortars */
+  __shared__ TC sop[4][6][GP * GP];   /* operators of the mortars */
   __shared__ TC t1[6][GLUE_MAXFP], t2[6][GLUE_MAXFP]; /* partial contractions */
   __shared__ TC fs[6][GLUE_MAXFP], fb[6][GLUE_MAXFP]; /* mortar fluctuations */
   const int gi = blockIdx.x;
   if (gi >= a.ngroups)
     return;
   const int lane = threadIdx.x;
+  const int lo = lane % GP, hi = lane / GP; /* padded (row, column) */
   /* group, descriptors, clocks and materials are read-only here: constant
    * address space, scalar loads */
   cptr<MortarGroup> gp = cglob(a.groups) + gi;
@@ -100,13 +125,16 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
     const int quad = nm == 1 ? quad0 : m;
     const int kI = quad < 0 ? 0 : 3 + (quad & 1), kJ = quad < 0 ? 0 : 3 + ((quad >> 1) & 1);
     const int kA = quad < 0 ? 0 : 1 + (quad & 1), kB = quad < 0 ? 0 : 1 + ((quad >> 1) & 1);
+    /* block offset, rows and columns of each operator (column-major) */
     const int base[6] = {glue_ops_index(0, Ns, Ng, 0), glue_ops_index(0, Nb, Ng, kI),
                          glue_ops_index(0, Nb, Ng, kJ), glue_ops_index(1, Ng, Ns, 0),
                          glue_ops_index(1, Ng, Nb, kA), glue_ops_index(1, Ng, Nb, kB)};
+    const int rows[6] = {Ng + 1, Ng + 1, Ng + 1, Ns + 1, Nb + 1, Nb + 1};
+    const int cols[6] = {Ns + 1, Nb + 1, Nb + 1, Ng + 1, Ng + 1, Ng + 1};
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      for (int t = lane; t < OP_BLOCK; t += 64)
-        sop[m][k][t] = ops[base[k] + t];
+      sop[m][k][lane] = (lo < rows[k] && hi < cols[k]) ? ops[base[k] + lo + rows[k] * hi]
+                                                        : TC(0);
   }
   cptr<TC> Mb = (cptr<TC>)mat_tab<TC>(B) + (size_t)b_elem * MAT_STRIDE;
   const TC Zpb = Mb[MAT_ZP], Zsb = Mb[MAT_ZS];
@@ -118,53 +146,28 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
     const int s_off = gp->s_off[m];
     const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
     const int Ns = S.N, ns1 = Ns + 1, nfs = ns1 * ns1;
-    const int Ng = Ns > Nb ? Ns : Nb, ng1 = Ng + 1;
     cptr<TC> Ms = (cptr<TC>)mat_tab<TC>(S) + (size_t)gp->s_elem[m] * MAT_STRIDE;
     const TC Zps = Ms[MAT_ZP], Zss = Ms[MAT_ZS];
     const TC izp = TC(1) / (Zps + Zpb), izs = TC(1) / (Zss + Zsb);
     const TC *Is = sop[m][OP_IS], *Ia = sop[m][OP_IA], *Ib = sop[m][OP_IB];
     const TC *Ws = sop[m][OP_WS], *Wa = sop[m][OP_WA], *Wb = sop[m][OP_WB];
-    const TC(*tsm)[GLUE_MAXFP] = ts[m];
 
-    /* interpolation to the mortar grid, first axis: lane (g, j) */
+    /* interpolation to the mortar grid, first axis: lane (g, j) = (lo, hi) */
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
     {
-      const int g = lane % ng1, j = lane / ng1;
-      if (j < ns1)
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-        {
-          TC row = 0;
-          for (int i = 0; i < ns1; ++i)
-            row += Is[g + ng1 * i] * tsm[c][i + ns1 * j];
-          t1[c][lane] = row;
-        }
-      if (j < nb1)
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-        {
-          TC row = 0;
-          for (int i = 0; i < nb1; ++i)
-            row += Ia[g + ng1 * i] * tb[c][i + nb1 * j];
-          t2[c][lane] = row;
-        }
+      t1[c][lane] = contract(Is, lo, ts[m][c], GP * hi, 1);
+      t2[c][lane] = contract(Ia, lo, tb[c], GP * hi, 1);
     }
     __syncthreads();
-    /* second axis and the Riemann solves: lane (g, h) */
-    if (lane < ng1 * ng1)
+    /* second axis and the Riemann solves: lane (g, h) = (lo, hi) */
     {
-      const int g = lane % ng1, h = lane / ng1;
       TC us[6], ub[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c)
       {
-        TC x = 0;
-        for (int j = 0; j < ns1; ++j)
-          x += Is[h + ng1 * j] * t1[c][g + ng1 * j];
-        us[c] = x;
-        TC y = 0;
-        for (int j = 0; j < nb1; ++j)
-          y += Ib[h + ng1 * j] * t2[c][g + ng1 * j];
-        ub[c] = y;
+        us[c] = contract(Is, hi, t1[c], lo, GP);
+        ub[c] = contract(Ib, hi, t2[c], lo, GP);
       }
       TC dv[3], dT[3];
       riemann_dispatch<TC>(A, sgs, us, us + 3, ub, ub + 3, Zps, Zss, Zpb, Zsb, izp, izs,
@@ -185,62 +188,30 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
       }
     }
     __syncthreads();
-    /* mass projection back to each face, first axis: lane (i, h) */
+    /* mass projection back to each face, first axis: lane (i, h) = (lo, hi) */
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
     {
-      const int h = lane / ns1, i = lane % ns1;
-      if (h < ng1)
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-        {
-          TC row = 0;
-          for (int g = 0; g < ng1; ++g)
-            row += Ws[i + ns1 * g] * fs[c][g + ng1 * h];
-          t1[c][lane] = row;
-        }
-      const int hb = lane / nb1, ib = lane % nb1;
-      if (hb < ng1)
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-        {
-          TC row = 0;
-          for (int g = 0; g < ng1; ++g)
-            row += Wa[ib + nb1 * g] * fb[c][g + ng1 * hb];
-          t2[c][lane] = row;
-        }
+      t1[c][lane] = contract(Ws, lo, fs[c], GP * hi, 1);
+      t2[c][lane] = contract(Wa, lo, fb[c], GP * hi, 1);
     }
     __syncthreads();
-    /* second axis: lane (i, j) of each face */
-    if (lane < nfs)
-    {
-      const int i = lane % ns1, j = lane / ns1;
+    /* second axis: lane (i, j) = (lo, hi) of each face */
 #pragma unroll
-      for (int c = 0; c < 6; ++c)
-      {
-        TC x = 0;
-        for (int h = 0; h < ng1; ++h)
-          x += Ws[j + ns1 * h] * t1[c][i + ns1 * h];
-        buf[s_off + c * nfs + lane] = x;
-      }
-    }
-    if (lane < nfb)
+    for (int c = 0; c < 6; ++c)
     {
-      const int i = lane % nb1, j = lane / nb1;
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-      {
-        TC x = 0;
-        for (int h = 0; h < ng1; ++h)
-          x += Wb[j + nb1 * h] * t2[c][i + nb1 * h];
-        acc[c] += x;
-      }
+      const TC x = contract(Ws, hi, t1[c], lo, GP);
+      if (lo < ns1 && hi < ns1)
+        buf[s_off + c * nfs + lo + ns1 * hi] = x;
+      acc[c] += contract(Wb, hi, t2[c], lo, GP);
     }
     __syncthreads();
   }
-  if (lane < nfb)
+  if (lo < nb1 && hi < nb1)
   {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-      buf[b_off + c * nfb + lane] = acc[c];
+      buf[b_off + c * nfb + lo + nb1 * hi] = acc[c];
   }
 }
 
