@@ -438,36 +438,6 @@ elastic_stage_kernel(LaunchArgs a)
   }
 }
 
-template <class T> constexpr size_t cmax(T a, T b) { return a > b ? a : b; }
-
-/* every order 3..7 in one launch; 512 threads, elements per block by N */
-template <class P>
-__global__ __launch_bounds__(512) void elastic_stage_mixed(LaunchArgs a)
-{
-  using TC = typename P::TC;
-  constexpr size_t B = cmax(
-      cmax(cmax(NodeSmem<3, 8, P>::bytes, NodeSmem<4, 4, P>::bytes),
-           cmax(NodeSmem<5, 2, P>::bytes, NodeSmem<6, 1, P>::bytes)),
-      NodeSmem<7, 1, P>::bytes);
-  __shared__ __align__(16) char smem[B];
-  int stride, end;
-  for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
-  {
-    const Task tk = load_task(a.tasks, ti);
-    const auto &sd = *(cglob(a.subs) + tk.sub);
-    const StageCtx cx = stage_ctx(sd, a);
-    switch (tk.N)
-    {
-    case 3: elastic_task<3, 8, P>(sd, tk, cx, a.mode, smem); break;
-    case 4: elastic_task<4, 4, P>(sd, tk, cx, a.mode, smem); break;
-    case 5: elastic_task<5, 2, P>(sd, tk, cx, a.mode, smem); break;
-    case 6: elastic_task<6, 1, P>(sd, tk, cx, a.mode, smem); break;
-    default: elastic_task<7, 1, P>(sd, tk, cx, a.mode, smem); break;
-    }
-    __syncthreads();
-  }
-}
-
 template <class P>
 static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s)
 {
@@ -484,10 +454,7 @@ static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s)
     BFAMX_CASE(6, 1)
     BFAMX_CASE(7, 1)
 #undef BFAMX_CASE
-  default:
-    hipLaunchKernelGGL((elastic_stage_mixed<P>), dim3(grid), dim3(512), 0, s,
-                       a);
-    break;
+  default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
