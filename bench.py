@@ -138,8 +138,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=0.6, help="arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
-    ap.add_argument("--only", choices=["fp32", "fp64", "fp64_plain", "fp32_rhs"],
-                    help="time one form only (for profiling runs)")
+    ap.add_argument("--only", choices=["fp32", "fp64", "fp64_plain", "fp32_rhs", "adapted"],
+                    help="time one form, or the adapted mesh, only (for profiling runs)")
     ap.add_argument("--eager", action="store_true",
                     help="launch stage by stage instead of replaying a captured graph")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -197,6 +197,7 @@ def main():
              ("fp64_plain", bx.FP64_PLAIN, False), ("fp32_rhs", bx.FP32_RHS, True)]
     if args.only:
         forms = [f for f in forms if f[0] == args.only]
+    only_adapted = args.only == "adapted"
     res = {}
     for name, prec, batched in forms:
         sub = W.upload(dev, cfg, prec)
@@ -223,7 +224,7 @@ def main():
         log(f"accuracy: {accuracy}")
 
     adapted = None
-    if not args.only and not args.no_adapted:
+    if (not args.only and not args.no_adapted) or only_adapted:
         oc = W.elastic3d_octree()
         subs_o = W.upload_multi(dev, oc, bx.FP32)
         for s_, sm in zip(subs_o, oc["subs"]):
@@ -266,6 +267,9 @@ def main():
                    "batch_rebuilds": st["nbatches"]}
         log(f"serving: {serving}")
     dev.close()
+    if only_adapted:
+        print(json.dumps({"adapted_mesh": adapted}), flush=True)
+        return
 
     head = res.get("fp32") or next(iter(res.values()))
     head_name = "fp32" if "fp32" in res else next(iter(res))
