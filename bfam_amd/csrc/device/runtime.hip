@@ -1460,12 +1460,30 @@ extern "C" int bfamx_batch_launch_graph(bfamx_batch_t *b)
 
 static int glue_release(bfamx_glue_t *g)
 {
-  /* restore the face tables of the glued faces */
-  for (const auto &f : g->saved)
+  /* restore the face tables of the glued faces: one download, patch and
+   * upload per subdomain */
+  for (size_t i = 0; i < g->subs.size(); ++i)
   {
-    bfamx_subdomain_t *s = g->subs[f.sub];
-    (void)hipMemcpy((void *)(s->h.nbr_elem + f.idx), &f.elem, 4, hipMemcpyHostToDevice);
-    (void)hipMemcpy((void *)(s->h.nbr_code + f.idx), &f.code, 1, hipMemcpyHostToDevice);
+    bfamx_subdomain_t *s = g->subs[i];
+    bool any = false;
+    for (const auto &f : g->saved)
+      any = any || f.sub == (int)i;
+    if (!any)
+      continue;
+    const size_t nfc = (size_t)s->K * 6;
+    std::vector<int32_t> ne(nfc);
+    std::vector<uint8_t> nc(nfc);
+    if (hipMemcpy(ne.data(), s->h.nbr_elem, 4 * nfc, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(nc.data(), s->h.nbr_code, nfc, hipMemcpyDeviceToHost) != hipSuccess)
+      continue;
+    for (const auto &f : g->saved)
+      if (f.sub == (int)i)
+      {
+        ne[f.idx] = f.elem;
+        nc[f.idx] = f.code;
+      }
+    (void)hipMemcpy((void *)s->h.nbr_elem, ne.data(), 4 * nfc, hipMemcpyHostToDevice);
+    (void)hipMemcpy((void *)s->h.nbr_code, nc.data(), nfc, hipMemcpyHostToDevice);
   }
   for (auto *s : g->subs)
     if (s->glue == g)
