@@ -336,10 +336,12 @@ class Device:
         _check(lib().bfamx_set_tuning(self.h, int(xcd_remap)))
 
     def set_kernel(self, precision, N, kind):
-        """elastic stage kernel for (precision, N): 0 line, 1 node, -1 unknown"""
+        """elastic stage kernel for (precision, N): 0 line, 1 node, -1 unknown
+        (the per-precision default until autotuned)"""
         _check(lib().bfamx_set_kernel(self.h, int(precision), int(N), int(kind)))
 
     def get_kernel(self, precision, N):
+        """the elastic stage kernel that runs for (precision, N): 0 line, 1 node"""
         k = C.c_int()
         _check(lib().bfamx_get_kernel(self.h, int(precision), int(N), C.byref(k)))
         return k.value

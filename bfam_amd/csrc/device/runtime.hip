@@ -858,10 +858,30 @@ static int grid_for(const bfamx_device_t *d, int ntasks)
   return d->num_xcd * ((ntasks + d->num_xcd - 1) / d->num_xcd);
 }
 
+/* the stage kernel of an order not (yet) autotuned: the faster one in the
+ * per-order measurements of both kernels on 30^3-sized bricks (node-parallel
+ * for FP32 from N = 4, for FP32_RHS at every order, for FP64 from N = 6) */
+static int default_node(int prec, int N)
+{
+  switch (prec)
+  {
+  case PREC_FP32: return N >= 4;
+  case PREC_FP32_RHS: return 1;
+  case PREC_FP64: return N >= 6;
+  default: return 0;
+  }
+}
+
 static int node_kernel(const bfamx_device_t *d, int prec, int N)
 {
-  return N >= 0 && N <= 8 && d->kchoice[prec & 3][N] == 1;
+  if (N < 3 || N > 7)
+    return 0;
+  const signed char k = d->kchoice[prec & 3][N];
+  return k < 0 ? default_node(prec, N) : k == 1;
 }
+
+/* does any order of the batch run the node-parallel kernel */
+static int batch_any_node(const bfamx_batch_t *b);
 
 static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
 {
@@ -1215,8 +1235,11 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
   }
   if (N < 0)
   {
+    /* until autotuned: one launch per order, which measured faster than
+     * the order-switching launch in every precision, and runs each order
+     * with the kernel a single-subdomain launch of it uses */
     const signed char c = dev->mixed_choice[prec][b->order_mask & 255];
-    b->grouped = c > 0;
+    b->grouped = c != 0 || batch_any_node(b);
   }
   std::vector<void *> allocs;
   void *p;
@@ -1273,6 +1296,14 @@ extern "C" int bfamx_batch_free(bfamx_batch_t *b)
   b->dev->used -= b->bytes;
   delete b;
   return BFAMX_OK;
+}
+
+static int batch_any_node(const bfamx_batch_t *b)
+{
+  for (int n = 0; n < 9; ++n)
+    if (b->grp_cnt[n] && node_kernel(b->dev, b->prec, n))
+      return 1;
+  return 0;
 }
 
 static GlueArgs glue_args(const bfamx_batch_t *b)
@@ -1803,7 +1834,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     need = 1;
   if (!need)
   {
-    b->grouped = b->N < 0 ? mc > 0 : 0;
+    b->grouped = b->N < 0 ? (mc > 0 || batch_any_node(b)) : 0;
     if (choice)
       *choice = b->grouped;
     return BFAMX_OK;
@@ -1859,8 +1890,13 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     }
     kc[n] = rc ? -1 : (ms[1] < ms[0] ? 1 : 0);
   }
-  /* (b) one order-switching launch or one launch per order */
+  /* (b) one order-switching launch or one launch per order.  The
+   * order-switching launch is the line kernel: it is a candidate only when
+   * every order of the batch runs the line kernel, so that each element is
+   * computed by the same code as in a single-subdomain launch */
   const int keep = b->grouped;
+  if (!rc && b->N < 0 && (mc < 0 || force) && batch_any_node(b))
+    mc = 1;
   if (!rc && b->N < 0 && (mc < 0 || force))
   {
     float ms[2] = {0, 0};
@@ -1890,7 +1926,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     b->grouped = keep;
     return rc;
   }
-  b->grouped = b->N < 0 ? mc > 0 : 0;
+  b->grouped = b->N < 0 ? (mc > 0 || batch_any_node(b)) : 0;
   if (choice)
     *choice = b->grouped;
   return BFAMX_OK;
@@ -1908,7 +1944,7 @@ extern "C" int bfamx_get_kernel(const bfamx_device_t *d, int prec, int N, int *k
 {
   if (!d || prec < 0 || prec > 3 || N < 0 || N > 8 || !kind)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_get_kernel: bad arguments");
-  *kind = d->kchoice[prec][N] == 1 ? 1 : 0;
+  *kind = node_kernel(d, prec, N);
   return BFAMX_OK;
 }
 

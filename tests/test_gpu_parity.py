@@ -249,6 +249,9 @@ def test_mixed_order_launch_strategies(dev, prec):
     per order give the same bits, and autotuning (which times real launches
     on the batch) leaves the state unchanged."""
     print(f"mixed vs grouped prec={prec}", flush=True)
+    # the order-switching kernel is the line kernel: compare line launches
+    for N in range(3, 8):
+        dev.set_kernel(prec, N, 0)
     res = []
     for strategy in (0, 1, "auto"):
         subs = []
@@ -263,7 +266,7 @@ def test_mixed_order_launch_strategies(dev, prec):
             # compare on the line kernels: the autotuner must not have
             # touched the state
             for N in range(3, 8):
-                dev.set_kernel(prec, N, -1)
+                dev.set_kernel(prec, N, 0)
         else:
             b.set_strategy(strategy)
         b.step(2)
@@ -271,6 +274,8 @@ def test_mixed_order_launch_strategies(dev, prec):
         b.close()
         for s in subs:
             s.close()
+    for N in range(3, 8):
+        dev.set_kernel(prec, N, -1)
     for r in res[1:]:
         for a, c in zip(res[0], r):
             assert np.array_equal(a, c)
@@ -311,6 +316,8 @@ def test_persistent_launch_invariant(dev, prec):
     task, for an element count that leaves partial last tasks (N=4: 4
     elements per wavefront)."""
     res = []
+    for N in (7, 4):
+        dev.set_kernel(prec, N, 0)  # persistence is a line-kernel mode
     for on in (1, 0):
         dev.set_persistent(on)
         for N in (7, 4):
@@ -324,6 +331,8 @@ def test_persistent_launch_invariant(dev, prec):
                 assert rel_max(res[-1], want) < TOL[prec]
             s.close()
     dev.set_persistent(0)
+    for N in (7, 4):
+        dev.set_kernel(prec, N, -1)
     assert np.array_equal(res[0], res[2]) and np.array_equal(res[1], res[3])
 
 
