@@ -394,3 +394,20 @@ def test_boundary_rejects_interior_face(dev):
         s.set_boundary(np.ones((cfg["mesh"].K, 6), np.int8))
     assert e.value.code == bx.EINVAL
     s.close()
+
+
+def test_default_kernel_policy():
+    """Until autotuned, each precision and order runs the kernel that won its
+    row of the per-order measurements; set_kernel overrides it and -1 goes
+    back to that default."""
+    d = bx.Device(0)
+    try:
+        assert d.get_kernel(bx.FP32, 7) == 1 and d.get_kernel(bx.FP32, 3) == 0
+        assert all(d.get_kernel(bx.FP32_RHS, N) == 1 for N in range(3, 8))
+        assert d.get_kernel(bx.FP64, 5) == 0 and d.get_kernel(bx.FP64, 7) == 1
+        d.set_kernel(bx.FP32, 7, 0)
+        assert d.get_kernel(bx.FP32, 7) == 0
+        d.set_kernel(bx.FP32, 7, -1)
+        assert d.get_kernel(bx.FP32, 7) == 1
+    finally:
+        d.close()
