@@ -13,15 +13,21 @@
  *   (bfamo_lsrk_t coefficients, bfam.c:9016-9040).
  *
  * One workgroup handles EPB elements, one thread per volume node:
- *   1. stream q[e] (NF*Np contiguous) from HBM; stage it in LDS
+ *   1. stream q[e] (NF*Np contiguous) from HBM and stage it in LDS: by
+ *      LDS-DMA when LDS holds it exactly (FP32, FP64), through registers
+ *      otherwise; the dq rows follow into LDS (FP32 rates) or registers
+ *      (FP64), except on stages with A = 0, which do not read dq
  *   2. face tasks (one thread per face node): minus trace from LDS, plus trace
- *      gathered from the neighbour's q through nbr_elem/nbr_code (the on-device
- *      replacement of the glue trace exchange), upwind fluctuations -> LDS
+ *      (velocity and the stress column of the face's axis) gathered from the
+ *      neighbour's q through nbr_elem/nbr_code (the on-device replacement of
+ *      the glue trace exchange), upwind fluctuations -> LDS
  *   3. 18 directional derivatives per node from LDS (2(N+1) flop each),
  *      folded into the rates one axis at a time
  *   4. node owners lift the fluctuations of their faces (fixed face order)
- *   5. dq read, dq and q_out written (q ping-pongs so neighbours read the
- *      stage input); nothing but the loop state is held across a barrier
+ *   5. dq and q_out written (q ping-pongs so neighbours read the stage
+ *      input); nothing but the loop state is held across a barrier
+ * The task, descriptor, clock and, for one-element blocks, the element
+ * tables are read through the constant address space (scalar loads).
  */
 #include "elastic_axis.hpp"
 #include "physics.hpp"

@@ -9,8 +9,9 @@
  * wavefronts:
  *   - the state moves between HBM and LDS as coalesced rows (lane l takes
  *     nodes VW*(l + 64 r) .. +VW-1 of each field row, 16-byte accesses);
- *     LDS holds it densely (node (i,j,k) at i + n1 j + n1^2 k) in the
- *     arithmetic type;
+ *     LDS holds it densely in the arithmetic type (node (i,j,k) at
+ *     i + n1 j + n1^2 k, swizzled at N = 7 so that every pass is free of
+ *     bank conflicts, see LineCfg);
  *   - lane l = (a, b) = (l % n1, l / n1) of its element's n1 x n1 lines;
  *   - x pass: lane owns the i-line (j, k) = (a, b) and accumulates the nine
  *     rates of that line in registers; the contraction with D uses the
