@@ -860,14 +860,14 @@ static int grid_for(const bfamx_device_t *d, int ntasks)
 
 /* the stage kernel of an order not (yet) autotuned: the faster one in the
  * per-order measurements of both kernels on 30^3-sized bricks (node-parallel
- * for FP32 from N = 4, for FP32_RHS at every order, for FP64 from N = 6) */
+ * for FP32 and FP32_RHS at every order, for FP64 from N = 5) */
 static int default_node(int prec, int N)
 {
   switch (prec)
   {
-  case PREC_FP32: return N >= 4;
+  case PREC_FP32:
   case PREC_FP32_RHS: return 1;
-  case PREC_FP64: return N >= 6;
+  case PREC_FP64: return N >= 5;
   default: return 0;
   }
 }

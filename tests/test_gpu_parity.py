@@ -402,9 +402,9 @@ def test_default_kernel_policy():
     back to that default."""
     d = bx.Device(0)
     try:
-        assert d.get_kernel(bx.FP32, 7) == 1 and d.get_kernel(bx.FP32, 3) == 0
+        assert all(d.get_kernel(bx.FP32, N) == 1 for N in range(3, 8))
         assert all(d.get_kernel(bx.FP32_RHS, N) == 1 for N in range(3, 8))
-        assert d.get_kernel(bx.FP64, 5) == 0 and d.get_kernel(bx.FP64, 7) == 1
+        assert d.get_kernel(bx.FP64, 4) == 0 and d.get_kernel(bx.FP64, 5) == 1
         d.set_kernel(bx.FP32, 7, 0)
         assert d.get_kernel(bx.FP32, 7) == 0
         d.set_kernel(bx.FP32, 7, -1)
