@@ -417,14 +417,19 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   }
 }
 
-/* waves per SIMD the register allocation must leave room for, for the
+/* waves per SIMD the register allocation must leave room for.  For the
  * one-element blocks of N = 6, 7 (two waves per SIMD each): 6 keeps three
  * blocks per CU for the FP32 stage; FP32_RHS and FP64 keep two (forcing
  * FP32_RHS to three spilled registers and measured slower, 3.01 against
- * 2.88 ms per 48^3 stage).  Smaller orders are left to the compiler. */
+ * 2.88 ms per 48^3 stage).  FP64 at N = 5 (two elements per block): 4
+ * keeps two blocks per CU at 13 spilled registers, 1.12 against 1.22 ms per
+ * 40^3 stage; at N = 3, 4 the spills cost more than the second block.
+ * Other cases are left to the compiler. */
 template <int N, int EPB, class P> struct NodeWaves
 {
-  static constexpr int value = EPB != 1 ? 1 : (sizeof(typename P::TS) == 4 ? 6 : 4);
+  static constexpr int value =
+      EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4)
+               : ((N == 5 && sizeof(typename P::TC) == 8) ? 4 : 1);
 };
 
 template <int N, int EPB, class P>
