@@ -93,7 +93,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
    * fields, 16 bytes per lane where every element block is 16-byte aligned */
   constexpr bool DMA_Q = sizeof(TS) == sizeof(TC);
   constexpr bool DMA_DQ = SM::DW > 0;
-  constexpr int DMA_W = (9 * Np * (int)sizeof(TS) / 4) % 4 == 0 ? 16 : 4;
+  constexpr int DMA_WQ = (9 * Np * (int)sizeof(TS) / 4) % 4 == 0 ? 16 : 4;
+  constexpr int DMA_WD = (9 * Np * (int)sizeof(TR) / 4) % 4 == 0 ? 16 : 4;
 
   const int tid = threadIdx.x;
   /* one element per block (N >= 6): the element index is uniform, and its
@@ -127,9 +128,9 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   TS qs[KEEP_Q ? NF : 1];
   if constexpr (DMA_Q)
   {
-    block_to_lds<DMA_W, BS>((gptr<const float>)(qin + (size_t)tk.e0 * NF * Np),
-                            reinterpret_cast<float *>(sq),
-                            tk.ne * NF * Np * (int)(sizeof(TS) / 4));
+    block_to_lds<DMA_WQ, BS>((gptr<const float>)(qin + (size_t)tk.e0 * NF * Np),
+                             reinterpret_cast<float *>(sq),
+                             tk.ne * NF * Np * (int)(sizeof(TS) / 4));
   }
   else if (act)
   {
@@ -151,12 +152,13 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   if constexpr (DMA_DQ)
   {
     if (rd_dq)
-      block_to_lds<DMA_W, BS>(
+      block_to_lds<DMA_WD, BS>(
           (gptr<const float>)glob(static_cast<const float *>(sd.dq)) + (size_t)tk.e0 * NF * Np,
           reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
   }
-  /* FP64 rates have no room in LDS: this node's dq values load into
-   * registers instead, also under the face and volume work */
+  /* rates without room in LDS (FP64, and FP32 where a third block per CU
+   * would not fit): this node's dq values load into registers instead, also
+   * under the face and volume work */
   TR dqs[DMA_DQ ? 1 : NF];
   if constexpr (!DMA_DQ)
   {
