@@ -52,9 +52,12 @@ using NodeSmem = ElasticSmem<N, EPB, typename P::TC, NodeDq<N, EPB, P>::value>;
  * LDS with LDS-DMA (global_load_lds), W bytes per lane: the LDS image is the
  * global one.  Each wave's destination is uniform (lane l lands at
  * base + W*l); completion is tracked by vmcnt.  W = 16 needs count and both
- * addresses 16-byte aligned.
+ * addresses 16-byte aligned.  CPOL is the load's cache policy: 0, or
+ * CPOL_NT for data no other block reads (streamed past L2 retention).
  */
-template <int W, int BLOCK>
+constexpr int CPOL_NT = 2;
+
+template <int W, int BLOCK, int CPOL = 0>
 __device__ __forceinline__ void block_to_lds(gptr<const float> src, float *dst,
                                              int count)
 {
@@ -67,9 +70,9 @@ __device__ __forceinline__ void block_to_lds(gptr<const float> src, float *dst,
     if (x < count)
     {
       if constexpr (W == 16)
-        __builtin_amdgcn_global_load_lds(src + x, d, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(src + x, d, 16, 0, CPOL);
       else
-        __builtin_amdgcn_global_load_lds(src + x, d, 4, 0, 0);
+        __builtin_amdgcn_global_load_lds(src + x, d, 4, 0, CPOL);
     }
   }
 }
@@ -144,7 +147,10 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       sq[(le * NF + f) * Np + n] = (TC)v;
     }
   }
-  /* the dq rows load into LDS under the face and volume work */
+  /* the dq rows load into LDS under the face and volume work.  Only this
+   * block reads them, and the outputs are written non-temporally: L2 then
+   * keeps the states that neighbouring blocks gather their face traces
+   * from (FP32 48^3 N=7: 1.635 -> 1.548 ms per stage) */
   const bool upd = !(mode == MODE_RHS || PART != PART_FUSED);
   /* a 2N-storage scheme starts every step with A = 0 (dq = dt L): the old
    * dq is not read then */
@@ -152,7 +158,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   if constexpr (DMA_DQ)
   {
     if (rd_dq)
-      block_to_lds<DMA_WD, BS>(
+      block_to_lds<DMA_WD, BS, CPOL_NT>(
           (gptr<const float>)glob(static_cast<const float *>(sd.dq)) + (size_t)tk.e0 * NF * Np,
           reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
   }
@@ -167,7 +173,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + (size_t)e * NF * Np + n;
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        dqs[f] = dqe[f * Np];
+        dqs[f] = __builtin_nontemporal_load(&dqe[f * Np]);
     }
   }
   /* LDS-DMA completes by vmcnt: every wave drains its own before the
@@ -414,8 +420,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
         dqo = dqs[f];
     }
     const TR d = ca * dqo + cdt * (TR)rate[f];
-    dqe[f * Np] = d;
-    qo[f * Np] = qv + cb * (TS)d;
+    __builtin_nontemporal_store(d, &dqe[f * Np]);
+    __builtin_nontemporal_store((TS)(qv + cb * (TS)d), &qo[f * Np]);
   }
 }
 
