@@ -274,10 +274,10 @@ __device__ __forceinline__ size_t qidx(int e, int f, int n, int NF, int Np)
 
 /*
  * Order in which a block walks the task list.  Hardware dispatches
- * consecutive workgroups round-robin over the XCDs; with remapping, XCD x
- * instead owns the contiguous slice [x*C, (x+1)*C) of the (Morton-ordered)
- * task list, so face neighbours of a task are mostly resident in the same
- * XCD's L2.
+ * consecutive workgroups round-robin over the XCDs; with remapping (an
+ * option, off by default), XCD x instead owns the contiguous slice
+ * [x*C, (x+1)*C) of the (Morton-ordered) task list, so face neighbours of a
+ * task are mostly resident in the same XCD's L2.
  */
 __device__ __forceinline__ int first_task(const LaunchArgs &a, int *stride,
                                           int *end)

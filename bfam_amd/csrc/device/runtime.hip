@@ -165,7 +165,11 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
       nx < 1)
     nx = 1;
   d->num_xcd = nx;
-  d->xcd_remap = 1;
+  /* hardware round-robin block order by default: with non-temporal dq and
+   * outputs it measured faster than giving each XCD a contiguous element
+   * range (48^3 N=7 stage: FP32 1.50 against 1.55 ms, FP64 2.85 against
+   * 2.99 ms) */
+  d->xcd_remap = 0;
   memset(d->mixed_choice, -1, sizeof(d->mixed_choice));
   d->persistent = 0;
   memset(d->line_bpc, 0, sizeof(d->line_bpc));

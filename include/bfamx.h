@@ -318,7 +318,7 @@ int bfamx_timer_start(bfamx_device_t *dev);
 int bfamx_timer_stop(bfamx_device_t *dev, float *ms);
 
 /* launch tuning: blocks remapped so each XCD walks a contiguous element
- * range (1, default) or hardware round-robin order (0) */
+ * range (1) or hardware round-robin order (0, default: measured faster) */
 int bfamx_set_tuning(bfamx_device_t *dev, int xcd_remap);
 /* elastic stage launches: 1 at most the resident wavefronts, each looping
  * over tasks and streaming the next task's state into LDS while it writes

@@ -161,7 +161,7 @@ def test_xcd_remap_invariant(dev):
         s.step(2)
         res.append(s.download())
         s.close()
-    dev.set_tuning(1)
+    dev.set_tuning(0)
     assert np.array_equal(res[0], res[1])
 
 
