@@ -332,7 +332,7 @@ class Device:
         _check(lib().bfamx_timer_stop(self.h, C.byref(ms)))
         return ms.value
 
-    def set_tuning(self, xcd_remap=0):
+    def set_tuning(self, xcd_remap=2):
         _check(lib().bfamx_set_tuning(self.h, int(xcd_remap)))
 
     def set_kernel(self, precision, N, kind):

@@ -274,16 +274,28 @@ __device__ __forceinline__ size_t qidx(int e, int f, int n, int NF, int Np)
 
 /*
  * Order in which a block walks the task list.  Hardware dispatches
- * consecutive workgroups round-robin over the XCDs; with remapping (an
- * option, off by default), XCD x instead owns the contiguous slice
- * [x*C, (x+1)*C) of the (Morton-ordered) task list, so face neighbours of a
- * task are mostly resident in the same XCD's L2.
+ * consecutive workgroups round-robin over the XCDs.  Mode 0 keeps that
+ * order; mode 1 gives XCD x the contiguous slice [x*C, (x+1)*C) of the
+ * (Morton-ordered) task list; mode 2 (the default) keeps one round-robin
+ * front over the list but hands each XCD whole 2x2x2 Morton cubes.
  */
 __device__ __forceinline__ int first_task(const LaunchArgs &a, int *stride,
                                           int *end)
 {
   const int b = blockIdx.x, G = gridDim.x;
-  if (!a.xcd_remap || G % a.nxcd)
+  if (a.xcd_remap == 2 && a.nxcd == 8 && G >= a.ntasks)
+  {
+    /* round-robin front, but each XCD's 8 blocks of a group of 64 take one
+     * 2x2x2 Morton cube of tasks, whose inner faces then share an L2; a
+     * partial last group keeps the identity */
+    *stride = G;
+    *end = a.ntasks;
+    const int g0 = b & ~63;
+    if (g0 + 64 > a.ntasks)
+      return b;
+    return g0 + 8 * (b & 7) + ((b & 63) >> 3);
+  }
+  if (a.xcd_remap != 1 || G % a.nxcd)
   {
     *stride = G;
     *end = a.ntasks;

@@ -165,11 +165,11 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
       nx < 1)
     nx = 1;
   d->num_xcd = nx;
-  /* hardware round-robin block order by default: with non-temporal dq and
-   * outputs it measured faster than giving each XCD a contiguous element
-   * range (48^3 N=7 stage: FP32 1.50 against 1.55 ms, FP64 2.85 against
-   * 2.99 ms) */
-  d->xcd_remap = 0;
+  /* block order: one round-robin front over the elements, each XCD taking
+   * whole 2x2x2 Morton cubes (48^3 N=7 stage: FP32 1.48 ms, against 1.50
+   * for the plain round-robin order and 1.55 for contiguous XCD slices;
+   * FP64 2.79, 2.87 and 2.99 ms) */
+  d->xcd_remap = 2;
   memset(d->mixed_choice, -1, sizeof(d->mixed_choice));
   d->persistent = 0;
   memset(d->line_bpc, 0, sizeof(d->line_bpc));
@@ -250,7 +250,9 @@ extern "C" int bfamx_set_tuning(bfamx_device_t *d, int xcd_remap)
 {
   if (!d)
     return bfamx_fail(BFAMX_EINVAL, "NULL device");
-  d->xcd_remap = xcd_remap ? 1 : 0;
+  if (xcd_remap < 0 || xcd_remap > 2)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_tuning: mode 0, 1 or 2");
+  d->xcd_remap = xcd_remap;
   return BFAMX_OK;
 }
 
@@ -857,7 +859,7 @@ static int grid_line(bfamx_device_t *d, int prec, int N, int ntasks);
 
 static int grid_for(const bfamx_device_t *d, int ntasks)
 {
-  if (!d->xcd_remap || d->num_xcd <= 1)
+  if (d->xcd_remap != 1 || d->num_xcd <= 1)
     return ntasks;
   return d->num_xcd * ((ntasks + d->num_xcd - 1) / d->num_xcd);
 }
@@ -1954,7 +1956,7 @@ extern "C" int bfamx_get_kernel(const bfamx_device_t *d, int prec, int N, int *k
 
 static int grid_line(bfamx_device_t *d, int prec, int N, int ntasks)
 {
-  const int nx = d->xcd_remap && d->num_xcd > 1 ? d->num_xcd : 1;
+  const int nx = d->xcd_remap == 1 && d->num_xcd > 1 ? d->num_xcd : 1;
   int g = nx * ((ntasks + nx - 1) / nx);
   if (!d->persistent)
     return g;

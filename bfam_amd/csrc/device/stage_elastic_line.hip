@@ -783,7 +783,7 @@ __device__ __forceinline__ int first_wave_task(const LaunchArgs &a, int *stride,
    * and every table read that follows from it, in scalar registers */
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.x, G = gridDim.x;
-  if (!a.xcd_remap || G % a.nxcd)
+  if (a.xcd_remap != 1 || G % a.nxcd)
   {
     *stride = G * LINE_WPB;
     *end = a.ntasks;

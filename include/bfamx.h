@@ -317,8 +317,10 @@ int bfamx_batch_rhs(bfamx_batch_t *b, double t, double *const *const *rhs);
 int bfamx_timer_start(bfamx_device_t *dev);
 int bfamx_timer_stop(bfamx_device_t *dev, float *ms);
 
-/* launch tuning: blocks remapped so each XCD walks a contiguous element
- * range (1) or hardware round-robin order (0, default: measured faster) */
+/* launch tuning, the order in which blocks take elements: 0 hardware
+ * round-robin over the XCDs, 1 a contiguous element range per XCD, 2 (the
+ * default, measured fastest) one round-robin front with each XCD taking
+ * whole 2x2x2 Morton cubes, so that their inner faces share an L2 */
 int bfamx_set_tuning(bfamx_device_t *dev, int xcd_remap);
 /* elastic stage launches: 1 at most the resident wavefronts, each looping
  * over tasks and streaming the next task's state into LDS while it writes
