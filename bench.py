@@ -19,8 +19,9 @@ The JSON line also carries:
   roofline           the FP32 stage kernel against HBM: algorithmic bytes
                      (q read + write, dq read + write, 4 B each = 16 B per
                      DOF; 12 B on the first stage of a step, which does not
-                     read dq: 15.2 B on average) over the mean stage
-                     duration, and the HBM traffic
+                     read dq, and on the last, whose dq nothing reads: 14.4 B
+                     on average) over the mean stage duration, and the HBM
+                     traffic
                      per launch measured with FETCH_SIZE / WRITE_SIZE,
                      calibrated on a plain copy kernel (read from the text
                      summary under profiles/);
@@ -54,11 +55,12 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per DOF-stage, averaged over the five stages of LSRK4(5):
 # q read + written and dq read + written, except that the first stage of a
-# step (A = 0) does not read dq: FP32 (4 x 16 + 12) / 5, FP32_RHS
-# (4 x 24 + 20) / 5, FP64 (4 x 32 + 24) / 5
-BYTES_PER_DOF = {"fp32": 15.2, "fp32_rhs": 23.2, "fp64": 30.4, "fp64_plain": 80}
-# the line kernel reads dq on every stage
-BYTES_PER_DOF_LINE = {"fp32": 16, "fp32_rhs": 24, "fp64": 32}
+# step (A = 0) does not read dq and the last one does not write it (the next
+# step's first stage does not read it): FP32 (12 + 3 x 16 + 12) / 5,
+# FP32_RHS (20 + 3 x 24 + 20) / 5, FP64 (24 + 3 x 32 + 24) / 5
+BYTES_PER_DOF = {"fp32": 14.4, "fp32_rhs": 22.4, "fp64": 28.8, "fp64_plain": 80}
+# the line kernel reads dq on every stage: (4 x 16 + 12) / 5 ...
+BYTES_PER_DOF_LINE = {"fp32": 15.2, "fp32_rhs": 23.2, "fp64": 30.4}
 TRAFFIC_FILES = {"line": "traffic_elastic_line_fp32_48cube_N7.txt",
                  "node-parallel": "traffic_elastic_stage_fp32_48cube_N7.txt"}
 

@@ -355,6 +355,9 @@ __device__ __forceinline__ void clock_advance(const SubDev &sd)
 struct StageCtx
 {
   int parity;
+  /* the dq this stage writes is never read: the last stage of a step of a
+   * scheme whose first stage has A = 0 (every 2N-storage scheme here) */
+  int dq_dead;
   double a, b, dt, t;
   double g, sw, cw;
 };
@@ -372,6 +375,7 @@ __device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
   if (mode == MODE_RHS)
   {
     c.parity = (int)(clk->stage & 1);
+    c.dq_dead = 0;
     c.a = 0;
     c.b = 0;
     c.dt = 0;
@@ -387,6 +391,7 @@ __device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
   const long long st = clk->stage;
   const int s = (int)(st % sd.nstages);
   c.parity = (int)(st & 1);
+  c.dq_dead = s == sd.nstages - 1 && sd.A[0] == 0.0;
   c.a = sd.A[s];
   c.b = sd.B[s];
   c.dt = clk->dt;

@@ -197,7 +197,8 @@ __device__ __forceinline__ void acoustic_task(const SD &sd, const Task &tk,
   {
     const size_t id = qidx(e, f, n, NF, Np);
     const TR d = ca * dqp[id] + cdt * (TR)rate[f];
-    dqp[id] = d;
+    if (!cx.dq_dead)
+      dqp[id] = d;
     qout[id] = qs[f] + cb * (TS)d;
   }
 }

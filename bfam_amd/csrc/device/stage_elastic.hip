@@ -420,7 +420,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
         dqo = dqs[f];
     }
     const TR d = ca * dqo + cdt * (TR)rate[f];
-    __builtin_nontemporal_store(d, &dqe[f * Np]);
+    if (!cx.dq_dead)
+      __builtin_nontemporal_store(d, &dqe[f * Np]);
     __builtin_nontemporal_store((TS)(qv + cb * (TS)d), &qo[f * Np]);
   }
 }

@@ -755,7 +755,8 @@ __device__ __forceinline__ void elastic_line_task(const SD &sd,
       {
         const TR d = ca * dqv[f % PD][r][t] + cdt * (TR)rv[r][t];
         const TS qold = q_from_lds ? (TS)qo_l[r][t] : qv[q_from_lds ? 0 : f % PD][r][t];
-        dqe[t] = d;
+        if (!cx.dq_dead)
+          dqe[t] = d;
         qo[t] = qold + cb * (TS)d;
       }
     }
