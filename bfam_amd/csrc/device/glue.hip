@@ -82,26 +82,33 @@ __device__ __forceinline__ TC contract(const TC *M, int r, const TC *x, int o, i
 }
 
 /*
- * One wavefront per interface group.  Every global read of the group (the b
- * trace, the traces of its 1 or 4 mortars' s faces and their operators) is
- * issued up front into LDS, so the group waits on memory once.  The tensor
- * products run sum-factorised, one axis per step with the partial result in
- * LDS: lane (g, j) first contracts along i, then lane (g, h) along j.  Each
- * step sums in the same order as the nested form, so the results are those
- * of the direct double sum.
+ * One workgroup per interface group, one wavefront per mortar (wave m of
+ * four takes mortar m; a conforming group uses wave 0 only).  Every global
+ * read of the group (the b trace, the traces of its mortars' s faces and
+ * their operators) is issued up front into LDS, so the group waits on
+ * memory once.  The tensor products run sum-factorised, one axis per step
+ * with the partial result in LDS: lane (g, j) first contracts along i, then
+ * lane (g, h) along j.  Each step sums in the same order as the nested form,
+ * so the results are those of the direct double sum.  The big face's four
+ * contributions are added in mortar order by wave 0, as a single wave
+ * walking the mortars in turn would add them.
  */
+constexpr int GLUE_WAVES = 4;
+
 template <class TS, class TC>
-__global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
+__global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
 {
-  __shared__ TC tb[6][GLUE_MAXFP];    /* b trace */
-  __shared__ TC ts[4][6][GLUE_MAXFP]; /* s traces of the mortars */
-  __shared__ TC sop[4][6][GP * GP];   /* operators of the mortars */
-  __shared__ TC t1[6][GLUE_MAXFP], t2[6][GLUE_MAXFP]; /* partial contractions */
-  __shared__ TC fs[6][GLUE_MAXFP], fb[6][GLUE_MAXFP]; /* mortar fluctuations */
+  __shared__ TC tb[6][GLUE_MAXFP];                     /* b trace */
+  __shared__ TC ts[GLUE_WAVES][6][GLUE_MAXFP];         /* s traces */
+  __shared__ TC sop[GLUE_WAVES][6][GP * GP];           /* operators */
+  __shared__ TC t1[GLUE_WAVES][6][GLUE_MAXFP], t2[GLUE_WAVES][6][GLUE_MAXFP];
+  __shared__ TC fs[GLUE_WAVES][6][GLUE_MAXFP], fb[GLUE_WAVES][6][GLUE_MAXFP];
   const int gi = blockIdx.x;
   if (gi >= a.ngroups)
     return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  /* the wave index is uniform in the wave */
+  const int m = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lo = lane % GP, hi = lane / GP; /* padded (row, column) */
   /* group, descriptors, clocks and materials are read-only here: constant
    * address space, scalar loads */
@@ -114,14 +121,18 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
   const TC sgs = (b_face & 1) ? TC(-1) : TC(1); /* normal sign of the s side */
   gptr<const TC> ops = glob(static_cast<const TC *>(a.ops));
   gptr<TC> buf = glob(static_cast<TC *>(a.buf));
+  const bool mine = m < nm; /* this wave has a mortar */
 
   /* ---- every load of the group ---- */
-  load_trace<TS, TC>(B, b_elem, b_face, tb, lane);
-  for (int m = 0; m < nm; ++m)
+  if (m == 0)
+    load_trace<TS, TC>(B, b_elem, b_face, tb, lane);
+  int Ns = 0;
+  if (mine)
   {
     const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
     load_trace<TS, TC>(S, gp->s_elem[m], gp->s_face[m], ts[m], lane);
-    const int Ns = S.N, Ng = Ns > Nb ? Ns : Nb;
+    Ns = S.N;
+    const int Ng = Ns > Nb ? Ns : Nb;
     const int quad = nm == 1 ? quad0 : m;
     const int kI = quad < 0 ? 0 : 3 + (quad & 1), kJ = quad < 0 ? 0 : 3 + ((quad >> 1) & 1);
     const int kA = quad < 0 ? 0 : 1 + (quad & 1), kB = quad < 0 ? 0 : 1 + ((quad >> 1) & 1);
@@ -140,78 +151,91 @@ __global__ __launch_bounds__(64) void mortar_kernel(GlueArgs a)
   const TC Zpb = Mb[MAT_ZP], Zsb = Mb[MAT_ZS];
   __syncthreads();
 
-  TC acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int m = 0; m < nm; ++m)
+  const int ns1 = Ns + 1, nfs = ns1 * ns1;
+  const TC *Is = sop[m][OP_IS], *Ia = sop[m][OP_IA], *Ib = sop[m][OP_IB];
+  const TC *Ws = sop[m][OP_WS], *Wa = sop[m][OP_WA], *Wb = sop[m][OP_WB];
+  TC Zps = 0, Zss = 0, izp = 0, izs = 0;
+  if (mine)
   {
-    const int s_off = gp->s_off[m];
     const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
-    const int Ns = S.N, ns1 = Ns + 1, nfs = ns1 * ns1;
     cptr<TC> Ms = (cptr<TC>)mat_tab<TC>(S) + (size_t)gp->s_elem[m] * MAT_STRIDE;
-    const TC Zps = Ms[MAT_ZP], Zss = Ms[MAT_ZS];
-    const TC izp = TC(1) / (Zps + Zpb), izs = TC(1) / (Zss + Zsb);
-    const TC *Is = sop[m][OP_IS], *Ia = sop[m][OP_IA], *Ib = sop[m][OP_IB];
-    const TC *Ws = sop[m][OP_WS], *Wa = sop[m][OP_WA], *Wb = sop[m][OP_WB];
-
+    Zps = Ms[MAT_ZP];
+    Zss = Ms[MAT_ZS];
+    izp = TC(1) / (Zps + Zpb);
+    izs = TC(1) / (Zss + Zsb);
     /* interpolation to the mortar grid, first axis: lane (g, j) = (lo, hi) */
 #pragma unroll
     for (int c = 0; c < 6; ++c)
     {
-      t1[c][lane] = contract(Is, lo, ts[m][c], GP * hi, 1);
-      t2[c][lane] = contract(Ia, lo, tb[c], GP * hi, 1);
+      t1[m][c][lane] = contract(Is, lo, ts[m][c], GP * hi, 1);
+      t2[m][c][lane] = contract(Ia, lo, tb[c], GP * hi, 1);
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (mine)
+  {
     /* second axis and the Riemann solves: lane (g, h) = (lo, hi) */
+    TC us[6], ub[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
     {
-      TC us[6], ub[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-      {
-        us[c] = contract(Is, hi, t1[c], lo, GP);
-        ub[c] = contract(Ib, hi, t2[c], lo, GP);
-      }
-      TC dv[3], dT[3];
-      riemann_dispatch<TC>(A, sgs, us, us + 3, ub, ub + 3, Zps, Zss, Zpb, Zsb, izp, izs,
-                           dv, dT);
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-      {
-        fs[c][lane] = dv[c];
-        fs[3 + c][lane] = dT[c];
-      }
-      riemann_dispatch<TC>(A, -sgs, ub, ub + 3, us, us + 3, Zpb, Zsb, Zps, Zss, izp, izs,
-                           dv, dT);
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-      {
-        fb[c][lane] = dv[c];
-        fb[3 + c][lane] = dT[c];
-      }
+      us[c] = contract(Is, hi, t1[m][c], lo, GP);
+      ub[c] = contract(Ib, hi, t2[m][c], lo, GP);
     }
-    __syncthreads();
+    TC dv[3], dT[3];
+    riemann_dispatch<TC>(A, sgs, us, us + 3, ub, ub + 3, Zps, Zss, Zpb, Zsb, izp, izs, dv,
+                         dT);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+      fs[m][c][lane] = dv[c];
+      fs[m][3 + c][lane] = dT[c];
+    }
+    riemann_dispatch<TC>(A, -sgs, ub, ub + 3, us, us + 3, Zpb, Zsb, Zps, Zss, izp, izs, dv,
+                         dT);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+      fb[m][c][lane] = dv[c];
+      fb[m][3 + c][lane] = dT[c];
+    }
+  }
+  __syncthreads();
+  if (mine)
+  {
     /* mass projection back to each face, first axis: lane (i, h) = (lo, hi) */
 #pragma unroll
     for (int c = 0; c < 6; ++c)
     {
-      t1[c][lane] = contract(Ws, lo, fs[c], GP * hi, 1);
-      t2[c][lane] = contract(Wa, lo, fb[c], GP * hi, 1);
+      t1[m][c][lane] = contract(Ws, lo, fs[m][c], GP * hi, 1);
+      t2[m][c][lane] = contract(Wa, lo, fb[m][c], GP * hi, 1);
     }
-    __syncthreads();
-    /* second axis: lane (i, j) = (lo, hi) of each face */
+  }
+  __syncthreads();
+  if (mine)
+  {
+    /* second axis: lane (i, j) = (lo, hi) of each face; the big face's
+     * contribution goes to fs[m], free now, for the sum below */
 #pragma unroll
     for (int c = 0; c < 6; ++c)
     {
-      const TC x = contract(Ws, hi, t1[c], lo, GP);
+      const TC x = contract(Ws, hi, t1[m][c], lo, GP);
       if (lo < ns1 && hi < ns1)
-        buf[s_off + c * nfs + lo + ns1 * hi] = x;
-      acc[c] += contract(Wb, hi, t2[c], lo, GP);
+        buf[gp->s_off[m] + c * nfs + lo + ns1 * hi] = x;
+      fs[m][c][lane] = contract(Wb, hi, t2[m][c], lo, GP);
     }
-    __syncthreads();
   }
-  if (lo < nb1 && hi < nb1)
+  __syncthreads();
+  if (m == 0 && lo < nb1 && hi < nb1)
   {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-      buf[b_off + c * nfb + lo + nb1 * hi] = acc[c];
+    {
+      TC acc = 0;
+      for (int k = 0; k < nm; ++k)
+        acc += fs[k][c][lane];
+      buf[b_off + c * nfb + lo + nb1 * hi] = acc;
+    }
   }
 }
 
@@ -222,13 +246,16 @@ hipError_t launch_mortar(int prec, const GlueArgs &a, hipStream_t s)
   switch (prec)
   {
   case PREC_FP64:
-    hipLaunchKernelGGL((mortar_kernel<double, double>), dim3(a.ngroups), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((mortar_kernel<double, double>), dim3(a.ngroups),
+                       dim3(64 * GLUE_WAVES), 0, s, a);
     break;
   case PREC_FP32_RHS:
-    hipLaunchKernelGGL((mortar_kernel<double, float>), dim3(a.ngroups), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((mortar_kernel<double, float>), dim3(a.ngroups),
+                       dim3(64 * GLUE_WAVES), 0, s, a);
     break;
   case PREC_FP32:
-    hipLaunchKernelGGL((mortar_kernel<float, float>), dim3(a.ngroups), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((mortar_kernel<float, float>), dim3(a.ngroups),
+                       dim3(64 * GLUE_WAVES), 0, s, a);
     break;
   default:
     return hipErrorInvalidValue;
