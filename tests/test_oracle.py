@@ -1,5 +1,5 @@
 """Pins of the CPU oracle (oracle/oracle.c) and of the library's host-side
-operator setup: reference-held constants, survey probe values of the compiled
+operator setup: reference-held constants, probe values printed by the compiled
 reference, closed forms and invariants that a dropped term, a wrong sign or a
 transposed operand would break.  CPU only."""
 import json
