@@ -17,11 +17,11 @@ The JSON line also carries:
                      the same initial state (LGL-quadrature weighted, all
                      nine fields);
   roofline           the FP32 stage kernel against HBM: algorithmic bytes
-                     (q read + write, dq read + write, 4 B each = 16 B per
-                     DOF; 12 B on the first stage of a step, which does not
-                     read dq, and on the last, whose dq nothing reads: 14.4 B
-                     on average) over the mean stage duration, and the HBM
-                     traffic
+                     (q read, previous q read, next q written, 4 B each =
+                     12 B per DOF; 8 B on the first stage of a step, which
+                     needs no dq: 11.2 B on average; dq is rebuilt from the
+                     two states, never stored) over the mean stage duration,
+                     and the HBM traffic
                      per launch measured with FETCH_SIZE / WRITE_SIZE,
                      calibrated on a plain copy kernel (read from the text
                      summary under profiles/);
@@ -53,14 +53,15 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per DOF-stage, averaged over the five stages of LSRK4(5):
-# q read + written and dq read + written, except that the first stage of a
-# step (A = 0) does not read dq and the last one does not write it (the next
-# step's first stage does not read it): FP32 (12 + 3 x 16 + 12) / 5,
-# FP32_RHS (20 + 3 x 24 + 20) / 5, FP64 (24 + 3 x 32 + 24) / 5
-BYTES_PER_DOF = {"fp32": 14.4, "fp32_rhs": 22.4, "fp64": 28.8, "fp64_plain": 80}
-# the line kernel reads dq on every stage: (4 x 16 + 12) / 5 ...
-BYTES_PER_DOF_LINE = {"fp32": 15.2, "fp32_rhs": 23.2, "fp64": 30.4}
+# algorithmic bytes per DOF-stage, averaged over the five stages of LSRK4(5).
+# FP32 and FP64 keep no dq: every stage reads q and writes the next q, and
+# every stage but the first of a step (A = 0) also reads the previous q, from
+# which it rebuilds dq: FP32 (8 + 4 x 12) / 5, FP64 (16 + 4 x 24) / 5.
+# FP32_RHS (FP64 state, FP32 dq) reads and writes dq as well, except that the
+# first stage does not read it and the last does not write it (the next
+# step's first stage does not read it): (20 + 3 x 24 + 20) / 5.  Both stage
+# kernels move the same bytes.
+BYTES_PER_DOF = {"fp32": 11.2, "fp32_rhs": 22.4, "fp64": 22.4, "fp64_plain": 80}
 TRAFFIC_FILES = {"line": "traffic_elastic_line_fp32_48cube_N7.txt",
                  "node-parallel": "traffic_elastic_stage_fp32_48cube_N7.txt"}
 
@@ -277,7 +278,7 @@ def main():
     head_name = "fp32" if "fp32" in res else next(iter(res))
     launch_s = head["ms_per_stage"] * 1e-3
     head_kernel = head.get("kernel")  # None for the unfused FP64 form
-    bpd = (BYTES_PER_DOF_LINE if head_kernel == "line" else BYTES_PER_DOF)[head_name]
+    bpd = BYTES_PER_DOF[head_name]
     achieved = bpd * ndof / launch_s / 1e9
     traffic = read_traffic(head_kernel) if head_name == "fp32" else None
 

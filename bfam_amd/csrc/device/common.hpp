@@ -121,7 +121,10 @@ struct SubDev
   int nstages;
   int src_elem; /* -1 none */
   int mms;      /* 1: manufactured forcing */
-  int pad0;
+  /* 1: stage s >= 1 rebuilds the previous dq from the two state buffers,
+   * dq_{s-1} = (q_s - q_{s-1}) / B_{s-1}, and dq is never stored (state and
+   * rates of one type, A[0] = 0, B[0..S-2] nonzero) */
+  int dq_rec;
   void *q[2];   /* state ping-pong (double or float) */
   void *dq;     /* rate register (double or float) */
   double *rhs;  /* FP64 RHS (plain path and MODE_RHS) */
@@ -149,6 +152,7 @@ struct SubDev
    * the mortar kernel before each stage: dv[3], dT[3] per face node */
   void *glue_buf;
   double A[8], B[8], C[9];
+  double AB[8]; /* A[s] / B[s-1] (dq_rec); AB[0] = 0 */
   double src_amp[9];
   double src_f0, src_t0;
   double mms_k, mms_omega;
@@ -356,8 +360,11 @@ struct StageCtx
 {
   int parity;
   /* the dq this stage writes is never read: the last stage of a step of a
-   * scheme whose first stage has A = 0 (every 2N-storage scheme here) */
+   * scheme whose first stage has A = 0 (every 2N-storage scheme here), and
+   * every stage when dq is rebuilt from the states */
   int dq_dead;
+  /* the old dq is q[parity] - q[parity ^ 1], and a is A[s] / B[s-1] */
+  int rec;
   double a, b, dt, t;
   double g, sw, cw;
 };
@@ -376,6 +383,7 @@ __device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
   {
     c.parity = (int)(clk->stage & 1);
     c.dq_dead = 0;
+    c.rec = 0;
     c.a = 0;
     c.b = 0;
     c.dt = 0;
@@ -391,8 +399,9 @@ __device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
   const long long st = clk->stage;
   const int s = (int)(st % sd.nstages);
   c.parity = (int)(st & 1);
-  c.dq_dead = s == sd.nstages - 1 && sd.A[0] == 0.0;
-  c.a = sd.A[s];
+  c.rec = sd.dq_rec;
+  c.dq_dead = c.rec || (s == sd.nstages - 1 && sd.A[0] == 0.0);
+  c.a = c.rec ? sd.AB[s] : sd.A[s];
   c.b = sd.B[s];
   c.dt = clk->dt;
   c.t = clk->t + sd.C[s] * c.dt;

@@ -490,6 +490,16 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   }
   for (int i = 0; i < 9; ++i)
     h.C[i] = s->lsrk.C[i];
+  /* FP32 and FP64 keep no dq between stages: stage s >= 1 rebuilds it from
+   * the state it reads and the one before it, still in the other ping-pong
+   * buffer.  Per node a middle stage then moves three state-sized values
+   * (q in, q_old in, q out) instead of four (plus dq in and out) */
+  h.dq_rec = (prec == BFAMX_FP32 || prec == BFAMX_FP64) && h.A[0] == 0.0;
+  for (int i = 0; i + 1 < h.nstages; ++i)
+    if (h.B[i] == 0.0)
+      h.dq_rec = 0;
+  for (int i = 0; i < 8; ++i)
+    h.AB[i] = i > 0 && i < h.nstages && h.dq_rec ? h.A[i] / h.B[i - 1] : 0.0;
 
   const int epb = epb_of(physics, prec, N);
   s->ntasks = (K + epb - 1) / epb;
@@ -1845,8 +1855,11 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
       *choice = b->grouped;
     return BFAMX_OK;
   }
-  /* save dq */
-  std::vector<void *> saved(b->subs.size(), nullptr);
+  /* the trial launches repeat the current stage without advancing the
+   * clock: they rewrite the next state's buffer and dq.  Save dq, or, where
+   * dq is rebuilt from the states, the previous state that the next
+   * buffer holds until the stage ends */
+  std::vector<void *> saved(b->subs.size(), nullptr), reg(b->subs.size(), nullptr);
   std::vector<void *> allocs;
   size_t bytes = 0;
   int rc = BFAMX_OK;
@@ -1854,8 +1867,16 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   {
     bfamx_subdomain_t *s = b->subs[i];
     const size_t nb = (size_t)s->K * s->nf * s->Np * (s->rate_fp32 ? 4 : 8);
-    rc = dev_alloc(d, allocs, &bytes, nb, &saved[i]);
-    if (!rc && hipMemcpyAsync(saved[i], s->h.dq, nb, hipMemcpyDeviceToDevice, d->stream) !=
+    reg[i] = s->h.dq;
+    if (s->h.dq_rec)
+    {
+      int parity = 0;
+      rc = current_parity(s, &parity);
+      reg[i] = s->h.q[parity ^ 1];
+    }
+    if (!rc)
+      rc = dev_alloc(d, allocs, &bytes, nb, &saved[i]);
+    if (!rc && hipMemcpyAsync(saved[i], reg[i], nb, hipMemcpyDeviceToDevice, d->stream) !=
                    hipSuccess)
       rc = bfamx_fail(BFAMX_EHIP, "autotune: dq save failed");
   }
@@ -1920,7 +1941,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     {
       bfamx_subdomain_t *s = b->subs[i];
       const size_t nb = (size_t)s->K * s->nf * s->Np * (s->rate_fp32 ? 4 : 8);
-      if (hipMemcpyAsync(s->h.dq, saved[i], nb, hipMemcpyDeviceToDevice, d->stream) !=
+      if (hipMemcpyAsync(reg[i], saved[i], nb, hipMemcpyDeviceToDevice, d->stream) !=
               hipSuccess &&
           !rc)
         rc = bfamx_fail(BFAMX_EHIP, "autotune: dq restore failed");

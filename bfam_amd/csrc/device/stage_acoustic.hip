@@ -192,11 +192,26 @@ __device__ __forceinline__ void acoustic_task(const SD &sd, const Task &tk,
   gptr<TR> dqp = glob(static_cast<TR *>(sd.dq));
   const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
   const TS cb = (TS)cx.b;
+  /* A = 0 (first stage of a step): no dq read.  FP32, FP64: the old dq is
+   * rebuilt from the previous state, B_{s-1} dq_{s-1} = q_s - q_{s-1},
+   * read from the other ping-pong buffer before this thread overwrites the
+   * node (a holds A_s / B_{s-1}) */
+  const bool rd_dq = cx.a != 0.0;
+  constexpr bool REC = sizeof(TS) == sizeof(TR);
+  const bool rec = REC && cx.rec;
 #pragma unroll
   for (int f = 0; f < NF; ++f)
   {
     const size_t id = qidx(e, f, n, NF, Np);
-    const TR d = ca * dqp[id] + cdt * (TR)rate[f];
+    TR dqo = 0;
+    if (rd_dq)
+    {
+      if constexpr (REC)
+        dqo = rec ? (TR)(qs[f] - qout[id]) : dqp[id];
+      else
+        dqo = dqp[id];
+    }
+    const TR d = ca * dqo + cdt * (TR)rate[f];
     if (!cx.dq_dead)
       dqp[id] = d;
     qout[id] = qs[f] + cb * (TS)d;

@@ -16,7 +16,9 @@
  *   1. stream q[e] (NF*Np contiguous) from HBM and stage it in LDS: by
  *      LDS-DMA when LDS holds it exactly (FP32, FP64), through registers
  *      otherwise; the dq rows follow into LDS (FP32 rates) or registers
- *      (FP64), except on stages with A = 0, which do not read dq
+ *      (FP64), except on stages with A = 0, which do not read dq.  FP32 and
+ *      FP64 store no dq: they read the previous stage's state instead and
+ *      rebuild dq_{s-1} = (q_s - q_{s-1}) / B_{s-1}
  *   2. face tasks (one thread per face node): minus trace from LDS, plus trace
  *      (velocity and the stress column of the face's axis) gathered from the
  *      neighbour's q through nbr_elem/nbr_code (the on-device replacement of
@@ -24,8 +26,9 @@
  *   3. 18 directional derivatives per node from LDS (2(N+1) flop each),
  *      folded into the rates one axis at a time
  *   4. node owners lift the fluctuations of their faces (fixed face order)
- *   5. dq and q_out written (q ping-pongs so neighbours read the stage
- *      input); nothing but the loop state is held across a barrier
+ *   5. q_out (and dq, FP32 rates on FP64 state only) written: q ping-pongs
+ *      so that neighbours read the stage input; nothing but the loop state
+ *      is held across a barrier
  * The task, descriptor, clock and, for one-element blocks, the element
  * tables are read through the constant address space (scalar loads).
  */
@@ -155,11 +158,17 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   /* a 2N-storage scheme starts every step with A = 0 (dq = dt L): the old
    * dq is not read then */
   const bool rd_dq = upd && cx.a != 0.0;
+  /* FP32, FP64: the old dq is rebuilt from the state before this one,
+   * which the other ping-pong buffer still holds (read here, at this
+   * block's own nodes, before the same nodes are written below) */
+  constexpr bool REC = sizeof(TS) == sizeof(TR);
+  const bool rec = REC && cx.rec;
+  const void *dsrc = rec ? sd.q[cx.parity ^ 1] : sd.dq;
   if constexpr (DMA_DQ)
   {
     if (rd_dq)
       block_to_lds<DMA_WD, BS, CPOL_NT>(
-          (gptr<const float>)glob(static_cast<const float *>(sd.dq)) + (size_t)tk.e0 * NF * Np,
+          (gptr<const float>)glob(static_cast<const float *>(dsrc)) + (size_t)tk.e0 * NF * Np,
           reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
   }
   /* rates without room in LDS (FP64, and FP32 where a third block per CU
@@ -170,7 +179,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   {
     if (act && rd_dq)
     {
-      gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + (size_t)e * NF * Np + n;
+      gptr<const TR> dqe = glob(static_cast<const TR *>(dsrc)) + (size_t)e * NF * Np + n;
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         dqs[f] = __builtin_nontemporal_load(&dqe[f * Np]);
@@ -418,6 +427,10 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
         dqo = (TR)sdq[(le * NF + f) * Np + n];
       else
         dqo = dqs[f];
+      /* rebuilt: B_{s-1} dq_{s-1} = q_s - q_{s-1}; a holds A_s / B_{s-1} */
+      if constexpr (REC)
+        if (rec)
+          dqo = (TR)qv - dqo;
     }
     const TR d = ca * dqo + cdt * (TR)rate[f];
     if (!cx.dq_dead)

@@ -399,6 +399,15 @@ __device__ __forceinline__ void elastic_line_task(const SD &sd,
   const TR ca = (TR)cx.a, cdt = (TR)cx.dt;
   const TS cb = (TS)cx.b;
   constexpr int PD = sizeof(TS) == 4 ? NF : 3;
+  /* stages with A = 0 (the first of each step) read no dq.  FP32 and FP64
+   * store none: the old dq is rebuilt from the previous stage's state,
+   * still in the other ping-pong buffer, B_{s-1} dq_{s-1} = q_s - q_{s-1}
+   * (a holds A_s / B_{s-1}); these nodes are overwritten only below, by
+   * this lane, after the read */
+  const bool rd_dq = cx.a != 0.0;
+  constexpr bool REC = sizeof(TS) == sizeof(TR);
+  const bool rec = REC && cx.rec;
+  const void *dsrc = rec ? sd.q[cx.parity ^ 1] : sd.dq;
   TR dqv[PD][ROWS][VW];
   TS qv[q_from_lds ? 1 : PD][ROWS][VW];
   auto fetch = [&](int f) {
@@ -409,10 +418,10 @@ __device__ __forceinline__ void elastic_line_task(const SD &sd,
       if (!coal_pos(r, &lel, &n))
         continue;
       const size_t off = ((size_t)(tk.e0 + lel) * NF + f) * Np + n;
-      gptr<const TR> dqe = glob(static_cast<const TR *>(sd.dq)) + off;
+      gptr<const TR> dqe = glob(static_cast<const TR *>(dsrc)) + off;
 #pragma unroll
       for (int t = 0; t < VW; ++t)
-        dqv[f % PD][r][t] = dqe[t];
+        dqv[f % PD][r][t] = rd_dq ? dqe[t] : TR(0);
       if constexpr (!q_from_lds)
       {
 #pragma unroll
@@ -753,8 +762,12 @@ __device__ __forceinline__ void elastic_line_task(const SD &sd,
 #pragma unroll
       for (int t = 0; t < VW; ++t)
       {
-        const TR d = ca * dqv[f % PD][r][t] + cdt * (TR)rv[r][t];
         const TS qold = q_from_lds ? (TS)qo_l[r][t] : qv[q_from_lds ? 0 : f % PD][r][t];
+        TR dqo = dqv[f % PD][r][t];
+        if constexpr (REC)
+          if (rec && rd_dq)
+            dqo = (TR)qold - dqo;
+        const TR d = ca * dqo + cdt * (TR)rv[r][t];
         if (!cx.dq_dead)
           dqe[t] = d;
         qo[t] = qold + cb * (TS)d;

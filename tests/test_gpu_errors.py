@@ -1,0 +1,107 @@
+"""Error paths of the C ABI on a device: invalid uploads, calls out of order
+and the HBM budget.  Each failing call must return its status code, leave
+no device memory behind and leave the device usable for a valid upload.
+The reference aborts on the first two kinds (bfam.h:36-57) and on a budget
+overrun (bfamo_device_malloc, bfam.c:8561-8589); here each is a return
+code."""
+import numpy as np
+import pytest
+
+import bfam_amd as bx
+import bfam_amd.workloads as W
+from util import random_elastic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = bx.Device(0)
+    yield d
+    d.close()
+
+
+def upload_code(dev, cfg, **override):
+    m = cfg["mesh"]
+    args = dict(r=m.r, w=m.w, Dr=m.Dr, vmapM=m.vmapM, vmapP=m.vmapP, h=m.h,
+                rho=cfg["rho"], cp=cfg["cp"], cs=cfg.get("cs"), N=m.N)
+    args.update(override)
+    N = args.pop("N")
+    with pytest.raises(bx.BfamxError) as e:
+        bx.Subdomain(dev, cfg["physics"], bx.FP32, N, args["r"], args["w"], args["Dr"],
+                     args["vmapM"], args["vmapP"], args["h"], args["rho"], args["cp"],
+                     args["cs"], x0=m.x0)
+    return e.value.code
+
+
+def test_invalid_uploads_leave_nothing_behind(dev):
+    cfg, q0 = random_elastic((2, 2, 2), 4, seed=3, src=False)
+    used0, _ = dev.memory()
+    K = cfg["mesh"].K
+    cs0 = np.array(cfg["cs"])
+    cs0[K // 2] = 0.0  # elastic needs a shear speed
+    h_bad = np.array(cfg["mesh"].h)
+    h_bad[1, 0] = -h_bad[1, 0]
+    vP = np.array(cfg["mesh"].vmapP)
+    vP[5] = K * cfg["mesh"].Np  # one past the last node
+    vM = np.array(cfg["mesh"].vmapM)
+    vM[[0, 1]] = vM[[1, 0]]  # not the face mask
+    assert upload_code(dev, cfg, cs=cs0) == bx.EINVAL
+    assert upload_code(dev, cfg, h=h_bad) == bx.EINVAL
+    assert upload_code(dev, cfg, vmapP=vP) == bx.EINVAL
+    assert upload_code(dev, cfg, vmapM=vM) == bx.EINVAL
+    assert dev.memory()[0] == used0
+    # a valid upload on the same device still steps
+    sub = W.upload(dev, cfg, bx.FP32)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    sub.step(2)
+    assert np.all(np.isfinite(sub.download()))
+    sub.close()
+    assert dev.memory()[0] == used0
+
+
+@pytest.mark.parametrize("N", [1, 2, 8])
+def test_elastic_orders_outside_3_to_7_are_unsupported(dev, N):
+    mesh = W.Mesh(3, [2, 2, 2], N)
+    K = mesh.K
+    cfg = {"mesh": mesh, "rho": np.ones(K), "cp": np.full(K, 2.0), "cs": np.ones(K),
+           "physics": bx.ELASTIC3D}
+    used0, _ = dev.memory()
+    assert upload_code(dev, cfg) == bx.EUNSUPPORTED
+    assert dev.memory()[0] == used0
+
+
+def test_calls_before_set_fields(dev):
+    cfg, q0 = random_elastic((2, 2, 2), 3, seed=4, src=False)
+    sub = W.upload(dev, cfg, bx.FP64)
+    for call in (sub.step, sub.stage, sub.download, sub.rhs):
+        with pytest.raises(bx.BfamxError) as e:
+            call()
+        assert e.value.code == bx.ESTATE
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    sub.step(1)
+    sub.close()
+
+
+def test_budget_overrun_is_an_error_not_an_abort():
+    cfg, q0 = random_elastic((4, 4, 4), 7, seed=5, src=False)
+    need = bx.subdomain_bytes(3, 7, cfg["mesh"].K, bx.ELASTIC3D, bx.FP32)
+    small = bx.Device(0, budget_bytes=need // 2)
+    try:
+        used0, budget = small.memory()
+        assert budget == need // 2
+        with pytest.raises(bx.BfamxError) as e:
+            W.upload(small, cfg, bx.FP32)
+        assert e.value.code == bx.EBUDGET
+        assert small.memory()[0] == used0
+    finally:
+        small.close()
+    big = bx.Device(0, budget_bytes=2 * need + (64 << 20))
+    try:
+        sub = W.upload(big, cfg, bx.FP32)
+        sub.set_fields(q0, 0.0, cfg["dt"])
+        sub.step(1)
+        assert np.all(np.isfinite(sub.download()))
+        sub.close()
+    finally:
+        big.close()
