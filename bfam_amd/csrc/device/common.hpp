@@ -152,7 +152,7 @@ struct SubDev
    * the mortar kernel before each stage: dv[3], dT[3] per face node */
   void *glue_buf;
   double A[8], B[8], C[9];
-  double AB[8]; /* A[s] / B[s-1] (dq_rec); AB[0] = 0 */
+  double AB[8]; /* the stage kernels' A: A[s] / B[s-1] with dq_rec, else A[s] */
   double src_amp[9];
   double src_f0, src_t0;
   double mms_k, mms_omega;
@@ -360,13 +360,13 @@ struct StageCtx
 {
   int parity;
   /* the dq this stage writes is never read: the last stage of a step of a
-   * scheme whose first stage has A = 0 (every 2N-storage scheme here), and
-   * every stage when dq is rebuilt from the states */
+   * scheme whose first stage has A = 0 (every 2N-storage scheme here).
+   * Where dq is rebuilt from the states (rec), no stage stores it */
   int dq_dead;
-  /* the old dq is q[parity] - q[parity ^ 1], and a is A[s] / B[s-1] */
-  int rec;
   double a, b, dt, t;
   double g, sw, cw;
+  /* the old dq is q[parity] - q[parity ^ 1], and a is A[s] / B[s-1] */
+  int rec;
 };
 
 /* SD: SubDev, possibly qualified with the constant address space.  The
@@ -400,8 +400,8 @@ __device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
   const int s = (int)(st % sd.nstages);
   c.parity = (int)(st & 1);
   c.rec = sd.dq_rec;
-  c.dq_dead = c.rec || (s == sd.nstages - 1 && sd.A[0] == 0.0);
-  c.a = c.rec ? sd.AB[s] : sd.A[s];
+  c.dq_dead = s == sd.nstages - 1 && sd.A[0] == 0.0;
+  c.a = sd.AB[s];
   c.b = sd.B[s];
   c.dt = clk->dt;
   c.t = clk->t + sd.C[s] * c.dt;

@@ -499,7 +499,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     if (h.B[i] == 0.0)
       h.dq_rec = 0;
   for (int i = 0; i < 8; ++i)
-    h.AB[i] = i > 0 && i < h.nstages && h.dq_rec ? h.A[i] / h.B[i - 1] : 0.0;
+    h.AB[i] = !h.dq_rec ? h.A[i] : i > 0 && i < h.nstages ? h.A[i] / h.B[i - 1] : 0.0;
 
   const int epb = epb_of(physics, prec, N);
   s->ntasks = (K + epb - 1) / epb;

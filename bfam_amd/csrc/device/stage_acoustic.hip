@@ -212,7 +212,7 @@ __device__ __forceinline__ void acoustic_task(const SD &sd, const Task &tk,
         dqo = dqp[id];
     }
     const TR d = ca * dqo + cdt * (TR)rate[f];
-    if (!cx.dq_dead)
+    if (!cx.dq_dead && !rec)
       dqp[id] = d;
     qout[id] = qs[f] + cb * (TS)d;
   }

@@ -163,7 +163,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
    * block's own nodes, before the same nodes are written below) */
   constexpr bool REC = sizeof(TS) == sizeof(TR);
   const bool rec = REC && cx.rec;
-  const void *dsrc = rec ? sd.q[cx.parity ^ 1] : sd.dq;
+  const void *qprev = sd.q[cx.parity ^ 1];
+  const void *dsrc = rec ? qprev : sd.dq;
   if constexpr (DMA_DQ)
   {
     if (rd_dq)
@@ -433,7 +434,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
           dqo = (TR)qv - dqo;
     }
     const TR d = ca * dqo + cdt * (TR)rate[f];
-    if (!cx.dq_dead)
+    if (!cx.dq_dead && !rec)
       __builtin_nontemporal_store(d, &dqe[f * Np]);
     __builtin_nontemporal_store((TS)(qv + cb * (TS)d), &qo[f * Np]);
   }

@@ -407,7 +407,8 @@ __device__ __forceinline__ void elastic_line_task(const SD &sd,
   const bool rd_dq = cx.a != 0.0;
   constexpr bool REC = sizeof(TS) == sizeof(TR);
   const bool rec = REC && cx.rec;
-  const void *dsrc = rec ? sd.q[cx.parity ^ 1] : sd.dq;
+  const void *qprev = sd.q[cx.parity ^ 1];
+  const void *dsrc = rec ? qprev : sd.dq;
   TR dqv[PD][ROWS][VW];
   TS qv[q_from_lds ? 1 : PD][ROWS][VW];
   auto fetch = [&](int f) {
@@ -768,7 +769,7 @@ __device__ __forceinline__ void elastic_line_task(const SD &sd,
           if (rec && rd_dq)
             dqo = (TR)qold - dqo;
         const TR d = ca * dqo + cdt * (TR)rv[r][t];
-        if (!cx.dq_dead)
+        if (!cx.dq_dead && !rec)
           dqe[t] = d;
         qo[t] = qold + cb * (TS)d;
       }
