@@ -62,8 +62,7 @@ HBM_PEAK_GBS = 8000.0
 # step's first stage does not read it): (20 + 3 x 24 + 20) / 5.  Both stage
 # kernels move the same bytes.
 BYTES_PER_DOF = {"fp32": 11.2, "fp32_rhs": 22.4, "fp64": 22.4, "fp64_plain": 80}
-TRAFFIC_FILES = {"line": "traffic_elastic_line_fp32_48cube_N7.txt",
-                 "node-parallel": "traffic_elastic_stage_fp32_48cube_N7.txt"}
+TRAFFIC_FILES = {"node-parallel": "traffic_elastic_stage_fp32_48cube_N7.txt"}
 
 
 def log(msg):
@@ -85,7 +84,8 @@ def read_traffic(kernel):
     """HBM bytes per launch of the FP32 stage kernel (`kernel`: "line" or
     "node-parallel") from the committed counter summary: FETCH_SIZE and
     WRITE_SIZE (KiB per dispatch) of the kernel, and of a 1 GiB copy kernel,
-    which reads and writes exactly 2^30 bytes."""
+    which reads and writes exactly 2^30 bytes.  None for a kernel with no
+    committed summary."""
     try:
         kv = {}
         with open(os.path.join(ROOT, "profiles", TRAFFIC_FILES[kernel])) as f:
