@@ -1,0 +1,68 @@
+"""Register and scratch budgets of the fused elastic stage kernels, read from
+the compiler's resource report for gfx950 (CPU only: hipcc cross-compiles).
+
+The node-parallel kernel at N = 6, 7 runs 512-thread blocks, and its speed
+rests on how many fit on a CU: three FP32 blocks need at most 80 VGPRs, two
+FP32_RHS or FP64 blocks at most 128.  None of these kernels may touch
+scratch memory: per-lane scratch is written through L2 to HBM by every
+thread (a 24-byte spill of the stage context once added 0.9 GB of writes to
+each 48^3 FP32 stage)."""
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+
+from bfam_amd import build as B  # noqa: E402
+
+PREC = {"fff": "Prec<float, float, float>", "dff": "Prec<double, float, float>",
+        "ddd": "Prec<double, double, double>"}
+MANGLED = {"fff": "NS_4PrecIfffEE", "dff": "NS_4PrecIdffEE", "ddd": "NS_4PrecIdddEE"}
+
+
+@pytest.fixture(scope="module")
+def node_report(tmp_path_factory):
+    if shutil.which("hipcc") is None:
+        pytest.skip("hipcc not found")
+    out = tmp_path_factory.mktemp("ru") / "stage_elastic.o"
+    src = os.path.join(B.CSRC, "device", "stage_elastic.hip")
+    p = subprocess.run(["hipcc"] + B.HIPFLAGS + B.INCS +
+                       ["-c", src, "-o", str(out), "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-2000:]
+    rep = {}
+    for blk in p.stderr.split("Function Name: ")[1:]:
+        name = blk.split()[0]
+
+        def get(key):
+            return int(re.search(key + r": (\d+)", blk).group(1))
+
+        rep[name] = {"vgpr": get("VGPRs"), "scratch": get(r"ScratchSize \[bytes/lane\]")}
+    return rep
+
+
+def node_kernel(rep, N, epb, prec):
+    key = f"elastic_stage_kernelILi{N}ELi{epb}E{MANGLED[prec]}"
+    hits = [v for k, v in rep.items() if key in k]
+    assert len(hits) == 1, key
+    return hits[0]
+
+
+@pytest.mark.parametrize("N", [6, 7])
+@pytest.mark.parametrize("prec,max_vgpr", [("fff", 80), ("dff", 128), ("ddd", 128)])
+def test_node_kernel_fits_its_blocks_without_scratch(node_report, N, prec, max_vgpr):
+    r = node_kernel(node_report, N, 1, prec)
+    assert r["scratch"] == 0, (N, PREC[prec], r)
+    assert r["vgpr"] <= max_vgpr, (N, PREC[prec], r)
+
+
+@pytest.mark.parametrize("N,epb", [(3, 8), (4, 4), (5, 2)])
+def test_node_kernel_fp32_no_scratch(node_report, N, epb):
+    for prec in ("fff", "dff"):
+        assert node_kernel(node_report, N, epb, prec)["scratch"] == 0, (N, PREC[prec])
