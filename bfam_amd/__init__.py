@@ -549,7 +549,8 @@ class Glue:
 # ------------------------------------------------------------------- output
 
 def vtk_write(path, dim, N, x0, h, r, fields, time=0.0, binary=True, Np_write=0):
-    """fields: dict name -> (K*(N+1)^dim,) array in bfam node order."""
+    """fields: dict name -> (K*(N+1)^dim,) array in bfam node order.
+    binary: 0 ascii, 1 (True) base64 binary, 2 zlib-compressed binary."""
     x0 = np.ascontiguousarray(x0, dtype=np.float64).ravel()
     h = np.ascontiguousarray(h, dtype=np.float64).ravel()
     K = x0.size // dim

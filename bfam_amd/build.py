@@ -59,7 +59,7 @@ def build(jobs=None, verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -8,28 +8,28 @@
  * ("poor man's high-order viz"), every element split into N_vtk^dim linear
  * cells (quads / hexahedra), a "time" cell array, and one point array per
  * field.  Binary output is VTK's inline base64 with a 64-bit byte-count
- * header; zlib compression (the reference's sc_vtk_write_compressed) is not
- * offered.  Affine elements: node coordinates come from each element's lower
- * corner and edge lengths.
+ * header; compressed output (binary = 2, the reference's
+ * compressor="vtkZLibDataCompressor" files written through
+ * sc_vtk_write_compressed, bfam.c:5931-5938, 8278-8281) splits each array
+ * into 32 KiB blocks deflated by zlib, after a header of 64-bit words:
+ * block count, block size, size of the last block, then each block's
+ * compressed size.  The header and the blocks are base64-encoded
+ * separately, as VTK reads them.  Affine elements: node coordinates come
+ * from each element's lower corner and edge lengths.
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <zlib.h>
 
 #include "bfamx_internal.h"
 
 static const char b64[] =
     "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
 
-/* base64 of [uint64 nbytes][data], as VTK reads "binary" inline arrays
- * with header_type="UInt64" */
-static void write_base64(FILE *fp, const void *data, size_t nbytes)
+/* base64 of n bytes, padded to a multiple of 4 characters */
+static void put_base64(FILE *fp, const unsigned char *buf, size_t total)
 {
-  const uint64_t hdr = nbytes;
-  const size_t total = 8 + nbytes;
-  unsigned char *buf = malloc(total);
-  memcpy(buf, &hdr, 8);
-  memcpy(buf + 8, data, nbytes);
   for (size_t i = 0; i < total; i += 3)
   {
     const unsigned a = buf[i];
@@ -41,47 +41,113 @@ static void write_base64(FILE *fp, const void *data, size_t nbytes)
     fputc(i + 1 < total ? b64[(v >> 6) & 63] : '=', fp);
     fputc(i + 2 < total ? b64[v & 63] : '=', fp);
   }
-  fputc('\n', fp);
-  free(buf);
 }
 
-static void write_f64(FILE *fp, const char *name, int ncomp, const double *v, size_t n,
-                      int binary)
+enum
 {
+  ZBLOCK = 1 << 15 /* uncompressed bytes per compressed block */
+};
+
+/* base64 of [uint64 nbytes][data], as VTK reads "binary" inline arrays
+ * with header_type="UInt64"; with compress, the zlib block layout above.
+ * Returns 0, or -1 if memory or zlib fails. */
+static int write_base64(FILE *fp, const void *data, size_t nbytes, int compress)
+{
+  if (!compress)
+  {
+    const uint64_t hdr = nbytes;
+    unsigned char *buf = malloc(8 + nbytes);
+    if (!buf)
+      return -1;
+    memcpy(buf, &hdr, 8);
+    memcpy(buf + 8, data, nbytes);
+    put_base64(fp, buf, 8 + nbytes);
+    fputc('\n', fp);
+    free(buf);
+    return 0;
+  }
+  const size_t nblk = (nbytes + ZBLOCK - 1) / ZBLOCK;
+  const size_t last = nbytes - (nblk ? (nblk - 1) * (size_t)ZBLOCK : 0);
+  uint64_t *hdr = malloc(sizeof(uint64_t) * (3 + nblk));
+  const uLong bound = compressBound(ZBLOCK);
+  unsigned char *z = malloc(nblk * (size_t)bound + 1);
+  if (!hdr || !z)
+  {
+    free(hdr);
+    free(z);
+    return -1;
+  }
+  hdr[0] = nblk;
+  hdr[1] = ZBLOCK;
+  hdr[2] = last;
+  size_t zt = 0;
+  for (size_t k = 0; k < nblk; ++k)
+  {
+    const size_t n = k + 1 < nblk ? (size_t)ZBLOCK : last;
+    uLongf zn = bound;
+    if (compress2(z + zt, &zn, (const Bytef *)data + k * (size_t)ZBLOCK, (uLong)n,
+                  Z_BEST_COMPRESSION) != Z_OK)
+    {
+      free(hdr);
+      free(z);
+      return -1;
+    }
+    hdr[3 + k] = zn;
+    zt += zn;
+  }
+  put_base64(fp, (const unsigned char *)hdr, sizeof(uint64_t) * (3 + nblk));
+  put_base64(fp, z, zt);
+  fputc('\n', fp);
+  free(hdr);
+  free(z);
+  return 0;
+}
+
+/* binary: 0 ascii, 1 base64, 2 zlib-compressed base64.  Each returns 0, or
+ * -1 if compression fails. */
+static int write_f64(FILE *fp, const char *name, int ncomp, const double *v, size_t n,
+                     int binary)
+{
+  int rc = 0;
   fprintf(fp,
           "        <DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"%d\" "
           "format=\"%s\">\n",
           name, ncomp, binary ? "binary" : "ascii");
   if (binary)
-    write_base64(fp, v, sizeof(double) * n * (size_t)ncomp);
+    rc = write_base64(fp, v, sizeof(double) * n * (size_t)ncomp, binary == 2);
   else
     for (size_t i = 0; i < n * (size_t)ncomp; ++i)
       fprintf(fp, "%.17g\n", v[i]);
   fprintf(fp, "        </DataArray>\n");
+  return rc;
 }
 
-static void write_i64(FILE *fp, const char *name, const int64_t *v, size_t n, int binary)
+static int write_i64(FILE *fp, const char *name, const int64_t *v, size_t n, int binary)
 {
+  int rc = 0;
   fprintf(fp, "        <DataArray type=\"Int64\" Name=\"%s\" format=\"%s\">\n", name,
           binary ? "binary" : "ascii");
   if (binary)
-    write_base64(fp, v, sizeof(int64_t) * n);
+    rc = write_base64(fp, v, sizeof(int64_t) * n, binary == 2);
   else
     for (size_t i = 0; i < n; ++i)
       fprintf(fp, "%lld\n", (long long)v[i]);
   fprintf(fp, "        </DataArray>\n");
+  return rc;
 }
 
-static void write_u8(FILE *fp, const char *name, const uint8_t *v, size_t n, int binary)
+static int write_u8(FILE *fp, const char *name, const uint8_t *v, size_t n, int binary)
 {
+  int rc = 0;
   fprintf(fp, "        <DataArray type=\"UInt8\" Name=\"%s\" format=\"%s\">\n", name,
           binary ? "binary" : "ascii");
   if (binary)
-    write_base64(fp, v, n);
+    rc = write_base64(fp, v, n, binary == 2);
   else
     for (size_t i = 0; i < n; ++i)
       fprintf(fp, "%u\n", (unsigned)v[i]);
   fprintf(fp, "        </DataArray>\n");
+  return rc;
 }
 
 int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *elem_x0,
@@ -90,7 +156,8 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *e
                     int binary, int Np_write)
 {
   if (!path || (dim != 2 && dim != 3) || N < 1 || K < 0 || !elem_x0 || !elem_h || !r ||
-      nfields < 0 || (nfields && (!names || !fields)) || Np_write == 1 || Np_write < 0)
+      nfields < 0 || (nfields && (!names || !fields)) || Np_write == 1 || Np_write < 0 ||
+      binary < 0 || binary > 2)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: bad arguments");
   const int n1 = N + 1;
   const int nv = Np_write > 0 ? Np_write : n1; /* points per direction */
@@ -164,19 +231,21 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *e
   fprintf(fp, "<?xml version=\"1.0\"?>\n");
   fprintf(fp,
           "<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
-          "header_type=\"UInt64\">\n");
+          "header_type=\"UInt64\"%s>\n",
+          binary == 2 ? " compressor=\"vtkZLibDataCompressor\"" : "");
+  int zerr = 0;
   fprintf(fp, "  <UnstructuredGrid>\n");
   fprintf(fp, "    <Piece NumberOfPoints=\"%zu\" NumberOfCells=\"%zu\">\n", npts, ncells);
   fprintf(fp, "      <Points>\n");
-  write_f64(fp, "Position", 3, xyz, npts, binary);
+  zerr |= write_f64(fp, "Position", 3, xyz, npts, binary);
   fprintf(fp, "      </Points>\n");
   fprintf(fp, "      <Cells>\n");
-  write_i64(fp, "connectivity", conn, (size_t)nvert * ncells, binary);
-  write_i64(fp, "offsets", offs, ncells, binary);
-  write_u8(fp, "types", type, ncells, binary);
+  zerr |= write_i64(fp, "connectivity", conn, (size_t)nvert * ncells, binary);
+  zerr |= write_i64(fp, "offsets", offs, ncells, binary);
+  zerr |= write_u8(fp, "types", type, ncells, binary);
   fprintf(fp, "      </Cells>\n");
   fprintf(fp, "      <CellData Scalars=\"time\">\n");
-  write_f64(fp, "time", 1, tcell, ncells, binary);
+  zerr |= write_f64(fp, "time", 1, tcell, ncells, binary);
   fprintf(fp, "      </CellData>\n");
   fprintf(fp, "      <PointData>\n");
   {
@@ -222,14 +291,16 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *e
               o[a + (size_t)nv * (b + (size_t)nv * c3)] = acc;
             }
       }
-      write_f64(fp, names[f], 1, val, npts, binary);
+      zerr |= write_f64(fp, names[f], 1, val, npts, binary);
     }
     free(t1);
     free(t2);
   }
   fprintf(fp, "      </PointData>\n");
   fprintf(fp, "    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n");
-  if (ferror(fp))
+  if (zerr)
+    rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: compression failed for %s", path);
+  else if (ferror(fp))
     rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: write error on %s", path);
 done:
   if (fp && fclose(fp) && !rc)

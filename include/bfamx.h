@@ -394,8 +394,10 @@ int bfamx_batch_set_glue(bfamx_batch_t *b, bfamx_glue_t *g);
  * One subdomain's nodal fields (host arrays of K*(N+1)^dim doubles, bfam
  * node order) to a VTK XML unstructured-grid file: every element split into
  * linear cells through Np_write equispaced points per direction (0: the LGL
- * nodes), a "time" cell array and one point array per field; ascii or
- * base64 binary.  The per-subdomain piece of bfam_vtk_write_file
+ * nodes), a "time" cell array and one point array per field.  binary: 0
+ * ascii, 1 base64 binary, 2 base64 binary compressed with zlib in 32 KiB
+ * blocks (compressor="vtkZLibDataCompressor", as the reference writes with
+ * its compress flag).  The per-subdomain piece of bfam_vtk_write_file
  * (bfam.c:6050-6572, 8234-8326) for affine elements.
  */
 int bfamx_vtk_write(const char *path, int dim, int N, int32_t K,

@@ -51,7 +51,7 @@ int main(int argc, char **argv)
       double *f0 = calloc((size_t)K * Np, sizeof(double));
       const char *names[1] = {"p"};
       const double *fields[1] = {f0};
-      for (int bin = 0; bin < 2; ++bin)
+      for (int bin = 0; bin < 3; ++bin)
         for (int nw = 0; nw <= 4; nw += 2)
           if (bfamx_vtk_write(vtk_path, dim, N, K, x0, h, rr, 1, names, fields, 0.5, bin, nw))
           {

@@ -21,7 +21,7 @@ def test_host_code_clean_under_sanitizers(tmp_path):
            "-fno-omit-frame-pointer", "-std=gnu99",
            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "bfam_amd", "csrc"),
            os.path.join(HERE, "host_sanitize.c")] + [os.path.join(HOST, s) for s in SOURCES] + \
-          ["-lm", "-o", str(exe)]
+          ["-lm", "-lz", "-o", str(exe)]
     b = subprocess.run(cmd, capture_output=True, text=True)
     if b.returncode != 0 and "asan" in (b.stderr or "").lower():
         pytest.skip("AddressSanitizer runtime not available: " + b.stderr[-300:])

@@ -1,9 +1,10 @@
 """VTK output of nodal fields: structure of the .vtu file, exact resampling
-of polynomials of degree <= N to equispaced points, and ascii / base64
-binary agreement.  CPU only."""
+of polynomials of degree <= N to equispaced points, and agreement of the
+ascii, base64 binary and zlib-compressed encodings.  CPU only."""
 import base64
 import struct
 import xml.etree.ElementTree as ET
+import zlib
 
 import numpy as np
 import pytest
@@ -12,22 +13,45 @@ import bfam_amd as bx
 import bfam_amd.workloads as W
 
 
-def read_array(da):
+def b64len(nbytes):
+    return 4 * ((nbytes + 2) // 3)
+
+
+def read_array(da, compressed=False):
+    """One DataArray as VTK's XML reader decodes it (header_type UInt64):
+    base64 of [byte count][data], or, compressed, the base64 of the header
+    [blocks, block size, last block size, compressed sizes...] followed by
+    the base64 of the deflated blocks."""
     fmt = da.get("format")
     typ = da.get("type")
     if fmt == "ascii":
         return np.array(da.text.split(), dtype=float)
-    raw = base64.b64decode("".join(da.text.split()))
-    n = struct.unpack("<Q", raw[:8])[0]
-    body = raw[8:8 + n]
+    text = "".join(da.text.split())
     dt = {"Float64": "<f8", "Int64": "<i8", "UInt8": "u1"}[typ]
-    return np.frombuffer(body, dtype=dt).astype(float)
+    if not compressed:
+        raw = base64.b64decode(text)
+        n = struct.unpack("<Q", raw[:8])[0]
+        return np.frombuffer(raw[8:8 + n], dtype=dt).astype(float)
+    nblk = struct.unpack("<Q", base64.b64decode(text[:b64len(8)])[:8])[0]
+    hl = b64len(8 * (3 + nblk))
+    hdr = struct.unpack(f"<{3 + nblk}Q", base64.b64decode(text[:hl]))
+    assert hdr[0] == nblk and hdr[1] == 1 << 15
+    data = base64.b64decode(text[hl:])
+    assert len(data) == sum(hdr[3:])
+    out, off = [], 0
+    for k in range(nblk):
+        blk = zlib.decompress(data[off:off + hdr[3 + k]])
+        off += hdr[3 + k]
+        assert len(blk) == (hdr[2] if k == nblk - 1 else hdr[1])
+        out.append(blk)
+    return np.frombuffer(b"".join(out), dtype=dt).astype(float)
 
 
 def arrays(path):
     root = ET.parse(path).getroot()
+    compressed = root.get("compressor") == "vtkZLibDataCompressor"
     piece = root.find("UnstructuredGrid/Piece")
-    out = {da.get("Name"): read_array(da) for da in piece.iter("DataArray")}
+    out = {da.get("Name"): read_array(da, compressed) for da in piece.iter("DataArray")}
     return piece, out
 
 
@@ -39,7 +63,7 @@ def test_vtk_roundtrip(tmp_path, dim, Np_write):
     X = mesh.coords()
     f = X[0] + 2 * X[1] ** 2 + (X[2] ** 3 if dim == 3 else 0)  # degree <= N
     res = {}
-    for binary in (0, 1):
+    for binary in (0, 1, 2):
         p = tmp_path / f"t{binary}.vtu"
         bx.vtk_write(p, dim, N, mesh.x0, mesh.h, mesh.r, {"f": f, "g": 2 * f}, time=1.5,
                      binary=binary, Np_write=Np_write)
@@ -65,3 +89,36 @@ def test_vtk_roundtrip(tmp_path, dim, Np_write):
         res[binary] = a
     for k in res[0]:
         assert np.allclose(res[0][k], res[1][k], rtol=1e-15, atol=0)
+        assert np.array_equal(res[1][k], res[2][k])
+
+
+def test_vtk_compressed_blocks(tmp_path):
+    """Arrays larger than one 32 KiB block: several deflated blocks, a
+    partial last one, the same values as the uncompressed file, and a
+    smaller file for smooth data."""
+    N = 3
+    mesh = W.Mesh(3, [8, 8, 8], N)
+    X = mesh.coords()
+    f = np.sin(2 * X[0]) * np.cos(X[1]) + X[2]
+    out = {}
+    for binary in (1, 2):
+        p = tmp_path / f"b{binary}.vtu"
+        bx.vtk_write(p, 3, N, mesh.x0, mesh.h, mesh.r, {"f": f}, binary=binary)
+        out[binary] = (p.stat().st_size, arrays(p)[1])
+    for k in out[1][1]:
+        assert np.array_equal(out[1][1][k], out[2][1][k]), k
+    root = ET.parse(tmp_path / "b2.vtu").getroot()
+    pos = next(da for da in root.iter("DataArray") if da.get("Name") == "Position")
+    text = "".join(pos.text.split())
+    hdr0 = struct.unpack("<3Q", base64.b64decode(text[:b64len(24)])[:24])
+    nbytes = 8 * 3 * mesh.K * (N + 1) ** 3
+    assert hdr0[0] == -(-nbytes // (1 << 15)) and hdr0[0] > 1
+    assert hdr0[2] == nbytes - (hdr0[0] - 1) * (1 << 15)
+    assert out[2][0] < out[1][0]
+
+
+def test_vtk_rejects_unknown_encoding(tmp_path):
+    mesh = W.Mesh(2, [2, 2], 2)
+    with pytest.raises(bx.BfamxError) as e:
+        bx.vtk_write(tmp_path / "x.vtu", 2, 2, mesh.x0, mesh.h, mesh.r, {}, binary=3)
+    assert e.value.code == bx.EINVAL
