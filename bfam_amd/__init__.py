@@ -159,6 +159,7 @@ SIGNATURES = {
     "bfamx_batch_rhs": (C.c_int, [_vp, C.c_double, _vp]),
     "bfamx_vtk_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int32, _dp, _dp, _dp,
                                   C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_int]),
+    "bfamx_vtk_write_index": (C.c_int, [C.c_char_p, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     "bfamx_serve_generate": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(Request)]),
@@ -561,6 +562,15 @@ def vtk_write(path, dim, N, x0, h, r, fields, time=0.0, binary=True, Np_write=0)
                                  np.ascontiguousarray(r, dtype=np.float64), len(names),
                                  C.cast(cn, C.c_void_p), _ptrs(arrs) if arrs else None,
                                  float(time), int(binary), int(Np_write)))
+
+
+def vtk_write_index(path, pieces, names, binary=True):
+    """.pvtu index over .vtu files (paths as they appear in the index)
+    written by vtk_write with fields `names` and the same `binary`."""
+    pc = (C.c_char_p * max(len(pieces), 1))(*[str(p).encode() for p in pieces])
+    cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    _check(lib().bfamx_vtk_write_index(str(path).encode(), len(pieces), C.cast(pc, C.c_void_p),
+                                       len(names), C.cast(cn, C.c_void_p), int(binary)))
 
 
 # ------------------------------------------------------------- adapted mesh

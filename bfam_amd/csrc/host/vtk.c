@@ -315,3 +315,50 @@ done:
   free(val);
   return rc;
 }
+
+int bfamx_vtk_write_index(const char *path, int npieces, const char *const *piece_paths,
+                          int nfields, const char *const *names, int binary)
+{
+  if (!path || npieces < 0 || (npieces && !piece_paths) || nfields < 0 ||
+      (nfields && !names) || binary < 0 || binary > 2)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: bad arguments");
+  for (int i = 0; i < npieces; ++i)
+    if (!piece_paths[i])
+      return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: piece %d is NULL", i);
+  for (int f = 0; f < nfields; ++f)
+    if (!names[f])
+      return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: name %d is NULL", f);
+  FILE *fp = fopen(path, "w");
+  if (!fp)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: cannot write %s", path);
+  const char *fmt = binary ? "binary" : "ascii";
+  fprintf(fp, "<?xml version=\"1.0\"?>\n");
+  fprintf(fp,
+          "<VTKFile type=\"PUnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
+          "header_type=\"UInt64\"%s>\n",
+          binary == 2 ? " compressor=\"vtkZLibDataCompressor\"" : "");
+  fprintf(fp, "  <PUnstructuredGrid GhostLevel=\"0\">\n");
+  fprintf(fp, "    <PPoints>\n");
+  fprintf(fp,
+          "      <PDataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" "
+          "format=\"%s\"/>\n",
+          fmt);
+  fprintf(fp, "    </PPoints>\n");
+  fprintf(fp, "    <PCellData Scalars=\"time\">\n");
+  fprintf(fp, "      <PDataArray type=\"Float64\" Name=\"time\" format=\"%s\"/>\n", fmt);
+  fprintf(fp, "    </PCellData>\n");
+  fprintf(fp, "    <PPointData>\n");
+  for (int f = 0; f < nfields; ++f)
+    fprintf(fp, "      <PDataArray type=\"Float64\" Name=\"%s\" format=\"%s\"/>\n",
+            names[f], fmt);
+  fprintf(fp, "    </PPointData>\n");
+  for (int i = 0; i < npieces; ++i)
+    fprintf(fp, "    <Piece Source=\"%s\"/>\n", piece_paths[i]);
+  fprintf(fp, "  </PUnstructuredGrid>\n</VTKFile>\n");
+  int rc = BFAMX_OK;
+  if (ferror(fp))
+    rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: write error on %s", path);
+  if (fclose(fp) && !rc)
+    rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: cannot close %s", path);
+  return rc;
+}

@@ -405,6 +405,16 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K,
                     int nfields, const char *const *names,
                     const double *const *fields, double time, int binary,
                     int Np_write);
+/*
+ * The index (.pvtu) over npieces .vtu files written by bfamx_vtk_write with
+ * the same field names and encoding (binary as there), so that a viewer
+ * opens all subdomains as one dataset: the PUnstructuredGrid master file of
+ * bfam_vtk_write_file_pvtu (bfam.c:8066-8170), one Piece per file.
+ * piece_paths are written as given (relative to the index's directory).
+ */
+int bfamx_vtk_write_index(const char *path, int npieces,
+                          const char *const *piece_paths, int nfields,
+                          const char *const *names, int binary);
 
 /* ---------------------------------------------------------------------- */
 /* serving: independent elastic solves admitted under an HBM budget as they */

@@ -58,6 +58,16 @@ int main(int argc, char **argv)
             printf("vtk\n");
             return 1;
           }
+      {
+        char idx[4096];
+        snprintf(idx, sizeof idx, "%s.pvtu", vtk_path);
+        const char *pieces[2] = {vtk_path, vtk_path};
+        if (bfamx_vtk_write_index(idx, 2, pieces, 1, names, 2))
+        {
+          printf("vtk index\n");
+          return 1;
+        }
+      }
       free(f0); free(ne); free(nc); free(vM); free(vP); free(EToE); free(lex); free(EToF); free(x0); free(h);
     }
   /* octree */

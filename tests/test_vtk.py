@@ -122,3 +122,40 @@ def test_vtk_rejects_unknown_encoding(tmp_path):
     with pytest.raises(bx.BfamxError) as e:
         bx.vtk_write(tmp_path / "x.vtu", 2, 2, mesh.x0, mesh.h, mesh.r, {}, binary=3)
     assert e.value.code == bx.EINVAL
+
+
+@pytest.mark.parametrize("binary", [0, 2])
+def test_vtk_index_over_subdomain_pieces(tmp_path, binary):
+    """A .pvtu index over two subdomain pieces: one Piece per file, in
+    order, and the point, cell and field arrays that each piece holds, with
+    the pieces' type, components and encoding."""
+    pieces = []
+    for s, N in enumerate((2, 3)):
+        mesh = W.Mesh(3, [2, 2, 1], N)
+        X = mesh.coords()
+        name = f"sub_{s:05d}.vtu"
+        bx.vtk_write(tmp_path / name, 3, N, mesh.x0, mesh.h, mesh.r,
+                     {"v1": X[0], "S11": X[1] * X[2]}, time=0.25, binary=binary)
+        pieces.append(name)
+    bx.vtk_write_index(tmp_path / "sub.pvtu", pieces, ["v1", "S11"], binary=binary)
+    root = ET.parse(tmp_path / "sub.pvtu").getroot()
+    assert root.get("type") == "PUnstructuredGrid"
+    assert (root.get("compressor") == "vtkZLibDataCompressor") == (binary == 2)
+    grid = root.find("PUnstructuredGrid")
+    assert [p.get("Source") for p in grid.findall("Piece")] == pieces
+    declared = {sec: [(d.get("Name"), d.get("type"), d.get("NumberOfComponents", "1"),
+                       d.get("format")) for d in grid.find(sec).findall("PDataArray")]
+                for sec in ("PPoints", "PCellData", "PPointData")}
+    for name in pieces:
+        piece = ET.parse(tmp_path / name).getroot().find("UnstructuredGrid/Piece")
+        for psec, sec in (("PPoints", "Points"), ("PCellData", "CellData"),
+                          ("PPointData", "PointData")):
+            have = [(d.get("Name"), d.get("type"), d.get("NumberOfComponents", "1"),
+                     d.get("format")) for d in piece.find(sec).findall("DataArray")]
+            assert have == declared[psec], (name, sec)
+
+
+def test_vtk_index_rejects_bad_arguments(tmp_path):
+    with pytest.raises(bx.BfamxError) as e:
+        bx.vtk_write_index(tmp_path / "x.pvtu", ["a.vtu"], ["f"], binary=5)
+    assert e.value.code == bx.EINVAL
