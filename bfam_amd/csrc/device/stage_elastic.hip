@@ -150,10 +150,11 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
       sq[(le * NF + f) * Np + n] = (TC)v;
     }
   }
-  /* the dq rows load into LDS under the face and volume work.  Only this
-   * block reads them, and the outputs are written non-temporally: L2 then
-   * keeps the states that neighbouring blocks gather their face traces
-   * from (FP32 48^3 N=7: 1.635 -> 1.548 ms per stage) */
+  /* the dq (or previous-state) rows load into LDS under the face and
+   * volume work.  Only this block reads them, and the outputs are written
+   * non-temporally: L2 then keeps the states that neighbouring blocks
+   * gather their face traces from (FP32 48^3 N=7: 1.635 -> 1.548 ms per
+   * stage) */
   const bool upd = !(mode == MODE_RHS || PART != PART_FUSED);
   /* a 2N-storage scheme starts every step with A = 0 (dq = dt L): the old
    * dq is not read then */
@@ -434,6 +435,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
           dqo = (TR)qv - dqo;
     }
     const TR d = ca * dqo + cdt * (TR)rate[f];
+    /* stored only where the next stage reads it and cannot rebuild it */
     if (!cx.dq_dead && !rec)
       __builtin_nontemporal_store(d, &dqe[f * Np]);
     __builtin_nontemporal_store((TS)(qv + cb * (TS)d), &qo[f * Np]);
