@@ -338,10 +338,11 @@ extern "C" size_t bfamx_subdomain_bytes(int dim, int N, int32_t K,
   auto add = [&](size_t x) { b += ((x ? x : 16) + rnd) & ~rnd; };
   switch (prec)
   {
-  case BFAMX_FP64: add(8 * ndof), add(8 * ndof), add(8 * ndof); break;
+  /* FP64 and FP32 keep no dq (it is rebuilt from the two states) */
+  case BFAMX_FP64: add(8 * ndof), add(8 * ndof); break;
   case BFAMX_FP64_PLAIN: add(8 * ndof), add(8 * ndof), add(8 * ndof); break;
   case BFAMX_FP32_RHS: add(8 * ndof), add(8 * ndof), add(4 * ndof); break;
-  default: add(4 * ndof), add(4 * ndof), add(4 * ndof); break;
+  default: add(4 * ndof), add(4 * ndof); break;
   }
   const size_t nfaces = 2 * dim;
   add(4 * K * nfaces), add(K * nfaces);
@@ -350,9 +351,15 @@ extern "C" size_t bfamx_subdomain_bytes(int dim, int N, int32_t K,
   add(8 * K * nfaces * FMAT_STRIDE), add(4 * K * nfaces * FMAT_STRIDE);
   add(8 * n1 * n1), add(4 * n1 * n1), add(8 * n1), add(8 * n1);
   add(sizeof(Clock)), add(sizeof(SubDev));
-  const int epb = dim == 2 ? acoustic_epb(N) : elastic_epb(N) < elastic_line_epw(N) ? elastic_epb(N) : elastic_line_epw(N);
-  add(sizeof(Task) * ((K + epb - 1) / epb)), add(sizeof(int));
-  add(16 * (size_t)K);
+  /* the task lists of both stage kernels, the task count, the partial
+   * sums of the reductions: as bfamx_subdomain_upload allocates them */
+  const int epb = epb_of(ph, prec, N);
+  add(sizeof(Task) * ((K + epb - 1) / epb));
+  const int epn = elastic_epb(N);
+  const int has_node = ph == BFAMX_ELASTIC3D && prec != BFAMX_FP64_PLAIN;
+  add(sizeof(Task) * (has_node ? (K + epn - 1) / epn : 1));
+  add(sizeof(int));
+  add(32 * (size_t)K);
   return b;
 }
 
@@ -542,7 +549,10 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   }
   else
     ALLOC(h.q[1], ssz * ndof);
-  ALLOC(h.dq, rsz * ndof);
+  /* no dq buffer where stages rebuild it from the states (h.dq stays
+   * NULL: no kernel reads or writes it then) */
+  if (!h.dq_rec)
+    ALLOC(h.dq, rsz * ndof);
   {
     int32_t *ne;
     uint8_t *nc;
@@ -788,7 +798,8 @@ extern "C" int bfamx_set_fields(bfamx_subdomain_t *s,
           buf[(e * nf + f) * Np + n] = fields[f][e * Np + n];
     HIPCHK(hipMemcpy(s->h.q[0], buf.data(), 8 * ndof, hipMemcpyHostToDevice));
   }
-  HIPCHK(hipMemset(s->h.dq, 0, (s->rate_fp32 ? 4 : 8) * ndof));
+  if (s->h.dq)
+    HIPCHK(hipMemset(s->h.dq, 0, (s->rate_fp32 ? 4 : 8) * ndof));
   Clock c;
   memset(&c, 0, sizeof(c));
   c.t = t0;

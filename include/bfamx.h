@@ -218,7 +218,9 @@ int bfamx_subdomain_upload(bfamx_device_t *dev, const bfamx_dgx_t *sub,
 int bfamx_subdomain_free(bfamx_subdomain_t *sub);
 int bfamx_subdomain_info(const bfamx_subdomain_t *sub, int *nfields,
                          size_t *device_bytes);
-/* device bytes bfamx_subdomain_upload would allocate (admission control) */
+/* device bytes bfamx_subdomain_upload would allocate (admission control),
+ * for the LSRK tables of bfamx_lsrk_init: with these FP32 and FP64 hold two
+ * states and no dq */
 size_t bfamx_subdomain_bytes(int dim, int N, int32_t K, bfamx_physics_t ph,
                              bfamx_precision_t prec);
 

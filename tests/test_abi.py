@@ -62,5 +62,7 @@ def test_subdomain_bytes_budget_formula():
     K = 4096
     b64 = bx.subdomain_bytes(3, 7, K, bx.ELASTIC3D, bx.FP64)
     b32 = bx.subdomain_bytes(3, 7, K, bx.ELASTIC3D, bx.FP32)
-    state64 = 9 * K * 512 * 8 * 3  # two state buffers and dq
-    assert b64 >= state64 and b32 < 0.6 * b64
+    state64 = 9 * K * 512 * 8 * 2  # two state buffers; dq is rebuilt, not stored
+    assert state64 <= b64 < 1.1 * state64 and b32 < 0.6 * b64
+    b32r = bx.subdomain_bytes(3, 7, K, bx.ELASTIC3D, bx.FP32_RHS)
+    assert b32r >= 9 * K * 512 * (8 * 2 + 4)  # FP32_RHS stores an FP32 dq

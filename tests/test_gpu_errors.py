@@ -9,7 +9,7 @@ import pytest
 
 import bfam_amd as bx
 import bfam_amd.workloads as W
-from util import random_elastic
+from util import random_acoustic, random_elastic
 
 pytestmark = pytest.mark.gpu
 
@@ -105,3 +105,23 @@ def test_budget_overrun_is_an_error_not_an_abort():
         sub.close()
     finally:
         big.close()
+
+
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP64_PLAIN, bx.FP32_RHS, bx.FP32])
+def test_resident_bytes_match_the_admission_estimate(dev, prec):
+    """bfamx_subdomain_bytes (admission control) is what the upload
+    allocates: FP32 and FP64 hold two states and no dq."""
+    for physics, dims, N in ((bx.ELASTIC3D, (3, 2, 2), 3), (bx.ELASTIC3D, (2, 2, 2), 7),
+                             (bx.ACOUSTIC2D, (4, 3), 5)):
+        if physics == bx.ACOUSTIC2D:
+            cfg, _ = random_acoustic(dims, N)
+        else:
+            cfg, _ = random_elastic(dims, N, seed=6, src=False)
+        cfg.pop("src", None)
+        used0, _ = dev.memory()
+        sub = W.upload(dev, cfg, prec)
+        nf, nbytes = sub.info()
+        m = cfg["mesh"]
+        assert nbytes == bx.subdomain_bytes(m.dim, N, m.K, physics, prec), (physics, N)
+        assert dev.memory()[0] - used0 == nbytes
+        sub.close()
