@@ -294,6 +294,32 @@ def test_mixed_order_launch_strategies(dev, prec):
             assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("prec", [bx.FP32, bx.FP64, bx.FP32_RHS])
+def test_autotune_mid_step_keeps_state(dev, prec):
+    """Autotuning in the middle of a step (its trial launches repeat the
+    current stage) restores what they overwrite: dq, or, where dq is rebuilt
+    from the states, the previous state.  The step then continues bit for
+    bit as without autotuning."""
+    print(f"autotune mid-step prec={prec}", flush=True)
+    res = []
+    for tune in (False, True):
+        cfg, q0 = random_elastic((3, 2, 2), 7, seed=77)
+        s = W.upload(dev, cfg, prec)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.stage()
+        s.stage()
+        b = bx.Batch(dev, [s])
+        if tune:
+            b.autotune(trials=2, force=True)
+        dev.set_kernel(prec, 7, 1)
+        b.step(1)
+        res.append(s.download())
+        b.close()
+        s.close()
+    dev.set_kernel(prec, 7, -1)
+    assert np.array_equal(res[0], res[1])
+
+
 def test_mms_energy_error_and_precisions(dev):
     """The device energy-norm error against the manufactured solution equals
     the oracle's energy of (q - exact); FP32_RHS (FP64 state) and FP32 stay
