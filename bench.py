@@ -37,7 +37,8 @@ The JSON line also carries:
                      3..7, 200..2000 steps, layered material, Ricker source)
                      admitted under an HBM budget and advanced together, one
                      batched launch per stage, FP32 rates on FP64 state:
-                     p50/p99 latency and aggregate DOF-stage-updates/s.
+                     p50/p99 latency and aggregate DOF-stage-updates/s;
+  serving_fp32       the same stream with FP32 state and rates.
 
 --copy-calibration runs only a 1 GiB device-to-device copy kernel (torch)
 so that the counters of a kernel of known traffic can be recorded.
@@ -256,19 +257,22 @@ def main():
             s_.close()
         log(f"adapted mesh: {adapted}")
 
-    serving = None
+    serving = {}
     if not args.only and args.serve_requests > 0:
         reqs = bx.serve_generate(seed=2024, nreq=args.serve_requests,
                                  rate_per_s=args.serve_rate)
         budget = int(args.serve_budget_gb * (1 << 30))
-        _, st = bx.serve(dev, reqs, bx.FP32_RHS, budget_bytes=budget, max_chunk=10)
-        serving = {"requests": args.serve_requests, "rate_per_s": args.serve_rate,
-                   "budget_bytes": budget, "precision": "fp32 rates, fp64 state",
-                   "p50_s": st["p50_s"], "p99_s": st["p99_s"], "max_s": st["max_s"],
-                   "dof_stage_per_s": st["dof_stage_per_s"], "wall_s": st["wall_s"],
-                   "max_active": st["max_active"], "peak_bytes": st["peak_bytes"],
-                   "batch_rebuilds": st["nbatches"]}
-        log(f"serving: {serving}")
+        for key, prec, label in (("serving", bx.FP32_RHS, "fp32 rates, fp64 state"),
+                                 ("serving_fp32", bx.FP32, "fp32 state and rates")):
+            _, st = bx.serve(dev, reqs, prec, budget_bytes=budget, max_chunk=10)
+            serving[key] = {
+                "requests": args.serve_requests, "rate_per_s": args.serve_rate,
+                "budget_bytes": budget, "precision": label,
+                "p50_s": st["p50_s"], "p99_s": st["p99_s"], "max_s": st["max_s"],
+                "dof_stage_per_s": st["dof_stage_per_s"], "wall_s": st["wall_s"],
+                "max_active": st["max_active"], "peak_bytes": st["peak_bytes"],
+                "batch_rebuilds": st["nbatches"]}
+            log(f"{key}: {serving[key]}")
     dev.close()
     if only_adapted:
         print(json.dumps({"adapted_mesh": adapted}), flush=True)
@@ -315,7 +319,8 @@ def main():
                      "bytes_per_dof": bpd},
         "cpu_baseline": cpu,
         "adapted_mesh": adapted,
-        "serving": serving,
+        "serving": serving.get("serving"),
+        "serving_fp32": serving.get("serving_fp32"),
     }
     print(json.dumps(out), flush=True)
 

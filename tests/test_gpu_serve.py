@@ -27,17 +27,18 @@ def alone_energy(dev, req, prec):
     return e
 
 
-def test_serve_alone_equals_mix(dev):
-    print("serve: 10 requests, alone vs mix", flush=True)
+@pytest.mark.parametrize("prec", [bx.FP32_RHS, bx.FP32])
+def test_serve_alone_equals_mix(dev, prec):
+    print(f"serve: 10 requests, alone vs mix, prec={prec}", flush=True)
     reqs = bx.serve_generate(seed=3, nreq=10, rate_per_s=200.0, n=(3, 6), N=(3, 7),
                              steps=(10, 40))
-    res, st = bx.serve(dev, reqs, bx.FP32_RHS, budget_bytes=1 << 30, max_chunk=7)
+    res, st = bx.serve(dev, reqs, prec, budget_bytes=1 << 30, max_chunk=7)
     print(st, flush=True)
     assert st["max_active"] >= 2  # the requests did share launches
     for q, r in zip(reqs, res):
         assert r["finished"] >= r["admitted"] >= r["arrival"] - 1e-9
         assert r["latency"] > 0 and r["energy"] > 0
-        assert r["energy"] == alone_energy(dev, q, bx.FP32_RHS)
+        assert r["energy"] == alone_energy(dev, q, prec)
     lat = sorted(r["latency"] for r in res)
     assert st["p50_s"] == lat[4] and st["p99_s"] == lat[-1]
 
