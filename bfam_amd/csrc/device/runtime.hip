@@ -500,7 +500,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   /* FP32 and FP64 keep no dq between stages: stage s >= 1 rebuilds it from
    * the state it reads and the one before it, still in the other ping-pong
    * buffer.  Per node a middle stage then moves three state-sized values
-   * (q in, q_old in, q out) instead of four (plus dq in and out) */
+   * (q in, q_old in, q out) instead of four (q in, dq in, q out, dq out) */
   h.dq_rec = (prec == BFAMX_FP32 || prec == BFAMX_FP64) && h.A[0] == 0.0;
   for (int i = 0; i + 1 < h.nstages; ++i)
     if (h.B[i] == 0.0)
