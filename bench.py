@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="launch stage by stage instead of replaying a captured graph")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--state", choices=["rest", "random"], default="rest",
+                    help="initial state of the timed runs: at rest (the Ricker "
+                         "configuration) or seeded random values at every node")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -168,7 +171,10 @@ def main():
     m = cfg["mesh"]
     ndof = 9 * m.K * m.Np
     log(f"setup: {args.n}^3 N={args.order} K={m.K} DOF={ndof} ({time.perf_counter() - t0:.1f} s)")
-    q0 = np.zeros((9, m.K * m.Np))
+    if args.state == "random":
+        q0 = np.random.default_rng(7).standard_normal((9, m.K * m.Np))
+    else:
+        q0 = np.zeros((9, m.K * m.Np))
     dev = bx.Device(0)
     K, Wm = args.steps, args.warmup
 
@@ -302,7 +308,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if head_name == "fp32" else "f64",
-        "data": "synthetic: PCG32 material, Ricker point source from rest",
+        "data": "synthetic: PCG32 material, Ricker point source from rest"
+                if args.state == "rest" else
+                "synthetic: PCG32 material, Ricker point source, seeded random initial state",
         "config": {"workload": f"elastic3d_ricker {args.n}^3 periodic hex brick, N={args.order}, "
                                "9 fields, LSRK4(5)", "elements": m.K, "dof": ndof,
                    "form": f"{head_name} fused stage, batched + HIP graph"},
