@@ -75,6 +75,31 @@ def test_elastic_lsrk_steps(dev, kernel, prec):
     assert rel_max(got, want) < TOL[prec]
 
 
+@pytest.mark.parametrize("method", ["W33", "HEUN", "FE"])
+@pytest.mark.parametrize("prec", [bx.FP32, bx.FP64, bx.FP32_RHS])
+def test_other_lsrk_tables(dev, method, prec):
+    """The other low-storage tables step as the oracle does; FP32 and FP64
+    rebuild dq from the states with each table's own A_s / B_{s-1}."""
+    import oracle
+    print(f"elastic lsrk {method} prec={prec}", flush=True)
+    cfg, q0 = random_elastic((4, 4, 3), 5, seed=9)
+    m = cfg["mesh"]
+    sub = bx.Subdomain(dev, bx.ELASTIC3D, prec, m.N, m.r, m.w, m.Dr, m.vmapM, m.vmapP, m.h,
+                       cfg["rho"], cfg["cp"], cfg["cs"], x0=m.x0,
+                       lsrk_method=getattr(bx, "LSRK_" + method))
+    s = cfg["src"]
+    sub.set_source(s["elem"], s["weights"], s["amp"], s["f0"], s["t0"])
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    sub.step(3)
+    t, st = sub.time()
+    got = sub.download()
+    sub.close()
+    want = oracle_problem(cfg).run(q0, cfg["dt"], 3, method=getattr(oracle, "LSRK_" + method))
+    ns = {"W33": 3, "HEUN": 2, "FE": 1}[method]
+    assert st == 3 * ns and abs(t - 3 * cfg["dt"]) < 1e-15
+    assert rel_max(got, want) < TOL[prec]
+
+
 def test_acoustic_plane_wave(dev):
     """Config acoustic2d_plane_wave: 100 KC54 steps, FP64, against the oracle
     and against the exact plane wave."""
