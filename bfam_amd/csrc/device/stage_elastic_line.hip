@@ -28,8 +28,9 @@
  *     a one-field LDS transpose, where the material combines them into the
  *     nine rates;
  *   - the rates go back to row order through the same transpose buffer, one
- *     field at a time, and dq and q are updated with coalesced row accesses
- *     (dq fetched ahead of use).
+ *     field at a time, and q is updated with coalesced row accesses: the
+ *     old dq, or for FP32 and FP64 the previous state it is rebuilt from,
+ *     is fetched ahead of use; only FP32_RHS stores dq.
  * Nothing crosses a workgroup barrier: each wave only touches its own LDS
  * region, and LDS operations of one wave execute in issue order.
  */
