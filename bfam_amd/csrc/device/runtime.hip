@@ -886,13 +886,14 @@ static int grid_for(const bfamx_device_t *d, int ntasks)
 }
 
 /* the stage kernel of an order not (yet) autotuned: the faster one in the
- * per-order measurements of both kernels on 30^3-sized bricks (node-parallel
- * for FP32 and FP32_RHS at every order, for FP64 from N = 5) */
+ * per-order measurements of both kernels on bricks of about 124 M DOF
+ * (node-parallel for FP32_RHS at every order, for FP32 at N = 4, 6, 7 and
+ * for FP64 from N = 5) */
 static int default_node(int prec, int N)
 {
   switch (prec)
   {
-  case PREC_FP32:
+  case PREC_FP32: return N != 3 && N != 5;
   case PREC_FP32_RHS: return 1;
   case PREC_FP64: return N >= 5;
   default: return 0;

@@ -303,8 +303,8 @@ int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force, int *choice);
 int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order);
 /* the elastic stage kernel for (precision, order) by hand: 0 line,
  * 1 node-parallel, -1 back to unknown (until autotuned, the node-parallel
- * kernel for FP32 and FP32_RHS at every order and for FP64 at N >= 5, the
- * line kernel otherwise); get returns the kernel that runs */
+ * kernel for FP32_RHS at every order, for FP32 at N = 4, 6, 7 and for FP64
+ * at N >= 5, the line kernel otherwise); get returns the kernel that runs */
 int bfamx_set_kernel(bfamx_device_t *dev, int precision, int N, int kind);
 int bfamx_get_kernel(const bfamx_device_t *dev, int precision, int N, int *kind);
 
