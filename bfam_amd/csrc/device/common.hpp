@@ -126,7 +126,7 @@ struct SubDev
    * rates of one type, A[0] = 0, B[0..S-2] nonzero) */
   int dq_rec;
   void *q[2];   /* state ping-pong (double or float) */
-  void *dq;     /* rate register (double or float) */
+  void *dq;     /* rate register (double or float); NULL with dq_rec */
   double *rhs;  /* FP64 RHS (plain path and MODE_RHS) */
   const int32_t *nbr_elem;
   const uint8_t *nbr_code;
@@ -365,7 +365,8 @@ struct StageCtx
   int dq_dead;
   double a, b, dt, t;
   double g, sw, cw;
-  /* the old dq is q[parity] - q[parity ^ 1], and a is A[s] / B[s-1] */
+  /* B[s-1] times the old dq is q[parity] - q[parity ^ 1], and a holds
+   * A[s] / B[s-1] */
   int rec;
 };
 
