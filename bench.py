@@ -102,8 +102,21 @@ def read_traffic(kernel):
     return fk * 1024.0 * (GiB / (fc * 1024.0)) + wk * 1024.0 * (GiB / (wc * 1024.0))
 
 
-def run_cpu_baseline(cfg, budget_s):
-    """FP64 C restatement (OpenMP) over whole LSRK steps of the same brick."""
+def cpu_model():
+    """the host CPU's model name (/proc/cpuinfo), or None"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def time_cpu_steps(cfg, budget_s, max_steps, label):
+    """whole LSRK steps of the FP64 C restatement on cfg from rest, until
+    budget_s or max_steps: (DOF-stage-updates/s, steps, seconds)"""
     import numpy as np
 
     import oracle
@@ -116,16 +129,41 @@ def run_cpu_baseline(cfg, budget_s):
     q = np.zeros((9, m.K * m.Np))
     ndof = 9 * m.K * m.Np
     steps, el = 0, 0.0
-    while steps == 0 or (el < budget_s and steps < 4):
+    while steps == 0 or (el < budget_s and steps < max_steps):
         t0 = time.perf_counter()
         q = P.run(q, cfg["dt"], 1, t0=steps * cfg["dt"])
         el += time.perf_counter() - t0
         steps += 1
-        log(f"cpu baseline: {steps} step(s), {el:.1f} s")
-    return {"value": ndof * 5 * steps / el, "unit": "DOF-stage-updates/s",
-            "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{steps} LSRK4(5) step(s) ({5 * steps} stages) of the same 48^3 N=7 "
-                      f"elastic brick, FP64, C restatement with OpenMP, {el:.1f} s"}
+        log(f"cpu baseline ({label}): {steps} step(s), {el:.1f} s")
+    return ndof * 5 * steps / el, steps, el
+
+
+def run_cpu_baseline(cfg, budget_s):
+    """FP64 C restatement (OpenMP) over whole LSRK steps of the same brick on
+    every host core, and on one core over a 16^3 brick of the same order and
+    material law (a bounded sample: one core would need about half a minute
+    per step of the 48^3 brick)."""
+    import oracle
+
+    import bfam_amd.workloads as W
+
+    cores = oracle.num_threads()
+    rate, steps, el = time_cpu_steps(cfg, budget_s, 4, f"{cores} cores")
+    N = cfg["mesh"].N
+    small = W.elastic3d_ricker(n=16, N=N, nsteps=200)
+    oracle.set_num_threads(1)
+    try:
+        rate1, steps1, el1 = time_cpu_steps(small, 0.5 * budget_s, 4, "1 core")
+    finally:
+        oracle.set_num_threads(cores)
+    return {"value": rate, "unit": "DOF-stage-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} LSRK4(5) step(s) ({5 * steps} stages) of the same 48^3 N={N} "
+                      f"elastic brick, FP64, C restatement with OpenMP, {el:.1f} s",
+            "cpu_model": cpu_model(),
+            "one_core": {"value": rate1, "unit": "DOF-stage-updates/s", "cores": 1,
+                         "sample": f"{steps1} LSRK4(5) step(s) of a 16^3 N={N} elastic brick "
+                                   f"(same material law and source), FP64, one thread, "
+                                   f"{el1:.1f} s"}}
 
 
 def main():

@@ -21,6 +21,7 @@ FP64, FP64_PLAIN, FP32_RHS, FP32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp64_plain": FP64_PLAIN, "fp32_rhs": FP32_RHS, "fp32": FP32}
 LSRK_KC54, LSRK_FE, LSRK_HEUN, LSRK_W33, LSRK_NOOP = 0, 1, 2, 3, 4
 OK, EINVAL, EBUDGET, EHIP, ESTATE, EUNSUPPORTED = 0, 1, 2, 3, 4, 5
+FLAG_SAME, FLAG_COARSEN, FLAG_REFINE = 0, 1, 2
 
 ELASTIC_FIELDS = ("v1", "v2", "v3", "s11", "s22", "s33", "s12", "s13", "s23")
 ACOUSTIC_FIELDS = ("p", "u", "v")
@@ -72,6 +73,31 @@ MORTAR_DTYPE = np.dtype([("s_sub", np.int32), ("s_elem", np.int32), ("b_sub", np
                          ("quad", np.int8), ("pad", np.int8)])
 
 
+class TransferMaps(C.Structure):
+    _fields_ = [("dst_to_adapt_flags", C.c_void_p), ("dst_to_dst_chld_id", C.c_void_p),
+                ("dst_to_src_subd_id", C.c_void_p), ("dst_to_src_elem_id", C.c_void_p),
+                ("num_dst", C.c_int32), ("coarse_dst_to_src_chld_id", C.c_void_p),
+                ("coarse_dst_to_src_subd_id", C.c_void_p),
+                ("coarse_dst_to_src_elem_id", C.c_void_p), ("num_coarse_dst", C.c_int32)]
+
+
+# (name, numpy dtype, length key) of the transfer-map arrays
+_MAP_ARRAYS = [("dst_to_adapt_flags", np.uint8, "num_dst"),
+               ("dst_to_dst_chld_id", np.int8, "num_dst"),
+               ("dst_to_src_subd_id", np.int32, "num_dst"),
+               ("dst_to_src_elem_id", np.int32, "num_dst"),
+               ("coarse_dst_to_src_chld_id", np.int8, "num_coarse_dst"),
+               ("coarse_dst_to_src_subd_id", np.int32, "num_coarse_dst"),
+               ("coarse_dst_to_src_elem_id", np.int32, "num_coarse_dst")]
+
+
+class DgxGlue(C.Structure):
+    _fields_ = [("sub_m", C.c_int32), ("sub_p", C.c_int32), ("partner", C.c_int32),
+                ("K", C.c_int32), ("EToEm", C.c_void_p), ("EToFm", C.c_void_p),
+                ("EToHm", C.c_void_p), ("EToEp", C.c_void_p), ("EToHp", C.c_void_p),
+                ("EToOp", C.c_void_p)]
+
+
 class RequestResult(C.Structure):
     _fields_ = [("arrival", C.c_double), ("admitted", C.c_double), ("finished", C.c_double),
                 ("latency", C.c_double), ("dof", C.c_int64), ("energy", C.c_double)]
@@ -118,6 +144,8 @@ SIGNATURES = {
     "bfamx_subdomain_free": (C.c_int, [_vp]),
     "bfamx_subdomain_info": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
     "bfamx_subdomain_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int32, C.c_int, C.c_int]),
+    "bfamx_subdomain_bytes_lsrk": (C.c_size_t, [C.c_int, C.c_int, C.c_int32, C.c_int, C.c_int,
+                                                C.POINTER(Lsrk)]),
     "bfamx_set_boundary": (C.c_int, [_vp, _i8p]),
     "bfamx_set_source": (C.c_int, [_vp, C.c_int32, _dp, _dp, C.c_double, C.c_double]),
     "bfamx_set_forcing_mms": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
@@ -153,12 +181,18 @@ SIGNATURES = {
     "bfamx_octree_sub": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int32),
                                    _vp, _vp, _vp, _vp, _vp, _vp]),
     "bfamx_octree_mortars": (C.c_int, [_vp, _vp]),
+    "bfamx_octree_elements": (C.c_int, [_vp, _vp, _vp]),
+    "bfamx_transfer_maps_init": (C.c_int, [C.POINTER(TransferMaps), _vp, _vp]),
+    "bfamx_transfer_maps_free": (C.c_int, [C.POINTER(TransferMaps)]),
+    "bfamx_transfer_fields": (C.c_int, [C.POINTER(TransferMaps), _PP, C.c_int, _PP, C.c_int,
+                                        _ip, _ip, C.c_double]),
     "bfamx_glue_upload": (C.c_int, [_vp, _PP, C.c_int, _vp, C.c_int32, _PP]),
     "bfamx_glue_free": (C.c_int, [_vp]),
     "bfamx_batch_set_glue": (C.c_int, [_vp, _vp]),
+    "bfamx_mortars_from_glue": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(C.c_int32)]),
     "bfamx_batch_rhs": (C.c_int, [_vp, C.c_double, _vp]),
     "bfamx_vtk_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int32, _dp, _dp, _dp,
-                                  C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_int]),
+                                  C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_int, _vp]),
     "bfamx_vtk_write_index": (C.c_int, [C.c_char_p, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     "bfamx_serve_generate": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int,
@@ -351,8 +385,11 @@ class Device:
         _check(lib().bfamx_set_persistent(self.h, int(on)))
 
 
-def subdomain_bytes(dim, N, K, physics, precision):
-    return lib().bfamx_subdomain_bytes(dim, N, K, physics, precision)
+def subdomain_bytes(dim, N, K, physics, precision, lsrk=None):
+    """device bytes of an upload; lsrk: an Lsrk table (None: KC54)"""
+    if lsrk is None:
+        return lib().bfamx_subdomain_bytes(dim, N, K, physics, precision)
+    return lib().bfamx_subdomain_bytes_lsrk(dim, N, K, physics, precision, C.byref(lsrk))
 
 
 class Subdomain:
@@ -378,8 +415,11 @@ class Subdomain:
                 arr("Dr", D.T), arr("vM", vM, np.int32), arr("vP", vmapP, np.int32))
         p = PhysDesc(physics, precision, arr("h", h), arr("x0", x0) if x0 is not None else None,
                      arr("rho", rho), arr("cp", cp), arr("cs", cs) if cs is not None else None)
-        t = Lsrk()
-        _check(lib().bfamx_lsrk_init(C.byref(t), lsrk_method))
+        if isinstance(lsrk_method, Lsrk):  # a custom table
+            t = lsrk_method
+        else:
+            t = Lsrk()
+            _check(lib().bfamx_lsrk_init(C.byref(t), lsrk_method))
         hh = C.c_void_p()
         _check(lib().bfamx_subdomain_upload(dev.h, C.byref(g), C.byref(p), C.byref(t),
                                             C.byref(hh)))
@@ -547,11 +587,42 @@ class Glue:
             self.h = None
 
 
+def mortars_from_glue(glues):
+    """Mortar records (MORTAR_DTYPE) from reference-format glue subdomains
+    (bfamx_mortars_from_glue): glues is a list of dicts with sub_m, sub_p
+    and the arrays EToEm, EToFm, EToHm, EToEp, EToHp, EToOp; each glue's
+    partner (the pair reversed) is found in the list."""
+    keep = []
+    arr = (DgxGlue * max(len(glues), 1))()
+    index = {(int(g["sub_m"]), int(g["sub_p"])): i for i, g in enumerate(glues)}
+    for i, g in enumerate(glues):
+        d = arr[i]
+        d.sub_m, d.sub_p = int(g["sub_m"]), int(g["sub_p"])
+        d.partner = index.get((d.sub_p, d.sub_m), -1)
+        d.K = len(g["EToEm"])
+        for name, dt in (("EToEm", np.int32), ("EToFm", np.int8), ("EToHm", np.int8),
+                         ("EToEp", np.int32), ("EToHp", np.int8), ("EToOp", np.int8)):
+            a = np.ascontiguousarray(g[name], dtype=dt)
+            keep.append(a)
+            setattr(d, name, a.ctypes.data if a.size else None)
+    n = C.c_int32()
+    _check(lib().bfamx_mortars_from_glue(C.cast(arr, C.c_void_p), len(glues), None,
+                                         C.byref(n)))
+    m = np.zeros(n.value, MORTAR_DTYPE)
+    if n.value:
+        _check(lib().bfamx_mortars_from_glue(C.cast(arr, C.c_void_p), len(glues),
+                                             m.ctypes.data, C.byref(n)))
+    return m
+
+
 # ------------------------------------------------------------------- output
 
-def vtk_write(path, dim, N, x0, h, r, fields, time=0.0, binary=True, Np_write=0):
+def vtk_write(path, dim, N, x0, h, r, fields, time=0.0, binary=True, Np_write=0,
+              mpirank=0, subdomain_id=0, root_id=0):
     """fields: dict name -> (K*(N+1)^dim,) array in bfam node order.
-    binary: 0 ascii, 1 (True) base64 binary, 2 zlib-compressed binary."""
+    binary: 0 ascii, 1 (True) base64 binary, 2 zlib-compressed binary;
+    mpirank, subdomain_id, root_id: the Int32 cell arrays of those names."""
+    ids = np.array([mpirank, subdomain_id, root_id], np.int32)
     x0 = np.ascontiguousarray(x0, dtype=np.float64).ravel()
     h = np.ascontiguousarray(h, dtype=np.float64).ravel()
     K = x0.size // dim
@@ -561,7 +632,7 @@ def vtk_write(path, dim, N, x0, h, r, fields, time=0.0, binary=True, Np_write=0)
     _check(lib().bfamx_vtk_write(str(path).encode(), dim, N, K, x0, h,
                                  np.ascontiguousarray(r, dtype=np.float64), len(names),
                                  C.cast(cn, C.c_void_p), _ptrs(arrs) if arrs else None,
-                                 float(time), int(binary), int(Np_write)))
+                                 float(time), int(binary), int(Np_write), ids.ctypes.data))
 
 
 def vtk_write_index(path, pieces, names, binary=True):
@@ -575,14 +646,70 @@ def vtk_write_index(path, pieces, names, binary=True):
 
 # ------------------------------------------------------------- adapted mesh
 
-def octree(n_base, levels, center, radius, order_by_level):
-    """Adapted periodic brick forest (bfamx_octree_build): returns a dict with
-    per-subdomain data (N, K, EToE, EToF, glued, x0, h, gid), the mortar
-    records (structured array of MORTAR_DTYPE) and leaf counts per level."""
+def _octree_handle(n_base, levels, center, radius, order_by_level):
     d = OctreeDesc(n_base, levels, (C.c_double * 3)(*center), radius,
                    (C.c_int * 4)(*(list(order_by_level) + [1] * (4 - len(order_by_level)))))
     h = C.c_void_p()
     _check(lib().bfamx_octree_build(C.byref(d), C.byref(h)))
+    return h
+
+
+def transfer_maps(dst, src):
+    """Transfer maps (bfamx_transfer_maps_init) from the forest `src` to the
+    forest `dst`, each given as the argument tuple of octree(): a dict of
+    numpy arrays named as bfamx_transfer_maps_t's fields, plus the
+    destination's per-element subdomain and local index ("dst_subd",
+    "dst_elem", bfamx_octree_elements)."""
+    hd = _octree_handle(*dst)
+    hs = None
+    try:
+        hs = _octree_handle(*src)
+        m = TransferMaps()
+        _check(lib().bfamx_transfer_maps_init(C.byref(m), hd, hs))
+        try:
+            out = {"num_dst": m.num_dst, "num_coarse_dst": m.num_coarse_dst}
+            for name, dt, nkey in _MAP_ARRAYS:
+                n = out[nkey] * (8 if nkey == "num_coarse_dst" else 1)
+                ptr = getattr(m, name)
+                a = np.zeros(n, dt)
+                if n and ptr:
+                    C.memmove(a.ctypes.data, ptr, a.nbytes)
+                out[name] = a
+        finally:
+            lib().bfamx_transfer_maps_free(C.byref(m))
+        K = out["num_dst"]
+        out["dst_subd"], out["dst_elem"] = np.zeros(K, np.int32), np.zeros(K, np.int32)
+        _check(lib().bfamx_octree_elements(hd, out["dst_subd"].ctypes.data,
+                                           out["dst_elem"].ctypes.data))
+        return out
+    finally:
+        lib().bfamx_octree_free(hd)
+        if hs is not None:
+            lib().bfamx_octree_free(hs)
+
+
+def transfer_fields(maps, src, dst, dt):
+    """Move the state of the Subdomains `src` to the Subdomains `dst` through
+    maps (a dict from transfer_maps()), on the device (bfamx_transfer_fields);
+    the destinations start at the sources' time with step dt."""
+    keep = {name: np.ascontiguousarray(maps[name], dtype=dt) for name, dt, _ in _MAP_ARRAYS}
+    m = TransferMaps()
+    for name, _, _ in _MAP_ARRAYS:
+        setattr(m, name, keep[name].ctypes.data if keep[name].size else None)
+    m.num_dst, m.num_coarse_dst = int(maps["num_dst"]), int(maps["num_coarse_dst"])
+    sa = (C.c_void_p * len(src))(*[s.h for s in src])
+    da = (C.c_void_p * len(dst))(*[s.h for s in dst])
+    _check(lib().bfamx_transfer_fields(C.byref(m), sa, len(src), da, len(dst),
+                                       np.ascontiguousarray(maps["dst_subd"], dtype=np.int32),
+                                       np.ascontiguousarray(maps["dst_elem"], dtype=np.int32),
+                                       float(dt)))
+
+
+def octree(n_base, levels, center, radius, order_by_level):
+    """Adapted periodic brick forest (bfamx_octree_build): returns a dict with
+    per-subdomain data (N, K, EToE, EToF, glued, x0, h, gid), the mortar
+    records (structured array of MORTAR_DTYPE) and leaf counts per level."""
+    h = _octree_handle(n_base, levels, center, radius, order_by_level)
     try:
         K, ns, nm = C.c_int32(), C.c_int(), C.c_int32()
         kl = np.zeros(4, np.int32)

@@ -62,6 +62,29 @@ hipError_t launch_plain_update(const SubDev *sd_dev, size_t ndof,
 hipError_t launch_clock_advance(const SubDev *subs, int nsubs, hipStream_t s);
 hipError_t launch_mortar(int prec, const GlueArgs &a, hipStream_t s);
 
+/* field transfer between adapted meshes: one source or destination
+ * subdomain (state of the current parity, order, elements) */
+struct XferSide
+{
+  void *q;
+  int N;
+  int K;
+};
+/* one destination element and its 1 (same / refine) or 8 (coarsen)
+ * sources; op[c][d]: which 1-D operator of the order pair (0 Pr, 1 Pr_cfb,
+ * 2 Pr_cft, 3 Pr_r2b, 4 Pr_r2t) source c takes along axis d */
+struct XferTask
+{
+  int32_t dst_sub, dst_elem;
+  int32_t nsrc;
+  int32_t src_sub[8], src_elem[8];
+  int8_t op[8][3];
+};
+/* ops: the FP64 table of glue_ops_index (kind 0) */
+hipError_t launch_transfer(int src_fp32, int dst_fp32, const XferSide *srcs,
+                           const XferSide *dsts, const XferTask *tasks, int ntasks,
+                           const double *ops, hipStream_t s);
+
 /* reductions: per-element partial sums (FP64) of the energy density (K), or
  * interleaved pairs (2K) of |qa - qb|^2, |qb|^2 and of the error against the
  * manufactured solution with time factors cos/sin(omega t) */

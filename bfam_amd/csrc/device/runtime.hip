@@ -76,6 +76,10 @@ struct bfamx_subdomain
   std::vector<void *> allocs;
   size_t bytes;
   int fields_set;
+  /* host record of the clock: t0 and dt of the last set_fields and the
+   * stages launched since (glued batch members must agree on all three) */
+  double t0_set, dt_set;
+  long long stages_run;
   bfamx_lsrk_t lsrk;
   bfamx_glue_t *glue; /* set while faces of this subdomain are glued */
 };
@@ -325,9 +329,48 @@ static int order_supported(int physics, int N)
   return physics == BFAMX_ACOUSTIC2D ? (N >= 1 && N <= 8) : (N >= 3 && N <= 7);
 }
 
+/* FP32 and FP64 keep no dq between stages when the table allows it:
+ * A[0] = 0 (the first stage reads no dq) and B[0..S-2] nonzero (so that
+ * dq_{s-1} = (q_s - q_{s-1}) / B_{s-1}) */
+static int rebuilds_dq(int prec, const bfamx_lsrk_t &t)
+{
+  if (prec != BFAMX_FP32 && prec != BFAMX_FP64)
+    return 0;
+  if (t.A[0] != 0.0)
+    return 0;
+  for (int i = 0; i + 1 < t.n_stages; ++i)
+    if (t.B[i] == 0.0)
+      return 0;
+  return 1;
+}
+
+static size_t subdomain_bytes(int dim, int N, int32_t K, int ph, int prec, int dq_stored);
+
 extern "C" size_t bfamx_subdomain_bytes(int dim, int N, int32_t K,
                                         bfamx_physics_t ph,
                                         bfamx_precision_t prec)
+{
+  bfamx_lsrk_t t;
+  bfamx_lsrk_init(&t, BFAMX_LSRK_KC54);
+  return bfamx_subdomain_bytes_lsrk(dim, N, K, ph, prec, &t);
+}
+
+extern "C" size_t bfamx_subdomain_bytes_lsrk(int dim, int N, int32_t K,
+                                             bfamx_physics_t ph,
+                                             bfamx_precision_t prec,
+                                             const bfamx_lsrk_t *lsrk)
+{
+  bfamx_lsrk_t t;
+  if (lsrk)
+    t = *lsrk;
+  else
+    bfamx_lsrk_init(&t, BFAMX_LSRK_KC54);
+  const int dq_stored = (prec == BFAMX_FP32 || prec == BFAMX_FP64) && !rebuilds_dq(prec, t);
+  return subdomain_bytes(dim, N, K, ph, prec, dq_stored);
+}
+
+/* dq_stored: an FP32 or FP64 subdomain whose table needs a stored dq */
+static size_t subdomain_bytes(int dim, int N, int32_t K, int ph, int prec, int dq_stored)
 {
   const size_t n1 = N + 1;
   const size_t Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
@@ -338,11 +381,19 @@ extern "C" size_t bfamx_subdomain_bytes(int dim, int N, int32_t K,
   auto add = [&](size_t x) { b += ((x ? x : 16) + rnd) & ~rnd; };
   switch (prec)
   {
-  /* FP64 and FP32 keep no dq (it is rebuilt from the two states) */
-  case BFAMX_FP64: add(8 * ndof), add(8 * ndof); break;
+  /* FP64 and FP32 keep no dq unless the table needs one (rebuilds_dq) */
+  case BFAMX_FP64:
+    add(8 * ndof), add(8 * ndof);
+    if (dq_stored)
+      add(8 * ndof);
+    break;
   case BFAMX_FP64_PLAIN: add(8 * ndof), add(8 * ndof), add(8 * ndof); break;
   case BFAMX_FP32_RHS: add(8 * ndof), add(8 * ndof), add(4 * ndof); break;
-  default: add(4 * ndof), add(4 * ndof); break;
+  default:
+    add(4 * ndof), add(4 * ndof);
+    if (dq_stored)
+      add(4 * ndof);
+    break;
   }
   const size_t nfaces = 2 * dim;
   add(4 * K * nfaces), add(K * nfaces);
@@ -407,6 +458,24 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
                                    nbr_e.data(), nbr_c.data());
   if (rc)
     return rc;
+  /* The kernels take each element's geometry as axis-aligned edge lengths,
+   * so a face must meet the opposite face of its neighbour (f ^ 1) node for
+   * node, or itself (a boundary or glued face).  A connectivity with
+   * rotated trees (face pairs of another axis, or a permuted node order)
+   * would upload and give a wrong derivative: refuse it. */
+  for (int32_t e = 0; e < K; ++e)
+    for (int f = 0; f < nfaces; ++f)
+    {
+      const size_t i = (size_t)e * nfaces + f;
+      const int f2 = nbr_c[i] & 7, perm = nbr_c[i] >> 3;
+      const int self = nbr_e[i] == e && f2 == f;
+      if (!self && (f2 != (f ^ 1) || perm != 0))
+        return bfamx_fail(BFAMX_EUNSUPPORTED,
+                          "element %d face %d meets face %d of element %d with node "
+                          "permutation %d: only axis-aligned connectivity (opposite "
+                          "faces, identity permutation) is supported",
+                          e, f, f2, nbr_e[i], perm);
+    }
 
   /* geometry and material tables */
   std::vector<double> geo((size_t)K * GEO_STRIDE, 0.0), mat((size_t)K * MAT_STRIDE, 0.0);
@@ -501,10 +570,7 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
    * the state it reads and the one before it, still in the other ping-pong
    * buffer.  Per node a middle stage then moves three state-sized values
    * (q in, q_old in, q out) instead of four (q in, dq in, q out, dq out) */
-  h.dq_rec = (prec == BFAMX_FP32 || prec == BFAMX_FP64) && h.A[0] == 0.0;
-  for (int i = 0; i + 1 < h.nstages; ++i)
-    if (h.B[i] == 0.0)
-      h.dq_rec = 0;
+  h.dq_rec = rebuilds_dq(prec, s->lsrk);
   for (int i = 0; i < 8; ++i)
     h.AB[i] = !h.dq_rec ? h.A[i] : i > 0 && i < h.nstages ? h.A[i] / h.B[i - 1] : 0.0;
 
@@ -806,6 +872,9 @@ extern "C" int bfamx_set_fields(bfamx_subdomain_t *s,
   c.dt = dt;
   HIPCHK(hipMemcpy(s->d_clock, &c, sizeof(c), hipMemcpyHostToDevice));
   s->fields_set = 1;
+  s->t0_set = t0;
+  s->dt_set = dt;
+  s->stages_run = 0;
   return refresh_clock(s);
 }
 
@@ -973,7 +1042,10 @@ extern "C" int bfamx_stage(bfamx_subdomain_t *s)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_stage: no state set");
   if (s->glue)
     return bfamx_fail(BFAMX_ESTATE, "glued subdomains advance through their batch");
-  return stage_once(s);
+  int rc = stage_once(s);
+  if (!rc)
+    ++s->stages_run;
+  return rc;
 }
 
 extern "C" int bfamx_step(bfamx_subdomain_t *s, int nsteps)
@@ -990,6 +1062,7 @@ extern "C" int bfamx_step(bfamx_subdomain_t *s, int nsteps)
       int rc = stage_once(s);
       if (rc)
         return rc;
+      ++s->stages_run;
     }
   return BFAMX_OK;
 }
@@ -1183,6 +1256,12 @@ extern "C" int bfamx_batch_create(bfamx_device_t *dev,
                         "batch members must share device, physics, precision");
     if (prec == BFAMX_FP64_PLAIN)
       return bfamx_fail(BFAMX_EUNSUPPORTED, "batches run the fused stage only");
+    /* a batch step runs the same number of stages for every member */
+    if (s->h.nstages != subs[0]->h.nstages)
+      return bfamx_fail(BFAMX_EINVAL,
+                        "batch members must use LSRK tables of one stage count "
+                        "(member %d: %d stages, member 0: %d)",
+                        i, s->h.nstages, subs[0]->h.nstages);
     if (s->N != N)
       N = -1;
   }
@@ -1440,27 +1519,52 @@ static int batch_stage(bfamx_batch_t *b)
   return BFAMX_OK;
 }
 
+/* stages of one step: bfamx_batch_create admits one stage count only */
 static int batch_nstages(const bfamx_batch_t *b)
 {
-  int ns = 0;
+  return b->subs[0]->h.nstages;
+}
+
+/* every member has a state; the members of a glued batch, whose mortars
+ * couple their traces at each stage, also share t0, dt and the stage count */
+static int batch_ready(const bfamx_batch_t *b)
+{
   for (auto *s : b->subs)
-    ns = s->h.nstages > ns ? s->h.nstages : ns;
-  return ns;
+    if (!s->fields_set)
+      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  if (b->glue)
+  {
+    const bfamx_subdomain_t *s0 = b->subs[0];
+    for (size_t i = 1; i < b->subs.size(); ++i)
+    {
+      const bfamx_subdomain_t *s = b->subs[i];
+      if (s->t0_set != s0->t0_set || s->dt_set != s0->dt_set ||
+          s->stages_run != s0->stages_run)
+        return bfamx_fail(BFAMX_ESTATE,
+                          "glued batch member %zu is out of step with member 0 "
+                          "(t0 %.17g / %.17g, dt %.17g / %.17g, stages %lld / %lld)",
+                          i, s->t0_set, s0->t0_set, s->dt_set, s0->dt_set,
+                          s->stages_run, s0->stages_run);
+    }
+  }
+  return BFAMX_OK;
 }
 
 extern "C" int bfamx_batch_step(bfamx_batch_t *b, int nsteps)
 {
   if (!b || nsteps < 0)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_batch_step: bad arguments");
-  for (auto *s : b->subs)
-    if (!s->fields_set)
-      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  int rc = batch_ready(b);
+  if (rc)
+    return rc;
   const int ns = batch_nstages(b);
   for (int i = 0; i < nsteps * ns; ++i)
   {
-    int rc = batch_stage(b);
+    rc = batch_stage(b);
     if (rc)
       return rc;
+    for (auto *s : b->subs)
+      ++s->stages_run;
   }
   return BFAMX_OK;
 }
@@ -1506,10 +1610,12 @@ extern "C" int bfamx_batch_launch_graph(bfamx_batch_t *b)
 {
   if (!b || !b->exec)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_launch_graph: not captured");
-  for (auto *s : b->subs)
-    if (!s->fields_set)
-      return bfamx_fail(BFAMX_ESTATE, "batch member without state");
+  int rc = batch_ready(b);
+  if (rc)
+    return rc;
   HIPCHK(hipGraphLaunch(b->exec, b->dev->stream));
+  for (auto *s : b->subs)
+    s->stages_run += (long long)b->graph_steps * batch_nstages(b);
   return BFAMX_OK;
 }
 
@@ -1603,6 +1709,13 @@ extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *
                         "glue: elastic fused subdomains of one precision, orders 3..7");
     if (s->glue)
       return bfamx_fail(BFAMX_ESTATE, "glue: subdomain %d is already glued", i);
+    /* the mortars couple the members' traces at every stage: one table */
+    const bfamx_lsrk_t &a = s->lsrk, &b0 = subs[0]->lsrk;
+    const size_t ns = (size_t)b0.n_stages;
+    if (a.n_stages != b0.n_stages || memcmp(a.A, b0.A, ns * sizeof(double)) ||
+        memcmp(a.B, b0.B, ns * sizeof(double)) || memcmp(a.C, b0.C, (ns + 1) * sizeof(double)))
+      return bfamx_fail(BFAMX_EINVAL,
+                        "glue: subdomain %d uses another LSRK table than subdomain 0", i);
   }
   /* group the mortars by b face: one conforming mortar, or four quadrants */
   std::vector<int32_t> ord(nm);
@@ -1900,19 +2013,46 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   a.mode = MODE_STAGE;
   a.xcd_remap = d->xcd_remap;
   a.nxcd = d->num_xcd;
-  /* mean time of `trials` launches of fn() after one warm-up */
+  /* each trial launch starts from the saved buffers: a launch rewrites
+   * what the next one would read (dq, or the previous state that the
+   * rebuilt dq comes from), and without the restore the timed data would
+   * grow by |A_s| (or |B_s A_s / B_{s-1}|) per launch */
+  auto restore = [&]() -> hipError_t {
+    for (size_t i = 0; i < b->subs.size(); ++i)
+      if (saved[i])
+      {
+        bfamx_subdomain_t *s = b->subs[i];
+        const size_t nb = (size_t)s->K * s->nf * s->Np * (s->rate_fp32 ? 4 : 8);
+        hipError_t e = hipMemcpyAsync(reg[i], saved[i], nb, hipMemcpyDeviceToDevice, d->stream);
+        if (e != hipSuccess)
+          return e;
+      }
+    return hipSuccess;
+  };
+  /* mean time of `trials` launches of fn() after one warm-up, each timed
+   * alone between events after the restore */
   auto time_it = [&](auto fn, float *ms) -> int {
-    hipError_t e = fn();
+    hipError_t e = restore();
     if (e == hipSuccess)
-      e = hipEventRecord(d->ev0, d->stream);
-    for (int t = 0; t < trials && e == hipSuccess; ++t)
       e = fn();
-    if (e == hipSuccess)
-      e = hipEventRecord(d->ev1, d->stream);
-    if (e == hipSuccess)
-      e = hipEventSynchronize(d->ev1);
-    if (e == hipSuccess)
-      e = hipEventElapsedTime(ms, d->ev0, d->ev1);
+    float total = 0.f;
+    for (int t = 0; t < trials && e == hipSuccess; ++t)
+    {
+      float one = 0.f;
+      e = restore();
+      if (e == hipSuccess)
+        e = hipEventRecord(d->ev0, d->stream);
+      if (e == hipSuccess)
+        e = fn();
+      if (e == hipSuccess)
+        e = hipEventRecord(d->ev1, d->stream);
+      if (e == hipSuccess)
+        e = hipEventSynchronize(d->ev1);
+      if (e == hipSuccess)
+        e = hipEventElapsedTime(&one, d->ev0, d->ev1);
+      total += one;
+    }
+    *ms = total;
     return e == hipSuccess ? BFAMX_OK
                            : bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
   };
@@ -1948,16 +2088,8 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     if (!rc)
       mc = (signed char)(ms[1] < ms[0] ? 1 : 0);
   }
-  for (size_t i = 0; i < b->subs.size(); ++i)
-    if (saved[i])
-    {
-      bfamx_subdomain_t *s = b->subs[i];
-      const size_t nb = (size_t)s->K * s->nf * s->Np * (s->rate_fp32 ? 4 : 8);
-      if (hipMemcpyAsync(reg[i], saved[i], nb, hipMemcpyDeviceToDevice, d->stream) !=
-              hipSuccess &&
-          !rc)
-        rc = bfamx_fail(BFAMX_EHIP, "autotune: dq restore failed");
-    }
+  if (restore() != hipSuccess && !rc)
+    rc = bfamx_fail(BFAMX_EHIP, "autotune: dq restore failed");
   (void)hipStreamSynchronize(d->stream);
   dev_free_all(d, allocs, bytes);
   if (rc)
@@ -2052,5 +2184,187 @@ extern "C" int bfamx_set_boundary(bfamx_subdomain_t *s, const int8_t *kind)
     nc[k] = (uint8_t)(NBR_BOUNDARY | (b << 3) | f);
   }
   HIPCHK(hipMemcpy((void *)s->h.nbr_code, nc.data(), nfc, hipMemcpyHostToDevice));
+  return BFAMX_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* adaptation: field transfer between meshes                               */
+/* ---------------------------------------------------------------------- */
+
+extern "C" int bfamx_transfer_fields(const bfamx_transfer_maps_t *m,
+                                     bfamx_subdomain_t *const *src, int nsrc,
+                                     bfamx_subdomain_t *const *dst, int ndst,
+                                     const int32_t *dst_subd, const int32_t *dst_elem,
+                                     double dt)
+{
+  if (!m || !src || nsrc < 1 || !dst || ndst < 1 || !dst_subd || !dst_elem ||
+      m->num_dst < 0 || (m->num_dst && (!m->dst_to_adapt_flags || !m->dst_to_dst_chld_id ||
+                                        !m->dst_to_src_subd_id || !m->dst_to_src_elem_id)) ||
+      m->num_coarse_dst < 0 ||
+      (m->num_coarse_dst && (!m->coarse_dst_to_src_chld_id || !m->coarse_dst_to_src_subd_id ||
+                             !m->coarse_dst_to_src_elem_id)))
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_transfer_fields: bad arguments");
+  bfamx_device_t *dev = src[0]->dev;
+  const int sfp32 = src[0]->state_fp32, dfp32 = dst[0]->state_fp32;
+  for (int i = 0; i < nsrc + ndst; ++i)
+  {
+    const bfamx_subdomain_t *s = i < nsrc ? src[i] : dst[i - nsrc];
+    if (!s || s->dev != dev || s->physics != BFAMX_ELASTIC3D ||
+        s->state_fp32 != (i < nsrc ? sfp32 : dfp32))
+      return bfamx_fail(BFAMX_EINVAL, "transfer: elastic subdomains of one device, one "
+                                      "state type among the sources and one among the "
+                                      "destinations");
+    if (s->glue)
+      return bfamx_fail(BFAMX_ESTATE, "transfer: free the glue of subdomain %d first", i);
+  }
+  /* the sources stand between steps, at one time */
+  double t = 0;
+  for (int i = 0; i < nsrc; ++i)
+  {
+    if (!src[i]->fields_set)
+      return bfamx_fail(BFAMX_ESTATE, "transfer: source %d has no state", i);
+    double ti;
+    int64_t st;
+    int rc = bfamx_get_time(src[i], &ti, &st);
+    if (rc)
+      return rc;
+    if (st % src[i]->h.nstages)
+      return bfamx_fail(BFAMX_ESTATE, "transfer: source %d is inside a step (stage %lld)", i,
+                        (long long)st);
+    if (i == 0)
+      t = ti;
+    else if (ti != t)
+      return bfamx_fail(BFAMX_ESTATE, "transfer: sources at different times (%.17g, %.17g)",
+                        t, ti);
+  }
+  /* tasks, with every index checked against the subdomains */
+  const int32_t nd = m->num_dst;
+  std::vector<XferTask> tasks(nd);
+  std::vector<std::vector<char>> seen(ndst);
+  int32_t covered = 0;
+  for (int i = 0; i < ndst; ++i)
+    seen[i].assign((size_t)dst[i]->K, 0);
+  auto src_ok = [&](int32_t s, int32_t e) { return s >= 0 && s < nsrc && e >= 0 && e < src[s]->K; };
+  for (int32_t k = 0; k < nd; ++k)
+  {
+    XferTask &t = tasks[k];
+    memset(&t, 0, sizeof(t));
+    const int32_t ds = dst_subd[k], de = dst_elem[k];
+    if (ds < 0 || ds >= ndst || de < 0 || de >= dst[ds]->K || seen[ds][de])
+      return bfamx_fail(BFAMX_EINVAL, "transfer: destination %d names (%d, %d): out of range "
+                                      "or named twice", k, ds, de);
+    seen[ds][de] = 1;
+    ++covered;
+    t.dst_sub = ds;
+    t.dst_elem = de;
+    const int flag = m->dst_to_adapt_flags[k], cid = m->dst_to_dst_chld_id[k];
+    if (flag == BFAMX_FLAG_COARSEN)
+    {
+      const int32_t c = -1 - m->dst_to_src_subd_id[k];
+      if (c < 0 || c >= m->num_coarse_dst)
+        return bfamx_fail(BFAMX_EINVAL, "transfer: destination %d: bad coarse index", k);
+      t.nsrc = 8;
+      for (int j = 0; j < 8; ++j)
+      {
+        const int32_t s = m->coarse_dst_to_src_subd_id[8 * c + j];
+        const int32_t e = m->coarse_dst_to_src_elem_id[8 * c + j];
+        const int ch = m->coarse_dst_to_src_chld_id[8 * c + j];
+        if (!src_ok(s, e) || ch < 0 || ch > 7)
+          return bfamx_fail(BFAMX_EINVAL, "transfer: destination %d: bad child %d", k, j);
+        t.src_sub[j] = s;
+        t.src_elem[j] = e;
+        for (int d = 0; d < 3; ++d)
+          t.op[j][d] = (int8_t)(1 + ((ch >> d) & 1));
+      }
+    }
+    else if (flag == BFAMX_FLAG_SAME || flag == BFAMX_FLAG_REFINE)
+    {
+      const int32_t s = m->dst_to_src_subd_id[k], e = m->dst_to_src_elem_id[k];
+      if (!src_ok(s, e) || cid < 0 || cid > 7)
+        return bfamx_fail(BFAMX_EINVAL, "transfer: destination %d: bad source", k);
+      t.nsrc = 1;
+      t.src_sub[0] = s;
+      t.src_elem[0] = e;
+      for (int d = 0; d < 3; ++d)
+        t.op[0][d] = (int8_t)(flag == BFAMX_FLAG_SAME ? 0 : 3 + ((cid >> d) & 1));
+    }
+    else
+      return bfamx_fail(BFAMX_EINVAL, "transfer: destination %d has flag %d", k, flag);
+  }
+  int64_t ktot = 0;
+  for (int i = 0; i < ndst; ++i)
+    ktot += dst[i]->K;
+  if (covered != ktot)
+    return bfamx_fail(BFAMX_EINVAL, "transfer: the maps name %d of the %lld destination "
+                                    "elements", covered, (long long)ktot);
+  /* sides: the sources' current states; the destinations' first buffer */
+  std::vector<XferSide> hs(nsrc), hd(ndst);
+  for (int i = 0; i < nsrc; ++i)
+  {
+    int parity = 0, rc = current_parity(src[i], &parity);
+    if (rc)
+      return rc;
+    hs[i].q = src[i]->h.q[parity];
+    hs[i].N = src[i]->N;
+    hs[i].K = src[i]->K;
+  }
+  for (int i = 0; i < ndst; ++i)
+  {
+    hd[i].q = dst[i]->h.q[0];
+    hd[i].N = dst[i]->N;
+    hd[i].K = dst[i]->K;
+  }
+  const std::vector<double> ops = glue_ops_table<double>();
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  void *p_src = NULL, *p_dst = NULL, *p_task = NULL, *p_ops = NULL;
+  int rc = dev_alloc(dev, allocs, &bytes, sizeof(XferSide) * nsrc, &p_src);
+  if (!rc)
+    rc = dev_alloc(dev, allocs, &bytes, sizeof(XferSide) * ndst, &p_dst);
+  if (!rc)
+    rc = dev_alloc(dev, allocs, &bytes, sizeof(XferTask) * (nd ? nd : 1), &p_task);
+  if (!rc)
+    rc = dev_alloc(dev, allocs, &bytes, sizeof(double) * ops.size(), &p_ops);
+  hipError_t e = hipSuccess;
+  if (!rc)
+  {
+    e = hipMemcpy(p_src, hs.data(), sizeof(XferSide) * nsrc, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(p_dst, hd.data(), sizeof(XferSide) * ndst, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nd)
+      e = hipMemcpy(p_task, tasks.data(), sizeof(XferTask) * nd, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(p_ops, ops.data(), sizeof(double) * ops.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = launch_transfer(sfp32, dfp32, (const XferSide *)p_src, (const XferSide *)p_dst,
+                          (const XferTask *)p_task, nd, (const double *)p_ops, dev->stream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess)
+      rc = bfamx_fail(BFAMX_EHIP, "transfer: %s", hipGetErrorString(e));
+  }
+  dev_free_all(dev, allocs, bytes);
+  if (rc)
+    return rc;
+  /* the destinations start a step at t with step dt */
+  for (int i = 0; i < ndst; ++i)
+  {
+    bfamx_subdomain_t *s = dst[i];
+    const size_t ndof = (size_t)s->K * s->nf * s->Np;
+    if (s->h.dq)
+      HIPCHK(hipMemset(s->h.dq, 0, (s->rate_fp32 ? 4 : 8) * ndof));
+    Clock c;
+    memset(&c, 0, sizeof(c));
+    c.t = t;
+    c.dt = dt;
+    HIPCHK(hipMemcpy(s->d_clock, &c, sizeof(c), hipMemcpyHostToDevice));
+    s->fields_set = 1;
+    s->t0_set = t;
+    s->dt_set = dt;
+    s->stages_run = 0;
+    rc = refresh_clock(s);
+    if (rc)
+      return rc;
+  }
   return BFAMX_OK;
 }

@@ -525,3 +525,165 @@ int bfamx_octree_mortars(const bfamx_octree_t *o, bfamx_mortar_t *m)
   memcpy(m, o->mortar, sizeof(bfamx_mortar_t) * (size_t)o->nmortar);
   return BFAMX_OK;
 }
+
+/* ------------------------------------------------------------ adaptation */
+
+int bfamx_octree_elements(const bfamx_octree_t *o, int32_t *sub_of, int32_t *local_of)
+{
+  if (!o)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_octree_elements: NULL mesh");
+  for (int32_t i = 0; i < o->K; ++i)
+  {
+    if (sub_of)
+      sub_of[i] = o->sub_of[i];
+    if (local_of)
+      local_of[i] = o->local_of[i];
+  }
+  return BFAMX_OK;
+}
+
+/* child id of a leaf within its parent (bit d: upper half along axis d),
+ * 0 for a tree root, as p4est_quadrant_child_id numbers children */
+static int child_id(const leaf_t *q, int L)
+{
+  if (q->level == 0)
+    return 0;
+  const int sh = L - q->level;
+  return ((q->x[0] >> sh) & 1) | (((q->x[1] >> sh) & 1) << 1) | (((q->x[2] >> sh) & 1) << 2);
+}
+
+int bfamx_transfer_maps_free(bfamx_transfer_maps_t *m)
+{
+  if (!m)
+    return BFAMX_OK;
+  free(m->dst_to_adapt_flags);
+  free(m->dst_to_dst_chld_id);
+  free(m->dst_to_src_subd_id);
+  free(m->dst_to_src_elem_id);
+  free(m->coarse_dst_to_src_chld_id);
+  free(m->coarse_dst_to_src_subd_id);
+  free(m->coarse_dst_to_src_elem_id);
+  memset(m, 0, sizeof(*m));
+  return BFAMX_OK;
+}
+
+/*
+ * Every destination leaf is a source leaf (same), a child of one (refine)
+ * or the parent of eight (coarsen); anything else is more than one level
+ * of change and an error, as bfam_domain_pxest_transfer_maps_init aborts
+ * (quadrant_compare, bfam.c:5175-5203).  Leaves are compared on the finer
+ * of the two meshes' grids, so the meshes may have different refinement
+ * depths.  Coarsened destinations store -1 - (their index in the coarse
+ * arrays) as source subdomain and element, and their eight children in
+ * child-id order, which is their storage (Morton) order, as the reference
+ * walks them (bfam.c:5323-5341).
+ */
+int bfamx_transfer_maps_init(bfamx_transfer_maps_t *m, const bfamx_octree_t *dst,
+                             const bfamx_octree_t *src)
+{
+  if (!m || !dst || !src)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_transfer_maps_init: bad arguments");
+  memset(m, 0, sizeof(*m));
+  if (dst->n_base != src->n_base)
+    return bfamx_fail(BFAMX_EINVAL,
+                      "transfer: the meshes have different base forests (%d^3 and %d^3 "
+                      "trees)", dst->n_base, src->n_base);
+  const int Lm = dst->levels > src->levels ? dst->levels : src->levels;
+  const int sd = Lm - dst->levels, ss = Lm - src->levels;
+  table_t tab = {0, 0, 0};
+  if (!table_init(&tab, (size_t)src->K))
+    return bfamx_fail(BFAMX_EINVAL, "transfer: out of host memory");
+  for (int32_t i = 0; i < src->K; ++i)
+  {
+    const leaf_t *q = &src->leaf[i];
+    const int x[3] = {q->x[0] << ss, q->x[1] << ss, q->x[2] << ss};
+    table_put(&tab, leaf_key(q->level, x), i);
+  }
+  const int32_t nd = dst->K;
+  m->num_dst = nd;
+  m->dst_to_adapt_flags = malloc((size_t)nd);
+  m->dst_to_dst_chld_id = malloc((size_t)nd);
+  m->dst_to_src_subd_id = malloc(sizeof(int32_t) * (size_t)nd);
+  m->dst_to_src_elem_id = malloc(sizeof(int32_t) * (size_t)nd);
+  int32_t *kind_src = malloc(sizeof(int32_t) * 8 * (size_t)nd);
+  int rc = BFAMX_OK;
+  int32_t ncoarse = 0;
+  if (!m->dst_to_adapt_flags || !m->dst_to_dst_chld_id || !m->dst_to_src_subd_id ||
+      !m->dst_to_src_elem_id || !kind_src)
+    rc = bfamx_fail(BFAMX_EINVAL, "transfer: out of host memory");
+  for (int32_t i = 0; i < nd && !rc; ++i)
+  {
+    const leaf_t *q = &dst->leaf[i];
+    const int l = q->level;
+    const int x[3] = {q->x[0] << sd, q->x[1] << sd, q->x[2] << sd};
+    m->dst_to_dst_chld_id[i] = (int8_t)child_id(q, dst->levels);
+    int32_t j = table_get(&tab, leaf_key(l, x));
+    if (j >= 0)
+    {
+      m->dst_to_adapt_flags[i] = BFAMX_FLAG_SAME;
+      m->dst_to_src_subd_id[i] = src->sub_of[j];
+      m->dst_to_src_elem_id[i] = src->local_of[j];
+      continue;
+    }
+    if (l >= 1)
+    {
+      const int s = 1 << (Lm - l + 1);
+      const int p[3] = {x[0] & ~(s - 1), x[1] & ~(s - 1), x[2] & ~(s - 1)};
+      j = table_get(&tab, leaf_key(l - 1, p));
+      if (j >= 0)
+      {
+        m->dst_to_adapt_flags[i] = BFAMX_FLAG_REFINE;
+        m->dst_to_src_subd_id[i] = src->sub_of[j];
+        m->dst_to_src_elem_id[i] = src->local_of[j];
+        continue;
+      }
+    }
+    int found = l < Lm;
+    if (found)
+    {
+      const int h = 1 << (Lm - l - 1);
+      for (int c = 0; c < 8 && found; ++c)
+      {
+        const int y[3] = {x[0] + ((c & 1) ? h : 0), x[1] + ((c & 2) ? h : 0),
+                          x[2] + ((c & 4) ? h : 0)};
+        j = table_get(&tab, leaf_key(l + 1, y));
+        found = j >= 0;
+        kind_src[8 * (size_t)ncoarse + c] = j;
+      }
+    }
+    if (!found)
+    {
+      rc = bfamx_fail(BFAMX_EINVAL,
+                      "transfer: destination element %d (level %d) is more than one "
+                      "level of refinement away from the source mesh", i, l);
+      break;
+    }
+    m->dst_to_adapt_flags[i] = BFAMX_FLAG_COARSEN;
+    m->dst_to_src_subd_id[i] = -1 - ncoarse;
+    m->dst_to_src_elem_id[i] = -1 - ncoarse;
+    ++ncoarse;
+  }
+  table_free(&tab);
+  if (!rc)
+  {
+    m->num_coarse_dst = ncoarse;
+    const size_t nc = 8 * (size_t)(ncoarse ? ncoarse : 1);
+    m->coarse_dst_to_src_chld_id = malloc(nc);
+    m->coarse_dst_to_src_subd_id = malloc(sizeof(int32_t) * nc);
+    m->coarse_dst_to_src_elem_id = malloc(sizeof(int32_t) * nc);
+    if (!m->coarse_dst_to_src_chld_id || !m->coarse_dst_to_src_subd_id ||
+        !m->coarse_dst_to_src_elem_id)
+      rc = bfamx_fail(BFAMX_EINVAL, "transfer: out of host memory");
+    for (size_t c = 0; !rc && c < 8 * (size_t)ncoarse; ++c)
+    {
+      const int32_t j = kind_src[c];
+      m->coarse_dst_to_src_chld_id[c] = (int8_t)child_id(&src->leaf[j], src->levels);
+      m->coarse_dst_to_src_subd_id[c] = src->sub_of[j];
+      m->coarse_dst_to_src_elem_id[c] = src->local_of[j];
+    }
+  }
+  free(kind_src);
+  if (rc)
+    bfamx_transfer_maps_free(m);
+  return rc;
+}

@@ -1,19 +1,21 @@
 /*
  * VTK XML unstructured-grid (.vtu) output of a subdomain's nodal fields.
  *
- * Follows what the reference writes per subdomain piece
+ * Writes what the reference writes per subdomain piece
  * (bfam_subdomain_dgx_vtk_write_vtu_piece, bfam.c:6050-6572, and the
  * resampling of bfam_subdomain_dgx_vtk_interp, bfam.c:5872-5929): the nodal
  * values, optionally resampled to Np_write equispaced points per direction
  * ("poor man's high-order viz"), every element split into N_vtk^dim linear
- * cells (quads / hexahedra), a "time" cell array, and one point array per
- * field.  Binary output is VTK's inline base64 with a 64-bit byte-count
- * header; compressed output (binary = 2, the reference's
- * compressor="vtkZLibDataCompressor" files written through
- * sc_vtk_write_compressed, bfam.c:5931-5938, 8278-8281) splits each array
- * into 32 KiB blocks deflated by zlib, after a header of 64-bit words:
- * block count, block size, size of the last block, then each block's
- * compressed size.  The header and the blocks are base64-encoded
+ * cells (VTK_PIXEL / VTK_VOXEL, vertices in lexicographic order), Int32
+ * connectivity and offsets, the cell arrays "time", "mpirank",
+ * "subdomain_id" and "root_id", and one point array per field.  Files are
+ * version 0.1 without a header_type, so VTK reads 32-bit header words:
+ * binary output is base64 of [uint32 byte count][data]; compressed output
+ * (binary = 2, the reference's compressor="vtkZLibDataCompressor" files
+ * written through sc_vtk_write_compressed, bfam.c:5931-5938, 8278-8281)
+ * splits each array into 32 KiB blocks deflated by zlib, after a header of
+ * 32-bit words: block count, block size, size of the last block, then each
+ * block's compressed size.  The header and the blocks are base64-encoded
  * separately, as VTK reads them.  Affine elements: node coordinates come
  * from each element's lower corner and edge lengths.
  */
@@ -48,27 +50,29 @@ enum
   ZBLOCK = 1 << 15 /* uncompressed bytes per compressed block */
 };
 
-/* base64 of [uint64 nbytes][data], as VTK reads "binary" inline arrays
- * with header_type="UInt64"; with compress, the zlib block layout above.
- * Returns 0, or -1 if memory or zlib fails. */
+/* base64 of [uint32 nbytes][data], as VTK reads "binary" inline arrays of
+ * version-0.1 files; with compress, the zlib block layout above.  Returns
+ * 0, or -1 if memory or zlib fails or an array does not fit 32-bit words. */
 static int write_base64(FILE *fp, const void *data, size_t nbytes, int compress)
 {
+  if (nbytes > UINT32_MAX)
+    return -1;
   if (!compress)
   {
-    const uint64_t hdr = nbytes;
-    unsigned char *buf = malloc(8 + nbytes);
+    const uint32_t hdr = (uint32_t)nbytes;
+    unsigned char *buf = malloc(4 + nbytes);
     if (!buf)
       return -1;
-    memcpy(buf, &hdr, 8);
-    memcpy(buf + 8, data, nbytes);
-    put_base64(fp, buf, 8 + nbytes);
+    memcpy(buf, &hdr, 4);
+    memcpy(buf + 4, data, nbytes);
+    put_base64(fp, buf, 4 + nbytes);
     fputc('\n', fp);
     free(buf);
     return 0;
   }
   const size_t nblk = (nbytes + ZBLOCK - 1) / ZBLOCK;
   const size_t last = nbytes - (nblk ? (nblk - 1) * (size_t)ZBLOCK : 0);
-  uint64_t *hdr = malloc(sizeof(uint64_t) * (3 + nblk));
+  uint32_t *hdr = malloc(sizeof(uint32_t) * (3 + nblk));
   const uLong bound = compressBound(ZBLOCK);
   unsigned char *z = malloc(nblk * (size_t)bound + 1);
   if (!hdr || !z)
@@ -77,9 +81,9 @@ static int write_base64(FILE *fp, const void *data, size_t nbytes, int compress)
     free(z);
     return -1;
   }
-  hdr[0] = nblk;
+  hdr[0] = (uint32_t)nblk;
   hdr[1] = ZBLOCK;
-  hdr[2] = last;
+  hdr[2] = (uint32_t)last;
   size_t zt = 0;
   for (size_t k = 0; k < nblk; ++k)
   {
@@ -92,10 +96,10 @@ static int write_base64(FILE *fp, const void *data, size_t nbytes, int compress)
       free(z);
       return -1;
     }
-    hdr[3 + k] = zn;
+    hdr[3 + k] = (uint32_t)zn;
     zt += zn;
   }
-  put_base64(fp, (const unsigned char *)hdr, sizeof(uint64_t) * (3 + nblk));
+  put_base64(fp, (const unsigned char *)hdr, sizeof(uint32_t) * (3 + nblk));
   put_base64(fp, z, zt);
   fputc('\n', fp);
   free(hdr);
@@ -122,16 +126,16 @@ static int write_f64(FILE *fp, const char *name, int ncomp, const double *v, siz
   return rc;
 }
 
-static int write_i64(FILE *fp, const char *name, const int64_t *v, size_t n, int binary)
+static int write_i32(FILE *fp, const char *name, const int32_t *v, size_t n, int binary)
 {
   int rc = 0;
-  fprintf(fp, "        <DataArray type=\"Int64\" Name=\"%s\" format=\"%s\">\n", name,
+  fprintf(fp, "        <DataArray type=\"Int32\" Name=\"%s\" format=\"%s\">\n", name,
           binary ? "binary" : "ascii");
   if (binary)
-    rc = write_base64(fp, v, sizeof(int64_t) * n, binary == 2);
+    rc = write_base64(fp, v, sizeof(int32_t) * n, binary == 2);
   else
     for (size_t i = 0; i < n; ++i)
-      fprintf(fp, "%lld\n", (long long)v[i]);
+      fprintf(fp, "%d\n", (int)v[i]);
   fprintf(fp, "        </DataArray>\n");
   return rc;
 }
@@ -153,7 +157,7 @@ static int write_u8(FILE *fp, const char *name, const uint8_t *v, size_t n, int 
 int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *elem_x0,
                     const double *elem_h, const double *r, int nfields,
                     const char *const *names, const double *const *fields, double time,
-                    int binary, int Np_write)
+                    int binary, int Np_write, const int32_t *ids)
 {
   if (!path || (dim != 2 && dim != 3) || N < 1 || K < 0 || !elem_x0 || !elem_h || !r ||
       nfields < 0 || (nfields && (!names || !fields)) || Np_write == 1 || Np_write < 0 ||
@@ -175,15 +179,24 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *e
   const int cpe = dim == 2 ? (nv - 1) * (nv - 1) : (nv - 1) * (nv - 1) * (nv - 1);
   const size_t ncells = (size_t)K * cpe;
   const int nvert = dim == 2 ? 4 : 8;
+  /* connectivity and offsets are Int32, as the reference's bfam_locidx_t */
+  if ((double)nvert * (double)ncells > (double)INT32_MAX)
+  {
+    free(I);
+    free(ref);
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: %zu cells do not fit Int32 offsets",
+                      ncells);
+  }
   double *xyz = malloc(sizeof(double) * 3 * npts);
-  int64_t *conn = malloc(sizeof(int64_t) * nvert * ncells);
-  int64_t *offs = malloc(sizeof(int64_t) * ncells);
+  int32_t *conn = malloc(sizeof(int32_t) * nvert * ncells);
+  int32_t *offs = malloc(sizeof(int32_t) * ncells);
   uint8_t *type = malloc(ncells);
   double *tcell = malloc(sizeof(double) * (ncells ? ncells : 1));
+  int32_t *icell = malloc(sizeof(int32_t) * (ncells ? ncells : 1));
   double *val = malloc(sizeof(double) * (npts ? npts : 1));
   FILE *fp = fopen(path, "w");
   int rc = BFAMX_OK;
-  if (!I || !ref || !xyz || !conn || !offs || !type || !tcell || !val || !fp)
+  if (!I || !ref || !xyz || !conn || !offs || !type || !tcell || !icell || !val || !fp)
   {
     rc = bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write: cannot write %s", path);
     goto done;
@@ -199,39 +212,29 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *e
                0.5 * (ref[idx[d]] + 1.0) * elem_h[(size_t)e * dim + d];
     }
   {
+    /* pixels / voxels with their vertices in lexicographic order, as the
+     * reference lists them (bfam.c:6160-6210) */
     size_t c = 0;
     for (int32_t e = 0; e < K; ++e)
     {
-      const int64_t base = (int64_t)e * Npw;
+      const int32_t base = e * Npw;
       const int nk = dim == 3 ? nv - 1 : 1;
       for (int k = 0; k < nk; ++k)
         for (int j = 0; j < nv - 1; ++j)
           for (int i = 0; i < nv - 1; ++i, ++c)
           {
-            int64_t *v = conn + (size_t)nvert * c;
-            const int64_t o = base + i + (int64_t)nv * (j + (int64_t)nv * k);
-            const int64_t di = 1, dj = nv, dk = (int64_t)nv * nv;
-            v[0] = o;
-            v[1] = o + di;
-            v[2] = o + di + dj;
-            v[3] = o + dj;
-            if (dim == 3)
-            {
-              v[4] = o + dk;
-              v[5] = o + di + dk;
-              v[6] = o + di + dj + dk;
-              v[7] = o + dj + dk;
-            }
-            offs[c] = (int64_t)nvert * (int64_t)(c + 1);
-            type[c] = dim == 2 ? 9 : 12; /* VTK_QUAD, VTK_HEXAHEDRON */
+            int32_t *v = conn + (size_t)nvert * c;
+            const int32_t o = base + i + nv * (j + nv * k);
+            for (int t = 0; t < nvert; ++t)
+              v[t] = o + (t & 1) + nv * ((t >> 1) & 1) + nv * nv * (t >> 2);
+            offs[c] = (int32_t)(nvert * (c + 1));
+            type[c] = dim == 2 ? 8 : 11; /* VTK_PIXEL, VTK_VOXEL */
             tcell[c] = time;
           }
     }
   }
   fprintf(fp, "<?xml version=\"1.0\"?>\n");
-  fprintf(fp,
-          "<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
-          "header_type=\"UInt64\"%s>\n",
+  fprintf(fp, "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\"%s byte_order=\"LittleEndian\">\n",
           binary == 2 ? " compressor=\"vtkZLibDataCompressor\"" : "");
   int zerr = 0;
   fprintf(fp, "  <UnstructuredGrid>\n");
@@ -240,12 +243,21 @@ int bfamx_vtk_write(const char *path, int dim, int N, int32_t K, const double *e
   zerr |= write_f64(fp, "Position", 3, xyz, npts, binary);
   fprintf(fp, "      </Points>\n");
   fprintf(fp, "      <Cells>\n");
-  zerr |= write_i64(fp, "connectivity", conn, (size_t)nvert * ncells, binary);
-  zerr |= write_i64(fp, "offsets", offs, ncells, binary);
+  zerr |= write_i32(fp, "connectivity", conn, (size_t)nvert * ncells, binary);
+  zerr |= write_i32(fp, "offsets", offs, ncells, binary);
   zerr |= write_u8(fp, "types", type, ncells, binary);
   fprintf(fp, "      </Cells>\n");
-  fprintf(fp, "      <CellData Scalars=\"time\">\n");
+  fprintf(fp, "      <CellData Scalars=\"time,mpirank,subdomain_id,root_id\">\n");
   zerr |= write_f64(fp, "time", 1, tcell, ncells, binary);
+  {
+    static const char *idname[3] = {"mpirank", "subdomain_id", "root_id"};
+    for (int a = 0; a < 3; ++a)
+    {
+      for (size_t c = 0; c < ncells; ++c)
+        icell[c] = ids ? ids[a] : 0;
+      zerr |= write_i32(fp, idname[a], icell, ncells, binary);
+    }
+  }
   fprintf(fp, "      </CellData>\n");
   fprintf(fp, "      <PointData>\n");
   {
@@ -312,6 +324,7 @@ done:
   free(offs);
   free(type);
   free(tcell);
+  free(icell);
   free(val);
   return rc;
 }
@@ -333,9 +346,7 @@ int bfamx_vtk_write_index(const char *path, int npieces, const char *const *piec
     return bfamx_fail(BFAMX_EINVAL, "bfamx_vtk_write_index: cannot write %s", path);
   const char *fmt = binary ? "binary" : "ascii";
   fprintf(fp, "<?xml version=\"1.0\"?>\n");
-  fprintf(fp,
-          "<VTKFile type=\"PUnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
-          "header_type=\"UInt64\"%s>\n",
+  fprintf(fp, "<VTKFile type=\"PUnstructuredGrid\" version=\"0.1\"%s byte_order=\"LittleEndian\">\n",
           binary == 2 ? " compressor=\"vtkZLibDataCompressor\"" : "");
   fprintf(fp, "  <PUnstructuredGrid GhostLevel=\"0\">\n");
   fprintf(fp, "    <PPoints>\n");
@@ -344,8 +355,11 @@ int bfamx_vtk_write_index(const char *path, int npieces, const char *const *piec
           "format=\"%s\"/>\n",
           fmt);
   fprintf(fp, "    </PPoints>\n");
-  fprintf(fp, "    <PCellData Scalars=\"time\">\n");
+  fprintf(fp, "    <PCellData Scalars=\"time,mpirank,subdomain_id,root_id\">\n");
   fprintf(fp, "      <PDataArray type=\"Float64\" Name=\"time\" format=\"%s\"/>\n", fmt);
+  fprintf(fp, "      <PDataArray type=\"Int32\" Name=\"mpirank\" format=\"%s\"/>\n", fmt);
+  fprintf(fp, "      <PDataArray type=\"Int32\" Name=\"subdomain_id\" format=\"%s\"/>\n", fmt);
+  fprintf(fp, "      <PDataArray type=\"Int32\" Name=\"root_id\" format=\"%s\"/>\n", fmt);
   fprintf(fp, "    </PCellData>\n");
   fprintf(fp, "    <PPointData>\n");
   for (int f = 0; f < nfields; ++f)

@@ -197,14 +197,16 @@ class SubMesh:
 
 
 def elastic3d_octree(n_base=16, levels=2, seed=2024, radius=0.2, orders=(7, 5, 3),
-                     cfl=0.5, nsteps=200, material="pcg"):
+                     cfl=0.5, nsteps=200, material="pcg", center=None):
     """Config: periodic unit cube of n_base^3 trees, leaves meeting a seeded
-    sphere refined `levels` times, 2:1 face balanced; leaves of level l have
-    order orders[l]; one subdomain per order, glued by mortars.  Material per
-    element from PCG32 in global (Morton) element order, or homogeneous
-    (material = (rho, cp, cs)); Ricker source at a seeded point."""
+    sphere (or one centred at `center`) refined `levels` times, 2:1 face
+    balanced; leaves of level l have order orders[l]; one subdomain per
+    order, glued by mortars.  Material per element from PCG32 in global
+    (Morton) element order, or homogeneous (material = (rho, cp, cs));
+    Ricker source at a seeded point."""
     g = bx.PCG32(seed, 13)
-    center = [g.uniform(0.3, 0.7) for _ in range(3)]
+    seeded = [g.uniform(0.3, 0.7) for _ in range(3)]
+    center = list(center) if center is not None else seeded
     o = bx.octree(n_base, levels, center, radius, list(orders)[:levels + 1])
     subs = [SubMesh(d) for d in o["subs"]]
     if material == "pcg":
@@ -228,9 +230,22 @@ def elastic3d_octree(n_base=16, levels=2, seed=2024, radius=0.2, orders=(7, 5, 3
                    "x": xs.tolist()}
             break
     return {"octree": o, "subs": subs, "mortars": o["mortars"], "rho": rho, "cp": cp,
+            "octree_args": (n_base, levels, center, radius, list(orders)[:levels + 1]),
             "cs": cs, "dt": dt, "nsteps": nsteps, "src": src, "center": center,
             "radius": radius, "K_level": o["K_level"], "physics": bx.ELASTIC3D,
             "dof": sum(9 * sm.K * sm.Np for sm in subs)}
+
+
+def adaptation_pair(n_base=4, levels=1, centers=((0.3, 0.4, 0.5), (0.55, 0.4, 0.5)),
+                    radius=0.15, orders=(5, 3), material=(1.3, 2.0, 1.1)):
+    """Two adapted meshes of one base forest whose refined sphere moves from
+    centers[0] to centers[1] (some leaves refine, some coarsen, the rest stay),
+    and the transfer maps from the first to the second."""
+    a = elastic3d_octree(n_base, levels, radius=radius, orders=orders, material=material,
+                         center=centers[0])
+    b = elastic3d_octree(n_base, levels, radius=radius, orders=orders, material=material,
+                         center=centers[1])
+    return a, b, bx.transfer_maps(b["octree_args"], a["octree_args"])
 
 
 def boundary_kinds(mesh, kind_of_face):
@@ -246,12 +261,13 @@ def boundary_kinds(mesh, kind_of_face):
     return out
 
 
-def upload(dev, cfg, precision):
+def upload(dev, cfg, precision, lsrk_method=bx.LSRK_KC54):
     """bfam_amd.Subdomain for a config dict (boundaries, source and forcing
     attached)."""
     m = cfg["mesh"]
     sub = bx.Subdomain(dev, cfg["physics"], precision, m.N, m.r, m.w, m.Dr, m.vmapM, m.vmapP,
-                       m.h, cfg["rho"], cfg["cp"], cfg.get("cs"), x0=m.x0)
+                       m.h, cfg["rho"], cfg["cp"], cfg.get("cs"), x0=m.x0,
+                       lsrk_method=lsrk_method)
     if cfg.get("bc") is not None:
         sub.set_boundary(cfg["bc"])
     if cfg.get("src") is not None:
@@ -262,14 +278,14 @@ def upload(dev, cfg, precision):
     return sub
 
 
-def upload_multi(dev, cfg, precision):
+def upload_multi(dev, cfg, precision, lsrk_method=bx.LSRK_KC54):
     """One bfam_amd.Subdomain per subdomain of an adapted-mesh config, with
     the source attached to the subdomain that holds it (no glue yet)."""
     subs = []
     for si, sm in enumerate(cfg["subs"]):
         sub = bx.Subdomain(dev, bx.ELASTIC3D, precision, sm.N, sm.r, sm.w, sm.Dr, sm.vmapM,
                            sm.vmapP, sm.h, cfg["rho"][si], cfg["cp"][si], cfg["cs"][si],
-                           x0=sm.x0)
+                           x0=sm.x0, lsrk_method=lsrk_method)
         s = cfg.get("src")
         if s is not None and s["sub"] == si:
             sub.set_source(s["elem"], s["weights"], s["amp"], s["f0"], s["t0"])

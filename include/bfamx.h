@@ -223,6 +223,10 @@ int bfamx_subdomain_info(const bfamx_subdomain_t *sub, int *nfields,
  * states and no dq */
 size_t bfamx_subdomain_bytes(int dim, int N, int32_t K, bfamx_physics_t ph,
                              bfamx_precision_t prec);
+/* the same for any LSRK table (NULL: KC54): FP32 and FP64 also hold a dq
+ * when the table has A[0] != 0 or a zero B[s] before the last stage */
+size_t bfamx_subdomain_bytes_lsrk(int dim, int N, int32_t K, bfamx_physics_t ph,
+                                  bfamx_precision_t prec, const bfamx_lsrk_t *lsrk);
 
 /*
  * Physical boundary conditions on faces that have no neighbour (faces the
@@ -371,6 +375,68 @@ int bfamx_octree_sub(const bfamx_octree_t *o, int s, int *N, int32_t *K,
                      double *h, int32_t *gid);
 /* the mortars, sorted by big side */
 int bfamx_octree_mortars(const bfamx_octree_t *o, bfamx_mortar_t *m);
+/* per element of the whole mesh (storage order): its subdomain and its
+ * index within that subdomain (the inverse of bfamx_octree_sub's gid) */
+int bfamx_octree_elements(const bfamx_octree_t *o, int32_t *sub_of,
+                          int32_t *local_of);
+
+/* ---------------------------------------------------------------------- */
+/* adaptation: moving the fields to a refined / coarsened mesh between steps */
+/* ---------------------------------------------------------------------- */
+#define BFAMX_FLAG_SAME 0    /* BFAM_FLAG_SAME, bfam.h:239 */
+#define BFAMX_FLAG_COARSEN 1 /* BFAM_FLAG_COARSEN, bfam.h:236 */
+#define BFAMX_FLAG_REFINE 2  /* BFAM_FLAG_REFINE, bfam.h:237 */
+
+/*
+ * Transfer maps from a source mesh to a destination mesh of the same base
+ * forest, field for field bfam_domain_pxest_transfer_maps_t (bfam.h:
+ * 1122-1135).  Per destination element (mesh storage order): its adapt
+ * flag, its own child id (bit d: upper half along axis d; 0 for a tree
+ * root), and its source element as (subdomain, element) or, when
+ * coarsened, -1 - c with c indexing the coarse arrays, which list the 8
+ * source children (child id, subdomain, element) in child-id order.
+ */
+typedef struct
+{
+  uint8_t *dst_to_adapt_flags;
+  int8_t *dst_to_dst_chld_id;
+  int32_t *dst_to_src_subd_id;
+  int32_t *dst_to_src_elem_id;
+  int32_t num_dst;
+  int8_t *coarse_dst_to_src_chld_id;
+  int32_t *coarse_dst_to_src_subd_id;
+  int32_t *coarse_dst_to_src_elem_id;
+  int32_t num_coarse_dst;
+} bfamx_transfer_maps_t;
+
+/* bfam_domain_pxest_transfer_maps_init (bfam.c:5205-5387): allocates the
+ * arrays; more than one level of change at an element is an error
+ * (BFAMX_EINVAL; the reference aborts, bfam.c:5193-5196) */
+int bfamx_transfer_maps_init(bfamx_transfer_maps_t *maps,
+                             const bfamx_octree_t *dst,
+                             const bfamx_octree_t *src);
+/* bfam_domain_pxest_transfer_maps_free (bfam.c:5389-5399) */
+int bfamx_transfer_maps_free(bfamx_transfer_maps_t *maps);
+
+/*
+ * Move the state of the source subdomains to the destination subdomains
+ * through the maps, on the device, with the 1-D operators the reference
+ * uploads per order pair for this (Pr, Pr_cfb, Pr_cft, Pr_r2b, Pr_r2t,
+ * bfam.c:8399-8404, built by create_interpolators, bfam.c:5606): h-same
+ * applies Pr(N_src -> N_dst) along each axis (a copy for equal orders),
+ * h-refine Pr_r2b / Pr_r2t by the destination's child-id bit per axis,
+ * h-coarsen sums the 8 children's Pr_cfb / Pr_cft images.  dst_subd and
+ * dst_elem give each destination element's place (bfamx_octree_elements
+ * of the destination mesh).  The sources must stand between steps at one
+ * time t; the destinations get that state at t with step dt, as
+ * bfamx_set_fields would set them.  Elastic, any precisions (FP64
+ * arithmetic, rounded to each destination's state type).
+ */
+int bfamx_transfer_fields(const bfamx_transfer_maps_t *maps,
+                          bfamx_subdomain_t *const *src, int nsrc,
+                          bfamx_subdomain_t *const *dst, int ndst,
+                          const int32_t *dst_subd, const int32_t *dst_elem,
+                          double dt);
 
 /*
  * Glue: mortars between subdomains (bfamx_mortar_t, sub indices into subs),
@@ -389,29 +455,66 @@ int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *subs,
 int bfamx_glue_free(bfamx_glue_t *g);
 int bfamx_batch_set_glue(bfamx_batch_t *b, bfamx_glue_t *g);
 
+/*
+ * The reference's glue subdomains as they are (bfam_subdomain_dgx_glue_data_t,
+ * bfam.h:1319-1362, filled by bfam_subdomain_dgx_glue_init, bfam.c:7480-7492):
+ * one per ordered pair of volume subdomains (minus sub_m, plus sub_p, as
+ * indices into the subdomain list later given to bfamx_glue_upload), with
+ * K pieces: EToEm / EToFm / EToHm on the minus side, EToEp (the piece's
+ * index in the partner glue, the pair (sub_p, sub_m), which is `partner`
+ * in this array; itself for a subdomain's own hanging faces), EToHp and
+ * EToOp.  The arrays are the reference's own (bfam_locidx_t = int32_t).
+ */
+typedef struct
+{
+  int32_t sub_m, sub_p;
+  int32_t partner;
+  int32_t K;
+  const int32_t *EToEm;
+  const int8_t *EToFm;
+  const int8_t *EToHm;
+  const int32_t *EToEp;
+  const int8_t *EToHp;
+  const int8_t *EToOp;
+} bfamx_dgx_glue_t;
+
+/* mortar records for bfamx_glue_upload from the reference's glue
+ * subdomains: one per hanging piece (from its small side) and one per
+ * conforming piece between two subdomains.  mortars = NULL only counts.
+ * Checks that every piece and its partner name each other with matching
+ * hanging numbers on opposite faces; a nonzero orientation (rotated trees)
+ * is BFAMX_EUNSUPPORTED. */
+int bfamx_mortars_from_glue(const bfamx_dgx_glue_t *glue, int nglue,
+                            bfamx_mortar_t *mortars, int32_t *nmortars);
+
 /* ---------------------------------------------------------------------- */
 /* output                                                                  */
 /* ---------------------------------------------------------------------- */
 /*
  * One subdomain's nodal fields (host arrays of K*(N+1)^dim doubles, bfam
- * node order) to a VTK XML unstructured-grid file: every element split into
- * linear cells through Np_write equispaced points per direction (0: the LGL
- * nodes), a "time" cell array and one point array per field.  binary: 0
- * ascii, 1 base64 binary, 2 base64 binary compressed with zlib in 32 KiB
- * blocks (compressor="vtkZLibDataCompressor", as the reference writes with
- * its compress flag).  The per-subdomain piece of bfam_vtk_write_file
- * (bfam.c:6050-6572, 8234-8326) for affine elements.
+ * node order) to a VTK XML unstructured-grid file, as the reference writes
+ * its per-subdomain piece (bfam_subdomain_dgx_vtk_write_vtu_piece,
+ * bfam.c:6050-6572, 8234-8326) for affine elements: every element split
+ * into linear VTK_PIXEL / VTK_VOXEL cells through Np_write equispaced
+ * points per direction (0: the LGL nodes), Int32 connectivity and offsets,
+ * the cell arrays "time", "mpirank", "subdomain_id" and "root_id" (ids[0],
+ * ids[1], ids[2]: the writer's rank, subdomain->id and subdomain->uid;
+ * NULL writes zeros), and one point array per field.  binary: 0 ascii,
+ * 1 base64 binary, 2 base64 binary compressed with zlib in 32 KiB blocks
+ * (compressor="vtkZLibDataCompressor", the reference's compress flag).
+ * Version-0.1 files: 32-bit header words, so an array is at most 4 GiB.
  */
 int bfamx_vtk_write(const char *path, int dim, int N, int32_t K,
                     const double *elem_x0, const double *elem_h, const double *r,
                     int nfields, const char *const *names,
                     const double *const *fields, double time, int binary,
-                    int Np_write);
+                    int Np_write, const int32_t *ids);
 /*
  * The index (.pvtu) over npieces .vtu files written by bfamx_vtk_write with
  * the same field names and encoding (binary as there), so that a viewer
  * opens all subdomains as one dataset: the PUnstructuredGrid master file of
- * bfam_vtk_write_file_pvtu (bfam.c:8066-8170), one Piece per file.
+ * bfam_vtk_write_file_pvtu (bfam.c:8066-8170), one Piece per file, with
+ * the same cell arrays (time, mpirank, subdomain_id, root_id).
  * piece_paths are written as given (relative to the index's directory).
  */
 int bfamx_vtk_write_index(const char *path, int npieces,

@@ -87,6 +87,10 @@ def lib():
         L.orc_ricker.argtypes = [C.c_double, C.c_double, C.c_double]
         L.orc_ricker.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_transfer.argtypes = [C.c_int, C.c_int32] + [C.c_void_p] * 15
+        L.orc_transfer.restype = C.c_int
+        L.orc_set_num_threads.restype = None
         _lib = L
     return _lib
 
@@ -232,8 +236,47 @@ def mms_forcing_coef(A, k, omega, rho, lam, mu):
     return F
 
 
+def transfer(maps, src_qs, src_N, dst_K, dst_N, nf=9):
+    """Fields moved to the destination subdomains through transfer maps
+    (a dict named as bfamx_transfer_maps_t's fields, plus dst_subd and
+    dst_elem): src_qs[s] is (nf, K_s * Np_s) in bfam layout; returns one
+    (nf, K_d * Np_d) array per destination subdomain."""
+    srcs = [np.ascontiguousarray(q, dtype=np.float64) for q in src_qs]
+    dsts = [np.zeros((nf, K * (N + 1) ** 3)) for K, N in zip(dst_K, dst_N)]
+    src_K = np.array([q.shape[1] // (N + 1) ** 3 for q, N in zip(srcs, src_N)], np.int32)
+    sp = (C.c_void_p * len(srcs))(*[q.ctypes.data for q in srcs])
+    dp = (C.c_void_p * len(dsts))(*[q.ctypes.data for q in dsts])
+    a = {k: np.ascontiguousarray(maps[k], dtype=dt) for k, dt in (
+        ("dst_to_adapt_flags", np.uint8), ("dst_to_dst_chld_id", np.int8),
+        ("dst_to_src_subd_id", np.int32), ("dst_to_src_elem_id", np.int32),
+        ("coarse_dst_to_src_chld_id", np.int8), ("coarse_dst_to_src_subd_id", np.int32),
+        ("coarse_dst_to_src_elem_id", np.int32), ("dst_subd", np.int32),
+        ("dst_elem", np.int32))}
+    a["src_N"] = np.array(src_N, np.int32)
+    a["src_K"] = src_K
+    a["dst_N"] = np.array(dst_N, np.int32)
+    a["dst_K"] = np.array(dst_K, np.int32)
+    ptr = lambda x: x.ctypes.data  # the arrays stay referenced by `a`
+    rc = lib().orc_transfer(nf, int(maps["num_dst"]), ptr(a["dst_to_adapt_flags"]),
+                            ptr(a["dst_to_dst_chld_id"]), ptr(a["dst_to_src_subd_id"]),
+                            ptr(a["dst_to_src_elem_id"]), ptr(a["coarse_dst_to_src_chld_id"]),
+                            ptr(a["coarse_dst_to_src_subd_id"]),
+                            ptr(a["coarse_dst_to_src_elem_id"]),
+                            ptr(a["src_N"]), ptr(a["src_K"]), sp,
+                            ptr(a["dst_subd"]), ptr(a["dst_elem"]),
+                            ptr(a["dst_N"]), ptr(a["dst_K"]), dp)
+    if rc:
+        raise RuntimeError("oracle transfer failed")
+    return dsts
+
+
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    """OpenMP threads of the oracle's parallel loops"""
+    lib().orc_set_num_threads(int(n))
 
 
 class OracleProblem:

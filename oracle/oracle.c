@@ -28,6 +28,17 @@ int orc_num_threads(void)
 #endif
 }
 
+/* threads of the following parallel loops (n >= 1) */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+  if (n >= 1)
+    omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* BFAM_LONG_REAL_APPROX_EQ (bfam.h:182, bfam.c:23) with long real = double */
 static int approx_eq(double x, double y, double K)
 {
@@ -1520,4 +1531,131 @@ void orc_lsrk_run_multi(const orc_problem_t *const *subs, int nsub,
     t += dt;
   }
   free(n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* field transfer between adapted meshes                                    */
+/* ------------------------------------------------------------------------ */
+
+/* y = P x along axis `axis` of a 3-D array x with extents n[3] (index
+ * i + n0 (j + n1 k)); P is (nb x n[axis]) column-major; y has the same
+ * extents except nb along `axis` */
+static void apply_axis(const double *P, int nb, int axis, const int *n,
+                       const double *x, double *y)
+{
+  int m[3] = {n[0], n[1], n[2]};
+  m[axis] = nb;
+  const int na = n[axis];
+  for (int k = 0; k < m[2]; ++k)
+    for (int j = 0; j < m[1]; ++j)
+      for (int i = 0; i < m[0]; ++i)
+      {
+        int o[3] = {i, j, k};
+        const int r = o[axis];
+        double acc = 0;
+        for (int c = 0; c < na; ++c)
+        {
+          o[axis] = c;
+          acc += P[r + c * nb] * x[o[0] + n[0] * (o[1] + n[1] * o[2])];
+        }
+        y[i + m[0] * (j + m[1] * k)] = acc;
+      }
+}
+
+/*
+ * Transfer of nf fields from source subdomains to destination subdomains
+ * after h-adaptation, driven by the reference's transfer maps
+ * (bfam_domain_pxest_transfer_maps_t, bfam.h:1122-1135, as filled by
+ * bfam_domain_pxest_transfer_maps_init, bfam.c:5205-5387) and the 1-D
+ * operators it uploads per order pair (Pr, Pr_cfb, Pr_cft, Pr_r2b, Pr_r2t,
+ * bfam.c:8399-8404, i.e. prj[0..4] of create_interpolators, bfam.c:5606).
+ * The kernel that applies them is not in the reference (it was application
+ * code); the operators fix it:
+ *   same    (flag 0): Pr(N_src -> N_dst) along each axis (identity, i.e. a
+ *                     copy, when the orders agree);
+ *   refine  (flag 2): along axis d, Pr_r2b if bit d of the destination's
+ *                     child id is 0, else Pr_r2t;
+ *   coarsen (flag 1): the sum over the 8 source children of, along axis d,
+ *                     Pr_cfb if bit d of the child's id is 0, else Pr_cft.
+ * Fields in bfam layout: q[s][f * K_s * Np_s + e * Np_s + n].
+ */
+int orc_transfer(int nf, int32_t num_dst, const uint8_t *flags,
+                 const int8_t *dst_chld, const int32_t *src_subd,
+                 const int32_t *src_elem, const int8_t *c_chld,
+                 const int32_t *c_subd, const int32_t *c_elem,
+                 const int *src_N, const int32_t *src_K,
+                 const double *const *src_q, const int32_t *dst_subd,
+                 const int32_t *dst_elem, const int *dst_N,
+                 const int32_t *dst_K, double *const *dst_q)
+{
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : rc)
+  for (int32_t k = 0; k < num_dst; ++k)
+  {
+    const int Nd = dst_N[dst_subd[k]], nd = Nd + 1;
+    const size_t Npd = (size_t)nd * nd * nd;
+    const int nsrc = flags[k] == 1 ? 8 : 1;
+    double *acc = calloc(Npd * nf, sizeof(double));
+    for (int c = 0; c < nsrc; ++c)
+    {
+      int32_t ss, se;
+      int cid;
+      if (flags[k] == 1)
+      {
+        const int32_t ci = -1 - src_subd[k];
+        ss = c_subd[8 * ci + c];
+        se = c_elem[8 * ci + c];
+        cid = c_chld[8 * ci + c];
+      }
+      else
+      {
+        ss = src_subd[k];
+        se = src_elem[k];
+        cid = dst_chld[k];
+      }
+      const int Ns = src_N[ss], ns = Ns + 1;
+      const size_t Nps = (size_t)ns * ns * ns;
+      const size_t blk = (size_t)ns * nd;
+      double *prj = malloc(sizeof(double) * 5 * blk);
+      double *mp = malloc(sizeof(double) * 5 * blk);
+      double *wmp = malloc(sizeof(double) * 5 * blk);
+      int null0 = 0;
+      rc |= orc_interpolator(Ns, Nd, prj, mp, wmp, &null0) != 0;
+      if (null0)
+        for (int i = 0; i < nd; ++i)
+          for (int j = 0; j < ns; ++j)
+            prj[i + j * nd] = i == j;
+      const double *op[3];
+      for (int d = 0; d < 3; ++d)
+      {
+        const int bit = (cid >> d) & 1;
+        const int kk = flags[k] == 0 ? 0 : (flags[k] == 2 ? 3 + bit : 1 + bit);
+        op[d] = prj + kk * blk;
+      }
+      double *t0 = malloc(sizeof(double) * (size_t)nd * ns * ns);
+      double *t1 = malloc(sizeof(double) * (size_t)nd * nd * ns);
+      double *t2 = malloc(sizeof(double) * Npd);
+      for (int f = 0; f < nf; ++f)
+      {
+        const double *x = src_q[ss] + (size_t)f * src_K[ss] * Nps + (size_t)se * Nps;
+        int n0[3] = {ns, ns, ns};
+        apply_axis(op[0], nd, 0, n0, x, t0);
+        int n1[3] = {nd, ns, ns};
+        apply_axis(op[1], nd, 1, n1, t0, t1);
+        int n2[3] = {nd, nd, ns};
+        apply_axis(op[2], nd, 2, n2, t1, t2);
+        for (size_t n = 0; n < Npd; ++n)
+          acc[f * Npd + n] += t2[n];
+      }
+      free(t0), free(t1), free(t2);
+      free(prj), free(mp), free(wmp);
+    }
+    const int sd = dst_subd[k];
+    for (int f = 0; f < nf; ++f)
+      for (size_t n = 0; n < Npd; ++n)
+        dst_q[sd][(size_t)f * dst_K[sd] * Npd + (size_t)dst_elem[k] * Npd + n] =
+            acc[f * Npd + n];
+    free(acc);
+  }
+  return rc ? -1 : 0;
 }

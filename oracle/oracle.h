@@ -182,6 +182,18 @@ void orc_mms_forcing_coef(const double *A, double k, double omega, double rho,
 double orc_ricker(double f0, double t0, double t);
 
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
+
+/* field transfer after h-adaptation through the transfer maps and the 1-D
+ * projection operators (bfam.h:1122-1135, bfam.c:5205-5387, 8399-8404) */
+int orc_transfer(int nf, int32_t num_dst, const uint8_t *flags,
+                 const int8_t *dst_chld, const int32_t *src_subd,
+                 const int32_t *src_elem, const int8_t *c_chld,
+                 const int32_t *c_subd, const int32_t *c_elem,
+                 const int *src_N, const int32_t *src_K,
+                 const double *const *src_q, const int32_t *dst_subd,
+                 const int32_t *dst_elem, const int *dst_N,
+                 const int32_t *dst_K, double *const *dst_q);
 
 #ifdef __cplusplus
 }

@@ -51,9 +51,10 @@ int main(int argc, char **argv)
       double *f0 = calloc((size_t)K * Np, sizeof(double));
       const char *names[1] = {"p"};
       const double *fields[1] = {f0};
+      const int32_t ids[3] = {0, 2, 7};
       for (int bin = 0; bin < 3; ++bin)
         for (int nw = 0; nw <= 4; nw += 2)
-          if (bfamx_vtk_write(vtk_path, dim, N, K, x0, h, rr, 1, names, fields, 0.5, bin, nw))
+          if (bfamx_vtk_write(vtk_path, dim, N, K, x0, h, rr, 1, names, fields, 0.5, bin, nw, ids))
           {
             printf("vtk\n");
             return 1;

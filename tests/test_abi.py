@@ -66,3 +66,24 @@ def test_subdomain_bytes_budget_formula():
     assert state64 <= b64 < 1.1 * state64 and b32 < 0.6 * b64
     b32r = bx.subdomain_bytes(3, 7, K, bx.ELASTIC3D, bx.FP32_RHS)
     assert b32r >= 9 * K * 512 * (8 * 2 + 4)  # FP32_RHS stores an FP32 dq
+
+
+def test_subdomain_bytes_count_a_stored_dq_for_tables_that_need_one():
+    """A table with A[0] != 0 cannot rebuild dq from the two states, so FP32
+    and FP64 hold a dq as well (checked without a device)."""
+    K, N = 1000, 5
+    ndof = 9 * K * (N + 1) ** 3
+    t = bx.Lsrk()
+    t.n_stages = 2
+    t.A[0], t.A[1] = 0.25, -0.5
+    t.B[0], t.B[1] = 1.0, 0.5
+    t.C[0], t.C[1], t.C[2] = 0.0, 1.0, 1.0
+    kc = bx.Lsrk()
+    bx.lib().bfamx_lsrk_init(C.byref(kc), bx.LSRK_KC54)
+    rnd = lambda x: (x + 255) & ~255
+    for prec, word in ((bx.FP64, 8), (bx.FP32, 4)):
+        base = bx.subdomain_bytes(3, N, K, bx.ELASTIC3D, prec)
+        assert bx.subdomain_bytes(3, N, K, bx.ELASTIC3D, prec, lsrk=kc) == base
+        assert bx.subdomain_bytes(3, N, K, bx.ELASTIC3D, prec, lsrk=t) == base + rnd(word * ndof)
+    b = bx.subdomain_bytes(3, N, K, bx.ELASTIC3D, bx.FP32_RHS)
+    assert bx.subdomain_bytes(3, N, K, bx.ELASTIC3D, bx.FP32_RHS, lsrk=t) == b

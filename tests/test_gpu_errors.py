@@ -125,3 +125,79 @@ def test_resident_bytes_match_the_admission_estimate(dev, prec):
         assert nbytes == bx.subdomain_bytes(m.dim, N, m.K, physics, prec), (physics, N)
         assert dev.memory()[0] - used0 == nbytes
         sub.close()
+
+
+def test_batch_members_must_share_the_stage_count(dev):
+    """A batch step runs one stage count for every member: a Heun (2-stage)
+    member beside KC54 (5-stage) ones is refused at batch creation."""
+    cfg, q0 = random_elastic((2, 2, 2), 4, seed=9, src=False)
+    kc = W.upload(dev, cfg, bx.FP32)
+    heun = W.upload(dev, cfg, bx.FP32, lsrk_method=bx.LSRK_HEUN)
+    with pytest.raises(bx.BfamxError) as e:
+        bx.Batch(dev, [kc, heun])
+    assert e.value.code == bx.EINVAL
+    # two Heun members batch and step as Heun steps (2 stages each)
+    heun2 = W.upload(dev, cfg, bx.FP32, lsrk_method=bx.LSRK_HEUN)
+    for s in (heun, heun2):
+        s.set_fields(q0, 0.0, cfg["dt"])
+    b = bx.Batch(dev, [heun, heun2])
+    b.step(3)
+    assert heun.time()[1] == 6 and heun2.time()[1] == 6
+    b.close()
+    for s in (kc, heun, heun2):
+        s.close()
+
+
+def custom_table(a0):
+    """a 2-stage table with A[0] = a0 (a nonzero A[0] needs a stored dq)"""
+    t = bx.Lsrk()
+    t.n_stages = 2
+    t.A[0], t.A[1] = a0, -0.5
+    t.B[0], t.B[1] = 1.0, 0.5
+    t.C[0], t.C[1], t.C[2] = 0.0, 1.0, 1.0
+    return t
+
+
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS, bx.FP32])
+def test_resident_bytes_match_the_estimate_for_any_table(dev, prec):
+    """bfamx_subdomain_bytes_lsrk counts the dq that a table with A[0] != 0
+    makes FP32 and FP64 store, as the upload allocates it."""
+    cfg, _ = random_elastic((2, 2, 2), 5, seed=10, src=False)
+    m = cfg["mesh"]
+    for a0 in (0.0, 0.25):
+        t = custom_table(a0)
+        used0, _ = dev.memory()
+        sub = W.upload(dev, cfg, prec, lsrk_method=t)
+        nbytes = sub.info()[1]
+        assert nbytes == bx.subdomain_bytes(3, 5, m.K, bx.ELASTIC3D, prec, lsrk=t), (prec, a0)
+        assert dev.memory()[0] - used0 == nbytes
+        sub.close()
+
+
+@pytest.mark.parametrize("case", ["permuted", "other_axis"])
+def test_rotated_connectivity_is_unsupported(dev, case):
+    """Two elements whose shared face is rotated (a permuted node order, or
+    an x face meeting a y face, as p4est connectivities with rotated trees
+    give): the kernels take axis-aligned geometry, so the upload refuses it
+    instead of computing a wrong derivative.  The same pair with the
+    identity orientation uploads."""
+    N, dim = 3, 3
+    EToE = np.tile(np.arange(2, dtype=np.int32)[:, None], (1, 6))
+    EToF = np.tile(np.arange(6, dtype=np.int8)[None, :], (2, 1))
+    EToE[0, 1], EToE[1, 0] = 1, 0
+    if case == "permuted":
+        EToF[0, 1], EToF[1, 0] = 0 + 6 * 1, 1 + 6 * 1
+    else:
+        EToE[1, 0], EToF[1, 0] = 1, 0  # element 1's -x face is a boundary
+        EToE[1, 2] = 0
+        EToF[0, 1], EToF[1, 2] = 2, 1
+    vM, vP = bx.build_vmaps(dim, N, EToE, EToF)
+    mesh = W.Mesh(3, [2, 1, 1], N, periodic=[False, False, False])
+    K = 2
+    cfg = {"mesh": mesh, "rho": np.ones(K), "cp": np.full(K, 2.0), "cs": np.ones(K),
+           "physics": bx.ELASTIC3D}
+    used0, _ = dev.memory()
+    assert upload_code(dev, cfg, vmapM=vM, vmapP=vP) == bx.EUNSUPPORTED
+    assert dev.memory()[0] == used0
+    sub = W.upload(dev, cfg, bx.FP32)  # the brick's own (aligned) maps
+    sub.close()

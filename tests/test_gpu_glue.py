@@ -84,7 +84,7 @@ def test_glued_rhs_mixed_order_conforming(dev, prec):
     assert rel_max_multi(got, want) < TOL[prec]
 
 
-@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS])
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS, bx.FP32])
 def test_glued_lsrk_steps(dev, kernel, prec):
     print(f"glued lsrk 3 steps prec={prec}", flush=True)
     cfg = octree_cfg()
@@ -130,6 +130,63 @@ def test_full_size_octree_rhs(dev):
     teardown(subs, glue, b)
     want = oracle_multi(cfg).rhs(q0, t)
     assert rel_max_multi(got, want) < 1e-12
+
+
+def test_glue_ingested_from_reference_arrays(dev):
+    """Mortars made from the reference-format glue subdomains (restated by
+    the oracle from the element geometry) glue the same mesh: the batch RHS
+    equals the one of the builder's own mortars bit for bit."""
+    print("glue ingested from reference arrays", flush=True)
+    from oracle.glue import glue_subdomains
+    from test_glue_ingest import mesh_arrays
+    cfg = octree_cfg()
+    ingested = bx.mortars_from_glue(glue_subdomains(*mesh_arrays(cfg)))
+    out = []
+    for mortars in (cfg["mortars"], ingested):
+        q0, subs, glue, b = setup(dev, dict(cfg, mortars=mortars), bx.FP64, 9)
+        out.append(b.rhs(0.3 * cfg["src"]["t0"]))
+        teardown(subs, glue, b)
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
+def test_glued_members_must_stay_in_step(dev):
+    """The mortars couple the members' traces at every stage, so a glued
+    batch refuses to step when a member was reset to another time, step size
+    or stage count, and glue refuses members with different LSRK tables."""
+    print("glued members out of step", flush=True)
+    cfg = octree_cfg()
+    q0, subs, glue, b = setup(dev, cfg, bx.FP64, seed=8)
+    b.step(1)
+    subs[1].set_fields(q0[1], 0.0, cfg["dt"])  # stage count back to 0
+    with pytest.raises(bx.BfamxError) as e:
+        b.step(1)
+    assert e.value.code == bx.ESTATE
+    for s, q in zip(subs, q0):
+        s.set_fields(q, 0.0, cfg["dt"])
+    subs[2].set_fields(q0[2], 0.0, 0.5 * cfg["dt"])  # another dt
+    with pytest.raises(bx.BfamxError) as e:
+        b.step(1)
+    assert e.value.code == bx.ESTATE
+    b.capture(1)
+    with pytest.raises(bx.BfamxError) as e:
+        b.launch_graph()
+    assert e.value.code == bx.ESTATE
+    for s, q in zip(subs, q0):
+        s.set_fields(q, 0.0, cfg["dt"])
+    b.launch_graph()
+    dev.sync()
+    assert all(s.time()[1] == 5 for s in subs)
+    teardown(subs, glue, b)
+    # members with another LSRK table
+    subs = W.upload_multi(dev, cfg, bx.FP64)
+    heun = W.upload_multi(dev, cfg, bx.FP64, lsrk_method=bx.LSRK_HEUN)
+    mixed = [subs[0], heun[1], subs[2]]
+    with pytest.raises(bx.BfamxError) as e:
+        bx.Glue(dev, mixed, cfg["mortars"])
+    assert e.value.code == bx.EINVAL
+    for s in subs + heun:
+        s.close()
 
 
 def test_glue_release_restores_faces(dev):

@@ -6,7 +6,9 @@ has the continuous operator's RHS (an error in a quadrant bit, a hanging
 half-mass factor or a projection direction is O(1)), the upwind flux
 dissipates energy across nonconforming faces, and a plane wave crosses the
 refined region unharmed.  CPU only."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -14,6 +16,8 @@ import pytest
 import bfam_amd as bx
 import bfam_amd.workloads as W
 from util import oracle_multi
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.fixture(scope="module")
@@ -70,6 +74,30 @@ def test_octree_counts_full_size():
     assert [sm.N for sm in cfg["subs"]] == [3, 5, 7]
     q = cfg["mortars"]["quad"]
     assert (q >= 0).all() and len(q) % 4 == 0
+
+
+def test_octree_matches_the_reference_mesh_probe():
+    """The reference's own mesh probe: a periodic 4x4x4 brick with one tree
+    refined once (2:1 balanced, N = 7) splits into 71 volume elements and 48
+    hanging glue elements of 64 nodes, each hanging small face appearing once
+    from its small side and once from its big side.  The same forest here
+    (a sphere inside one tree refines that tree only) has 71 leaves and 24
+    two-sided hanging mortars, four per face of the refined tree."""
+    probe = json.load(open(os.path.join(GOLDEN, "reference_constants.json")))["mesh_probe"]
+    o = bx.octree(probe["brick"][0], 1, (0.375, 0.625, 0.125), 0.01, (probe["N"], probe["N"]))
+    assert o["K"] == probe["volume_elements"] == 71
+    assert list(o["K_level"][:2]) == [63, 8]
+    m = o["mortars"]
+    assert len(m) * 2 == probe["hanging_glue_elements"]
+    assert (m["quad"] >= 0).all()
+    N = probe["N"]
+    assert (N + 1) ** 2 == probe["glue_Np"]
+    # four quadrants on each of the six faces of the one refined tree
+    big = {}
+    for r in m:
+        big.setdefault((int(r["b_elem"]), int(r["b_face"])), []).append(int(r["quad"]))
+    assert len(big) == 6 and all(sorted(q) == [0, 1, 2, 3] for q in big.values())
+    assert sorted(f ^ 1 for _, f in big) == list(range(6))
 
 
 def test_glued_free_stream(small):

@@ -18,23 +18,25 @@ def b64len(nbytes):
 
 
 def read_array(da, compressed=False):
-    """One DataArray as VTK's XML reader decodes it (header_type UInt64):
-    base64 of [byte count][data], or, compressed, the base64 of the header
-    [blocks, block size, last block size, compressed sizes...] followed by
-    the base64 of the deflated blocks."""
+    """One DataArray as VTK's XML reader decodes it in a version-0.1 file
+    (32-bit header words, as the reference writes): base64 of [byte
+    count][data], or, compressed, the base64 of the header [blocks, block
+    size, last block size, compressed sizes...] followed by the base64 of
+    the deflated blocks."""
     fmt = da.get("format")
     typ = da.get("type")
     if fmt == "ascii":
         return np.array(da.text.split(), dtype=float)
     text = "".join(da.text.split())
-    dt = {"Float64": "<f8", "Int64": "<i8", "UInt8": "u1"}[typ]
+    dt = {"Float64": "<f8", "Int32": "<i4", "UInt8": "u1"}[typ]
     if not compressed:
         raw = base64.b64decode(text)
-        n = struct.unpack("<Q", raw[:8])[0]
-        return np.frombuffer(raw[8:8 + n], dtype=dt).astype(float)
-    nblk = struct.unpack("<Q", base64.b64decode(text[:b64len(8)])[:8])[0]
-    hl = b64len(8 * (3 + nblk))
-    hdr = struct.unpack(f"<{3 + nblk}Q", base64.b64decode(text[:hl]))
+        n = struct.unpack("<I", raw[:4])[0]
+        assert len(raw) == 4 + n
+        return np.frombuffer(raw[4:4 + n], dtype=dt).astype(float)
+    nblk = struct.unpack("<I", base64.b64decode(text[:b64len(12)])[:4])[0]
+    hl = b64len(4 * (3 + nblk))
+    hdr = struct.unpack(f"<{3 + nblk}I", base64.b64decode(text[:hl]))
     assert hdr[0] == nblk and hdr[1] == 1 << 15
     data = base64.b64decode(text[hl:])
     assert len(data) == sum(hdr[3:])
@@ -66,7 +68,9 @@ def test_vtk_roundtrip(tmp_path, dim, Np_write):
     for binary in (0, 1, 2):
         p = tmp_path / f"t{binary}.vtu"
         bx.vtk_write(p, dim, N, mesh.x0, mesh.h, mesh.r, {"f": f, "g": 2 * f}, time=1.5,
-                     binary=binary, Np_write=Np_write)
+                     binary=binary, Np_write=Np_write, mpirank=3, subdomain_id=5, root_id=9)
+        root = ET.parse(p).getroot()
+        assert root.get("version") == "0.1" and root.get("header_type") is None
         piece, a = arrays(p)
         nv = Np_write or N + 1
         npts = mesh.K * nv ** dim
@@ -79,13 +83,22 @@ def test_vtk_roundtrip(tmp_path, dim, Np_write):
         assert np.array_equal(a["g"], 2 * a["f"])
         conn = a["connectivity"]
         assert conn.min() == 0 and conn.max() == npts - 1
-        assert (a["types"] == (9 if dim == 2 else 12)).all()
+        assert (a["types"] == (8 if dim == 2 else 11)).all()  # VTK_PIXEL, VTK_VOXEL
         assert (a["time"] == 1.5).all()
-        # every cell is an axis-aligned box with positive extent
+        assert (a["mpirank"] == 3).all() and (a["subdomain_id"] == 5).all()
+        assert (a["root_id"] == 9).all() and len(a["root_id"]) == ncells
+        assert [d.get("Name") for d in piece.find("CellData").findall("DataArray")] == \
+            ["time", "mpirank", "subdomain_id", "root_id"]
+        # every cell is an axis-aligned box with its vertices in
+        # lexicographic order (x fastest), as pixels and voxels take them
         nvert = 4 if dim == 2 else 8
         c = conn.reshape(-1, nvert).astype(int)
-        ext = xyz[c[:, 2], :dim] - xyz[c[:, 0], :dim]
-        assert (ext[:, 0] > 0).all() and (ext[:, 1] > 0).all()
+        lo = xyz[c[:, 0], :dim]
+        ext = xyz[c[:, -1], :dim] - lo
+        assert (ext > 0).all()
+        for t in range(nvert):
+            bits = np.array([(t >> d) & 1 for d in range(dim)])
+            assert np.allclose(xyz[c[:, t], :dim], lo + bits * ext)
         res[binary] = a
     for k in res[0]:
         assert np.allclose(res[0][k], res[1][k], rtol=1e-15, atol=0)
@@ -110,7 +123,7 @@ def test_vtk_compressed_blocks(tmp_path):
     root = ET.parse(tmp_path / "b2.vtu").getroot()
     pos = next(da for da in root.iter("DataArray") if da.get("Name") == "Position")
     text = "".join(pos.text.split())
-    hdr0 = struct.unpack("<3Q", base64.b64decode(text[:b64len(24)])[:24])
+    hdr0 = struct.unpack("<3I", base64.b64decode(text[:b64len(12)])[:12])
     nbytes = 8 * 3 * mesh.K * (N + 1) ** 3
     assert hdr0[0] == -(-nbytes // (1 << 15)) and hdr0[0] > 1
     assert hdr0[2] == nbytes - (hdr0[0] - 1) * (1 << 15)
