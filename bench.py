@@ -184,6 +184,9 @@ def main():
                     help="time one form, or the adapted mesh, only (for profiling runs)")
     ap.add_argument("--eager", action="store_true",
                     help="launch stage by stage instead of replaying a captured graph")
+    ap.add_argument("--kernel", choices=["line", "node"],
+                    help="run this elastic stage kernel instead of the autotuned one "
+                         "(for profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--state", choices=["rest", "random"], default="rest",
                     help="initial state of the timed runs: at rest (the Ricker "
@@ -220,7 +223,10 @@ def main():
         sub.set_fields(q0, 0.0, cfg["dt"])
         if fused:
             b = bx.Batch(dev, [sub])
-            b.autotune(trials=3)  # stage kernel for this precision and order
+            if args.kernel:
+                dev.set_kernel(sub.precision, args.order, 1 if args.kernel == "node" else 0)
+            else:
+                b.autotune(trials=3)  # stage kernel for this precision and order
             if not batched:
                 b.close()  # the single-subdomain path keeps the tuned kernel
         if batched:

@@ -65,12 +65,12 @@ class OctreeDesc(C.Structure):
 class MortarRec(C.Structure):
     _fields_ = [("s_sub", C.c_int32), ("s_elem", C.c_int32), ("b_sub", C.c_int32),
                 ("b_elem", C.c_int32), ("s_face", C.c_int8), ("b_face", C.c_int8),
-                ("quad", C.c_int8), ("pad", C.c_int8)]
+                ("quad", C.c_int8), ("order", C.c_int8)]
 
 
 MORTAR_DTYPE = np.dtype([("s_sub", np.int32), ("s_elem", np.int32), ("b_sub", np.int32),
                          ("b_elem", np.int32), ("s_face", np.int8), ("b_face", np.int8),
-                         ("quad", np.int8), ("pad", np.int8)])
+                         ("quad", np.int8), ("order", np.int8)])
 
 
 class TransferMaps(C.Structure):
@@ -93,7 +93,7 @@ _MAP_ARRAYS = [("dst_to_adapt_flags", np.uint8, "num_dst"),
 
 class DgxGlue(C.Structure):
     _fields_ = [("sub_m", C.c_int32), ("sub_p", C.c_int32), ("partner", C.c_int32),
-                ("K", C.c_int32), ("EToEm", C.c_void_p), ("EToFm", C.c_void_p),
+                ("K", C.c_int32), ("N", C.c_int32), ("EToEm", C.c_void_p), ("EToFm", C.c_void_p),
                 ("EToHm", C.c_void_p), ("EToEp", C.c_void_p), ("EToHp", C.c_void_p),
                 ("EToOp", C.c_void_p)]
 
@@ -589,9 +589,10 @@ class Glue:
 
 def mortars_from_glue(glues):
     """Mortar records (MORTAR_DTYPE) from reference-format glue subdomains
-    (bfamx_mortars_from_glue): glues is a list of dicts with sub_m, sub_p
-    and the arrays EToEm, EToFm, EToHm, EToEp, EToHp, EToOp; each glue's
-    partner (the pair reversed) is found in the list."""
+    (bfamx_mortars_from_glue): glues is a list of dicts with sub_m, sub_p,
+    optionally the glue order N, and the arrays EToEm, EToFm, EToHm, EToEp,
+    EToHp, EToOp; each glue's partner (the pair reversed) is found in the
+    list."""
     keep = []
     arr = (DgxGlue * max(len(glues), 1))()
     index = {(int(g["sub_m"]), int(g["sub_p"])): i for i, g in enumerate(glues)}
@@ -600,6 +601,7 @@ def mortars_from_glue(glues):
         d.sub_m, d.sub_p = int(g["sub_m"]), int(g["sub_p"])
         d.partner = index.get((d.sub_p, d.sub_m), -1)
         d.K = len(g["EToEm"])
+        d.N = int(g.get("N", 0))
         for name, dt in (("EToEm", np.int32), ("EToFm", np.int8), ("EToHm", np.int8),
                          ("EToEp", np.int32), ("EToHp", np.int8), ("EToOp", np.int8)):
             a = np.ascontiguousarray(g[name], dtype=dt)

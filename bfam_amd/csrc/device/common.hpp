@@ -202,6 +202,7 @@ struct MortarGroup
   int32_t s_sub[4], s_elem[4], s_off[4];
   int8_t b_face, nm, quad0, pad;
   int8_t s_face[4];
+  int8_t s_ng[4]; /* glue grid order of each mortar; 0: max(N_s, N_b) */
 };
 
 struct GlueArgs

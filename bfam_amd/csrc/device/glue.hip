@@ -3,7 +3,8 @@
  * between different orders), run before the stage kernel of a glued batch.
  *
  * The reference builds, per glue element, the interpolation of the minus
- * trace to the glue grid of order N_g = max(N_m, N_p) (prj{0,3,4}), the
+ * trace to the glue grid of order N_g = max(N_m, N_p), or the order its
+ * glue_order callback sets (prj{0,3,4}), the
  * projection back (prj{0,1,2}) and the mass projection
  * (bfam_subdomain_dgx_glue_init, bfam.c:7381-7414; the operators of
  * create_interpolators, bfam.c:5606-5691) but leaves applying them to the
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
     const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
     load_trace<TS, TC>(S, gp->s_elem[m], gp->s_face[m], ts[m], lane);
     Ns = S.N;
-    const int Ng = Ns > Nb ? Ns : Nb;
+    const int Ng = gp->s_ng[m] > 0 ? gp->s_ng[m] : (Ns > Nb ? Ns : Nb);
     const int quad = nm == 1 ? quad0 : m;
     const int kI = quad < 0 ? 0 : 3 + (quad & 1), kJ = quad < 0 ? 0 : 3 + ((quad >> 1) & 1);
     const int kA = quad < 0 ? 0 : 1 + (quad & 1), kB = quad < 0 ? 0 : 1 + ((quad >> 1) & 1);

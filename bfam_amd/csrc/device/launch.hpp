@@ -71,14 +71,17 @@ struct XferSide
   int K;
 };
 /* one destination element and its 1 (same / refine) or 8 (coarsen)
- * sources; op[c][d]: which 1-D operator of the order pair (0 Pr, 1 Pr_cfb,
- * 2 Pr_cft, 3 Pr_r2b, 4 Pr_r2t) source c takes along axis d */
+ * sources; bits 4d..4d+3 of op[c]: which 1-D operator of the order pair
+ * (0 Pr, 1 Pr_cfb, 2 Pr_cft, 3 Pr_r2b, 4 Pr_r2t) source c takes along axis
+ * d.  Every member is a 32-bit word, so that the kernel reads the record
+ * with aligned loads only (packed bytes were read two at a time at odd
+ * addresses) */
 struct XferTask
 {
   int32_t dst_sub, dst_elem;
   int32_t nsrc;
   int32_t src_sub[8], src_elem[8];
-  int8_t op[8][3];
+  int32_t op[8];
 };
 /* ops: the FP64 table of glue_ops_index (kind 0) */
 hipError_t launch_transfer(int src_fp32, int dst_fp32, const XferSide *srcs,

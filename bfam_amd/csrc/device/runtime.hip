@@ -1739,6 +1739,11 @@ extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *
         r.b_elem >= subs[r.b_sub]->K || r.s_face < 0 || r.s_face > 5 ||
         r.b_face != (r.s_face ^ 1) || r.quad < -1 || r.quad > 3)
       return bfamx_fail(BFAMX_EINVAL, "glue: mortar %d is malformed", i);
+    const int nmax = std::max(subs[r.s_sub]->N, subs[r.b_sub]->N);
+    if (r.order != 0 && (r.order < nmax || r.order > 7))
+      return bfamx_fail(BFAMX_EINVAL,
+                        "glue: mortar %d has order %d: 0, or %d..7 (at least both sides' "
+                        "orders)", i, (int)r.order, nmax);
   }
   std::vector<MortarGroup> groups;
   /* face -> offset in the glue buffer; every glued face appears once */
@@ -1792,6 +1797,7 @@ extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *
       g.s_sub[k] = q.s_sub;
       g.s_elem[k] = q.s_elem;
       g.s_face[k] = q.s_face;
+      g.s_ng[k] = q.order;
       rc = slot(q.s_sub, q.s_elem, q.s_face, &g.s_off[k]);
     }
     groups.push_back(g);
@@ -2274,7 +2280,7 @@ extern "C" int bfamx_transfer_fields(const bfamx_transfer_maps_t *m,
         t.src_sub[j] = s;
         t.src_elem[j] = e;
         for (int d = 0; d < 3; ++d)
-          t.op[j][d] = (int8_t)(1 + ((ch >> d) & 1));
+          t.op[j] |= (1 + ((ch >> d) & 1)) << (4 * d);
       }
     }
     else if (flag == BFAMX_FLAG_SAME || flag == BFAMX_FLAG_REFINE)
@@ -2286,7 +2292,7 @@ extern "C" int bfamx_transfer_fields(const bfamx_transfer_maps_t *m,
       t.src_sub[0] = s;
       t.src_elem[0] = e;
       for (int d = 0; d < 3; ++d)
-        t.op[0][d] = (int8_t)(flag == BFAMX_FLAG_SAME ? 0 : 3 + ((cid >> d) & 1));
+        t.op[0] |= (flag == BFAMX_FLAG_SAME ? 0 : 3 + ((cid >> d) & 1)) << (4 * d);
     }
     else
       return bfamx_fail(BFAMX_EINVAL, "transfer: destination %d has flag %d", k, flag);
@@ -2328,20 +2334,28 @@ extern "C" int bfamx_transfer_fields(const bfamx_transfer_maps_t *m,
   hipError_t e = hipSuccess;
   if (!rc)
   {
-    e = hipMemcpy(p_src, hs.data(), sizeof(XferSide) * nsrc, hipMemcpyHostToDevice);
+    /* copies on the kernel's stream: ordered before it; the host arrays live
+     * until the synchronisation below */
+    hipStream_t st = dev->stream;
+    e = hipMemcpyAsync(p_src, hs.data(), sizeof(XferSide) * nsrc, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-      e = hipMemcpy(p_dst, hd.data(), sizeof(XferSide) * ndst, hipMemcpyHostToDevice);
+      e = hipMemcpyAsync(p_dst, hd.data(), sizeof(XferSide) * ndst, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && nd)
-      e = hipMemcpy(p_task, tasks.data(), sizeof(XferTask) * nd, hipMemcpyHostToDevice);
+      e = hipMemcpyAsync(p_task, tasks.data(), sizeof(XferTask) * nd, hipMemcpyHostToDevice,
+                         st);
     if (e == hipSuccess)
-      e = hipMemcpy(p_ops, ops.data(), sizeof(double) * ops.size(), hipMemcpyHostToDevice);
+      e = hipMemcpyAsync(p_ops, ops.data(), sizeof(double) * ops.size(),
+                         hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
       e = launch_transfer(sfp32, dfp32, (const XferSide *)p_src, (const XferSide *)p_dst,
                           (const XferTask *)p_task, nd, (const double *)p_ops, dev->stream);
     if (e == hipSuccess)
       e = hipStreamSynchronize(dev->stream);
     if (e != hipSuccess)
+    {
+      (void)hipStreamSynchronize(dev->stream); /* nothing pending on the buffers */
       rc = bfamx_fail(BFAMX_EHIP, "transfer: %s", hipGetErrorString(e));
+    }
   }
   dev_free_all(dev, allocs, bytes);
   if (rc)

@@ -41,9 +41,10 @@ __global__ void __launch_bounds__(XFER_THREADS)
   {
     const XferSide &ss = srcs[t.src_sub[c]];
     const int ns = ss.N + 1, nps = ns * ns * ns;
-    const double *Px = ops + glue_ops_index(0, ss.N, dd.N, t.op[c][0]);
-    const double *Py = ops + glue_ops_index(0, ss.N, dd.N, t.op[c][1]);
-    const double *Pz = ops + glue_ops_index(0, ss.N, dd.N, t.op[c][2]);
+    const int op = t.op[c];
+    const double *Px = ops + glue_ops_index(0, ss.N, dd.N, op & 15);
+    const double *Py = ops + glue_ops_index(0, ss.N, dd.N, (op >> 4) & 15);
+    const double *Pz = ops + glue_ops_index(0, ss.N, dd.N, (op >> 8) & 15);
     const TS *q = static_cast<const TS *>(ss.q) + (size_t)t.src_elem[c] * 9 * nps;
     for (int f = 0; f < 9; ++f)
     {

@@ -35,6 +35,9 @@ static int check_glue(const bfamx_dgx_glue_t *g, int ng, int i)
   if (a->partner < 0 || a->partner >= ng)
     return bfamx_fail(BFAMX_EINVAL, "glue %d: partner %d out of range", i, a->partner);
   const bfamx_dgx_glue_t *p = &g[a->partner];
+  if (a->N < 0 || a->N > 7 || p->N != a->N)
+    return bfamx_fail(BFAMX_EINVAL, "glue %d: order %d (partner %d): orders 0..7, the same "
+                                    "on both glues of a pair", i, a->N, p->N);
   if (p->sub_m != a->sub_p || p->sub_p != a->sub_m || p->K != a->K)
     return bfamx_fail(BFAMX_EINVAL,
                       "glue %d (%d -> %d, %d pieces) and its partner %d (%d -> %d, %d "
@@ -96,7 +99,7 @@ int bfamx_mortars_from_glue(const bfamx_dgx_glue_t *glue, int nglue,
         m->b_elem = p->EToEm[kp];
         m->b_face = (int8_t)fp;
         m->quad = (int8_t)(hp ? hp - 1 : -1);
-        m->pad = 0;
+        m->order = (int8_t)a->N;
       }
       ++n;
     }

@@ -390,7 +390,7 @@ int bfamx_octree_build(const bfamx_octree_desc_t *d, bfamx_octree_t **out)
             m->b_elem = o->local_of[j];
             m->b_face = (int8_t)(f ^ 1);
             m->quad = -1;
-            m->pad = 0;
+            m->order = 0;
           }
         }
       }
@@ -424,7 +424,7 @@ int bfamx_octree_build(const bfamx_octree_desc_t *d, bfamx_octree_t **out)
           m->b_elem = o->local_of[i];
           m->b_face = (int8_t)f;
           m->quad = (int8_t)qd;
-          m->pad = 0;
+          m->order = 0;
         }
       }
       else

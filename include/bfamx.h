@@ -354,11 +354,15 @@ typedef struct
  * mortar; face b_face of (b_sub, b_elem) is the same face (quad = -1, the
  * orders differ) or the big face of a 2:1 hanging interface, the mortar
  * being its quadrant quad (bit 0: upper half along the face's first
- * tangential axis, bit 1: along the second) */
+ * tangential axis, bit 1: along the second).  order: the order N_g of the
+ * mortar's glue grid, as bfam's glue_order callback sets it for user glue
+ * (bfam_glue_order_t, bfam.h:1211-1212, applied at bfam.c:7815-7817); 0
+ * takes the reference's default max(N_s, N_b).  Otherwise it must lie in
+ * max(N_s, N_b)..7, as the reference requires N_g >= both (bfam.c:7355). */
 typedef struct
 {
   int32_t s_sub, s_elem, b_sub, b_elem;
-  int8_t s_face, b_face, quad, pad;
+  int8_t s_face, b_face, quad, order;
 } bfamx_mortar_t;
 
 typedef struct bfamx_octree bfamx_octree_t;
@@ -463,13 +467,16 @@ int bfamx_batch_set_glue(bfamx_batch_t *b, bfamx_glue_t *g);
  * K pieces: EToEm / EToFm / EToHm on the minus side, EToEp (the piece's
  * index in the partner glue, the pair (sub_p, sub_m), which is `partner`
  * in this array; itself for a subdomain's own hanging faces), EToHp and
- * EToOp.  The arrays are the reference's own (bfam_locidx_t = int32_t).
+ * EToOp, and its order N (the glue grid's N_g, which bfam's glue_order
+ * callback may have raised above both sides' orders).  The arrays are the
+ * reference's own (bfam_locidx_t = int32_t).
  */
 typedef struct
 {
   int32_t sub_m, sub_p;
   int32_t partner;
   int32_t K;
+  int32_t N; /* the glue subdomain's own order N_g (0: max of the sides) */
   const int32_t *EToEm;
   const int8_t *EToFm;
   const int8_t *EToHm;
@@ -480,7 +487,8 @@ typedef struct
 
 /* mortar records for bfamx_glue_upload from the reference's glue
  * subdomains: one per hanging piece (from its small side) and one per
- * conforming piece between two subdomains.  mortars = NULL only counts.
+ * conforming piece between two subdomains, with the glue's order N.
+ * mortars = NULL only counts.
  * Checks that every piece and its partner name each other with matching
  * hanging numbers on opposite faces; a nonzero orientation (rotated trees)
  * is BFAMX_EUNSUPPORTED. */

@@ -41,7 +41,7 @@ class Problem(C.Structure):
 class Mortar(C.Structure):
     _fields_ = [("s_sub", C.c_int32), ("s_elem", C.c_int32), ("b_sub", C.c_int32),
                 ("b_elem", C.c_int32), ("s_face", C.c_int8), ("b_face", C.c_int8),
-                ("quad", C.c_int8), ("pad", C.c_int8)]
+                ("quad", C.c_int8), ("order", C.c_int8)]
 
 
 def build():
@@ -369,8 +369,9 @@ class OracleMulti:
         m = np.ascontiguousarray(mortars)
         self._m = (Mortar * max(len(m), 1))()
         for i, r in enumerate(m):
+            order = int(r["order"]) if "order" in (r.dtype.names or ()) else 0
             self._m[i] = Mortar(*[int(r[k]) for k in ("s_sub", "s_elem", "b_sub", "b_elem",
-                                                     "s_face", "b_face", "quad")], 0)
+                                                     "s_face", "b_face", "quad")], order)
         self.nm = len(m)
         self._subs = (C.POINTER(Problem) * len(self.problems))(
             *[C.pointer(P.p) for P in self.problems])

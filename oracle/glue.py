@@ -111,10 +111,12 @@ def _recv_key(e):
     return (e["s"], e["ns"], e["k"], e["f"], e["h"])
 
 
-def glue_subdomains(x0, h, sub_of, local_of):
+def glue_subdomains(x0, h, sub_of, local_of, orders=None):
     """one dict per glue subdomain (ordered pair of volume subdomains that
-    share glued faces): sub_m, sub_p and the arrays EToEm, EToFm, EToHm,
-    EToEp, EToHp, EToOp, as bfam_subdomain_dgx_glue_init fills them"""
+    share glued faces): sub_m, sub_p, the glue order N (max of the two
+    sides' orders, bfam.c:7814, when `orders` gives each subdomain's order;
+    else 0) and the arrays EToEm, EToFm, EToHm, EToEp, EToHp, EToOp, as
+    bfam_subdomain_dgx_glue_init fills them"""
     entries = face_mapping(x0, h, sub_of)
     pairs = sorted({(e["s"], e["ns"]) for e in entries})
     out = []
@@ -126,6 +128,7 @@ def glue_subdomains(x0, h, sub_of, local_of):
         es.sort(key=_recv_key)
         out.append({
             "sub_m": s, "sub_p": ns, "K": len(es),
+            "N": max(orders[s], orders[ns]) if orders is not None else 0,
             "EToEm": np.array([local_of[e["k"]] for e in es], np.int32),
             "EToFm": np.array([e["f"] for e in es], np.int8),
             "EToHm": np.array([e["h"] for e in es], np.int8),

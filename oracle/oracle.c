@@ -1298,7 +1298,8 @@ double orc_energy(const orc_problem_t *p, const double *q)
 /* subdomains glued by mortars.  The reference builds the glue operators but
  * has no kernel that applies them; the use restated here follows their
  * wiring in bfam_subdomain_dgx_glue_init (bfam.c:7381-7414): each side's
- * trace is interpolated to the glue grid of order N_g = max(N_m, N_p) with
+ * trace is interpolated to the glue grid of order N_g = max(N_m, N_p) (or the
+ * order a glue_order callback sets, bfam.c:7815-7817) with
  * interpolation{0,3,4} = prj{0,3,4}(N_m -> N_g) (the hanging halves of the
  * big face by refine-to-bottom / -top along each face axis), the upwind flux
  * is evaluated on the glue grid, and the result returns to each side's face
@@ -1377,7 +1378,9 @@ static void mortar_flux(const orc_problem_t *const *subs, const orc_mortar_t *m,
                         double *const *rhs)
 {
   const orc_problem_t *Ps = subs[m->s_sub], *Pb = subs[m->b_sub];
-  const int Ns = Ps->N, Nb = Pb->N, Ng = ORC_MAX(Ns, Nb);
+  const int Ns = Ps->N, Nb = Pb->N;
+  /* N_g = max(N_m, N_p), or the glue_order callback's choice (bfam.c:7815-7817) */
+  const int Ng = m->order > 0 ? m->order : ORC_MAX(Ns, Nb);
   const int ns1 = Ns + 1, nb1 = Nb + 1, ng1 = Ng + 1;
   const int A = m->s_face / 2;
   const double sg = (m->s_face & 1) ? 1.0 : -1.0; /* n_s = sg e_A, n_b = -n_s */

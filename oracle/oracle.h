@@ -162,7 +162,7 @@ void orc_lsrk_run(const orc_problem_t *p, int method, double t0, double dt,
 typedef struct
 {
   int32_t s_sub, s_elem, b_sub, b_elem;
-  int8_t s_face, b_face, quad, pad;
+  int8_t s_face, b_face, quad, order; /* order: glue grid N_g, 0 = max */
 } orc_mortar_t;
 /* q[s], rhs[s]: the field blocks of subdomain s */
 void orc_rhs_multi(const orc_problem_t *const *subs, int nsub,

@@ -150,6 +150,37 @@ def test_glue_ingested_from_reference_arrays(dev):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32])
+def test_glued_rhs_raised_glue_order(dev, prec):
+    """Mortars on glue grids of order 7 (bfam's glue_order callback) against
+    the oracle with the same orders; an order below both sides' is refused."""
+    print(f"glue order 7 prec={prec}", flush=True)
+    cfg = octree_cfg()
+    m = cfg["mortars"].copy()
+    m["order"] = 7
+    cfg = dict(cfg, mortars=m)
+    q0, subs, glue, b = setup(dev, cfg, prec, 10)
+    t = 0.2 * cfg["src"]["t0"]
+    got = b.rhs(t)
+    teardown(subs, glue, b)
+    want = oracle_multi(cfg).rhs(q0, t)
+    assert rel_max_multi(got, want) < TOL[prec]
+    # the default order gives the same flux up to rounding, but not the same bits
+    q0, subs, glue, b = setup(dev, octree_cfg(), prec, 10)
+    base = b.rhs(t)
+    teardown(subs, glue, b)
+    assert rel_max_multi(got, base) < TOL[prec]
+    assert any(not np.array_equal(x, y) for x, y in zip(got, base))
+    bad = m.copy()
+    bad["order"][0] = 3  # below the big side's order 4 or 5
+    subs = W.upload_multi(dev, cfg, prec)
+    with pytest.raises(bx.BfamxError) as e:
+        bx.Glue(dev, subs, bad)
+    assert e.value.code == bx.EINVAL
+    for s in subs:
+        s.close()
+
+
 def test_glued_members_must_stay_in_step(dev):
     """The mortars couple the members' traces at every stage, so a glued
     batch refuses to step when a member was reset to another time, step size

@@ -246,3 +246,25 @@ def test_conforming_mixed_order_glue():
     assert np.abs(tot).max() < 1e-12 * mag
     qc = [np.ones((9, sm.K * sm.Np)) for sm in cfg["subs"]]
     assert max(np.abs(r).max() for r in P.rhs(qc, 0.0)) < 1e-11
+
+
+@pytest.mark.parametrize("order", [5, 7])
+def test_glue_order_leaves_a_linear_flux_unchanged(small, order):
+    """A raised glue grid order (bfam's glue_order callback): the traces are
+    polynomials of degree <= max(N_s, N_b) on each mortar and the upwind flux
+    is linear in them with constant coefficients there, so interpolating to
+    any N_g >= max(N_s, N_b) and projecting back with the exact mass gives the
+    same face values: the RHS must not change beyond rounding, while a wrong
+    operator for the raised order would change it at O(1)."""
+    cfg = dict(small)
+    rng = np.random.default_rng(17)
+    qs = [rng.standard_normal((9, sm.K * sm.Np)) for sm in cfg["subs"]]
+    base = oracle_multi(cfg).rhs(qs, 0.1)
+    m = cfg["mortars"].copy()
+    m["order"] = order
+    raised = oracle_multi(dict(cfg, mortars=m)).rhs(qs, 0.1)
+    scale = max(np.abs(x).max() for x in base)
+    assert max(np.abs(x - y).max() for x, y in zip(base, raised)) < 1e-12 * scale
+    # the raised order is used: the rounding of the longer sums differs
+    assert any(not np.array_equal(x, y) for x, y in zip(base, raised))
+
