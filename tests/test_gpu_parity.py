@@ -224,6 +224,29 @@ def test_mms_forcing(dev):
     assert np.sqrt(d2 / n2) < 1e-3
 
 
+@pytest.mark.parametrize("prec", [bx.FP64, bx.FP32_RHS, bx.FP32])
+def test_mms_order_7(dev, kernel, prec):
+    """The precision-study configuration's order (N = 7) on a 3^3 brick: the
+    forced RHS and 10 LSRK4(5) steps of the manufactured solution against
+    the oracle in every fused precision and with both stage kernels (the
+    one-element-per-block kernel instances), and the error against the exact
+    solution."""
+    print(f"mms 3^3 N=7 prec={prec} kernel={kernel}", flush=True)
+    cfg = W.elastic3d_mms(n=3, N=7)
+    m = cfg["mesh"]
+    q0 = W.mms_exact(m, cfg["A"], cfg["k"], cfg["omega"], 0.0)
+    P = oracle_problem(cfg)
+    sub = W.upload(dev, cfg, prec)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    assert rel_max(sub.rhs(0.1), P.rhs(q0, 0.1)) < TOL[prec]
+    sub.step(10)
+    d2, n2 = sub.mms_error(cfg["A"])
+    q = sub.download()
+    sub.close()
+    assert rel_max(q, P.run(q0, cfg["dt"], 10)) < TOL[prec]
+    assert np.sqrt(d2 / n2) < 1e-3
+
+
 def test_full_size_rhs_against_oracle(dev):
     """The 48^3 N=7 elastic brick at full size: one FP64 right-hand side
     evaluation (with the Ricker source active) against the oracle on the
