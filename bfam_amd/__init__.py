@@ -153,6 +153,7 @@ SIGNATURES = {
     "bfamx_download_fields": (C.c_int, [_vp, _PP]),
     "bfamx_get_time": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "bfamx_rhs": (C.c_int, [_vp, C.c_double, _PP]),
+    "bfamx_rhs_noalloc": (C.c_int, [_vp, C.c_double, _PP]),
     "bfamx_stage": (C.c_int, [_vp]),
     "bfamx_step": (C.c_int, [_vp, C.c_int]),
     "bfamx_reduce_energy": (C.c_int, [_vp, C.POINTER(C.c_double)]),
@@ -480,6 +481,14 @@ class Subdomain:
         out = np.zeros((self.nf, self.K * self.Np))
         rows = [out[f] for f in range(self.nf)]
         _check(lib().bfamx_rhs(self.h, float(t), _ptrs(rows)))
+        return out
+
+    def rhs_noalloc(self, t=0.0):
+        """bfamx_rhs_noalloc: the RHS staged in the library's fixed device
+        array instead of a buffer allocated on first use"""
+        out = np.zeros((self.nf, self.K * self.Np))
+        rows = [out[f] for f in range(self.nf)]
+        _check(lib().bfamx_rhs_noalloc(self.h, float(t), _ptrs(rows)))
         return out
 
     def stage(self):

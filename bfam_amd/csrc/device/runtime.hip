@@ -55,7 +55,15 @@ struct bfamx_device
    * from and joined back to `stream` (parallel branches when captured) */
   hipStream_t side[8];
   hipEvent_t fork_ev, join_ev[8];
+  /* device address of this GPU's copy of `scratch` (bfamx_rhs_noalloc) */
+  double *scratch;
 };
+
+/* Staging buffer of bfamx_rhs_noalloc: one copy per GPU, held until the
+ * process ends.  Host code uses only the per-device address that
+ * hipGetSymbolAddress returns (bfamx_device::scratch), never this name. */
+constexpr size_t RHS_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[RHS_SCRATCH_BYTES / sizeof(double)];
 
 struct bfamx_subdomain
 {
@@ -178,6 +186,20 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
   d->persistent = 0;
   memset(d->line_bpc, 0, sizeof(d->line_bpc));
   memset(d->kchoice, -1, sizeof(d->kchoice));
+  /* this device's copy of the no-allocation staging buffer, looked up once
+   * with the device current */
+  d->scratch = NULL;
+  {
+    void *ptr = NULL;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+    {
+      delete d;
+      return bfamx_fail(BFAMX_EHIP, "bfamx_device_create: scratch lookup failed: %s",
+                        hipGetErrorString(e));
+    }
+    d->scratch = (double *)ptr;
+  }
   /* default priority only: streams of different priorities have hung on
    * this hardware */
   HIPCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
@@ -1080,6 +1102,67 @@ extern "C" int bfamx_rhs(bfamx_subdomain_t *s, double t, double *const *rhs)
   return rhs_one(s, t, rhs);
 }
 
+/* the MODE_RHS launch of one subdomain, writing L(q, t) as FP64 into the
+ * buffer the descriptor's rhs names */
+static int rhs_launch(bfamx_subdomain_t *s, double t)
+{
+  LaunchArgs a = launch_args(s, MODE_RHS);
+  a.t_rhs = t;
+  a.g_rhs = s->h.src_elem >= 0 ? host_ricker(s->h.src_f0, s->h.src_t0, t) : 0.0;
+  a.sw_rhs = s->h.mms ? sin(s->h.mms_omega * t) : 0.0;
+  a.cw_rhs = s->h.mms ? cos(s->h.mms_omega * t) : 0.0;
+  return s->prec == BFAMX_FP64_PLAIN ? run_plain_rhs(s, a) : run_fused(s, a);
+}
+
+/*
+ * bfamx_rhs without any allocation: the RHS is staged in this GPU's copy
+ * of `scratch` (64 MiB) instead of the subdomain's lazily allocated FP64
+ * buffer, and copied field by field into the caller's arrays with strided
+ * copies.  The RHS kernels write every entry of the staging area they use
+ * (the unfused volume kernel assigns before the surface kernel adds), so
+ * nothing needs clearing.  Calls that overlap on one GPU share the buffer
+ * and race.
+ */
+extern "C" int bfamx_rhs_noalloc(bfamx_subdomain_t *s, double t, double *const *rhs)
+{
+  if (!s || !rhs)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_rhs_noalloc: NULL argument");
+  for (int f = 0; f < s->nf; ++f)
+    if (!rhs[f])
+      return bfamx_fail(BFAMX_EINVAL, "bfamx_rhs_noalloc: field %d is NULL", f);
+  if (!s->fields_set)
+    return bfamx_fail(BFAMX_ESTATE, "bfamx_rhs_noalloc: no state set");
+  if (s->glue)
+    return bfamx_fail(BFAMX_ESTATE, "glued subdomains: use bfamx_batch_rhs");
+  const size_t Np = s->Np, nf = s->nf, K = s->K;
+  const size_t ndof = K * nf * Np;
+  if (8 * ndof > RHS_SCRATCH_BYTES)
+    return bfamx_fail(BFAMX_EINVAL,
+                      "bfamx_rhs_noalloc: %zu FP64 values exceed the %zu-byte scratch "
+                      "(at most %zu values)", ndof, RHS_SCRATCH_BYTES,
+                      RHS_SCRATCH_BYTES / sizeof(double));
+  double *const stage = s->dev->scratch;
+  double *const keep = s->h.rhs;
+  /* the descriptor changes under no launch of this subdomain */
+  HIPCHK(hipStreamSynchronize(s->dev->stream));
+  s->h.rhs = stage;
+  int rc = push_sub(s);
+  if (!rc)
+    rc = rhs_launch(s, t);
+  hipError_t e = hipSuccess;
+  if (!rc)
+    e = hipStreamSynchronize(s->dev->stream);
+  /* [K][nf][Np] on the device to rhs[f][K * Np] on the host */
+  for (size_t f = 0; f < nf && !rc && e == hipSuccess; ++f)
+    e = hipMemcpy2D(rhs[f], Np * sizeof(double), stage + f * Np, nf * Np * sizeof(double),
+                    Np * sizeof(double), K, hipMemcpyDeviceToHost);
+  s->h.rhs = keep;
+  const int rc2 = push_sub(s);
+  if (!rc && e != hipSuccess)
+    rc = bfamx_fail(BFAMX_EHIP, "bfamx_rhs_noalloc: %s", hipGetErrorString(e));
+  return rc ? rc : rc2;
+}
+
 /* RHS of one subdomain into host arrays; glued faces use the glue buffer as
  * the last mortar launch left it */
 static int rhs_one(bfamx_subdomain_t *s, double t, double *const *rhs)
@@ -1097,12 +1180,7 @@ static int rhs_one(bfamx_subdomain_t *s, double t, double *const *rhs)
     if (rc)
       return rc;
   }
-  LaunchArgs a = launch_args(s, MODE_RHS);
-  a.t_rhs = t;
-  a.g_rhs = s->h.src_elem >= 0 ? host_ricker(s->h.src_f0, s->h.src_t0, t) : 0.0;
-  a.sw_rhs = s->h.mms ? sin(s->h.mms_omega * t) : 0.0;
-  a.cw_rhs = s->h.mms ? cos(s->h.mms_omega * t) : 0.0;
-  rc = s->prec == BFAMX_FP64_PLAIN ? run_plain_rhs(s, a) : run_fused(s, a);
+  rc = rhs_launch(s, t);
   if (rc)
     return rc;
   HIPCHK(hipStreamSynchronize(s->dev->stream));

@@ -258,6 +258,16 @@ int bfamx_get_time(bfamx_subdomain_t *sub, double *t, int64_t *stages);
 
 /* L(q, t) of the current state into host arrays (parity checks) */
 int bfamx_rhs(bfamx_subdomain_t *sub, double t, double *const *rhs);
+/*
+ * bfamx_rhs allocating nothing (no counterpart in the reference): for
+ * callers that cannot allocate device memory, such as tight serving loops
+ * or harnesses that forbid it.  bfamx_rhs stages the FP64 RHS in a device
+ * buffer it allocates on first use; this variant stages it in a fixed
+ * 64 MiB device array of the library (one per GPU, held until the process
+ * ends), so K * nfields * Np must be at most 8,388,608 (BFAMX_EINVAL
+ * otherwise).  Calls that overlap on one GPU share that array and race.
+ */
+int bfamx_rhs_noalloc(bfamx_subdomain_t *sub, double t, double *const *rhs);
 /* one LSRK stage / nsteps full steps with the subdomain's dt */
 int bfamx_stage(bfamx_subdomain_t *sub);
 int bfamx_step(bfamx_subdomain_t *sub, int nsteps);
