@@ -201,6 +201,12 @@ SIGNATURES = {
     "bfamx_request_bytes": (C.c_size_t, [C.POINTER(Request), C.c_int]),
     "bfamx_request_build": (C.c_int, [_vp, C.POINTER(Request), C.c_int, _PP,
                                       C.POINTER(C.c_double)]),
+    "bfamx_request_inputs": (C.c_int, [C.POINTER(Request), _vp, _vp, _vp, C.POINTER(C.c_int32),
+                                       _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
+    "bfamx_serve_cb": (C.c_int, [_vp, C.POINTER(Request), C.c_int, C.c_int, C.c_size_t, C.c_int,
+                                 C.c_void_p, C.c_void_p, C.POINTER(RequestResult),
+                                 C.POINTER(ServeStats)]),
     "bfamx_serve": (C.c_int, [_vp, C.POINTER(Request), C.c_int, C.c_int, C.c_size_t, C.c_int,
                               C.POINTER(RequestResult), C.POINTER(ServeStats)]),
 }
@@ -769,14 +775,41 @@ def request_build(dev, req, precision):
     return Subdomain._adopt(dev, h, ELASTIC3D, precision, req.N, req.n ** 3), dt.value
 
 
-def serve(dev, reqs, precision=FP32_RHS, budget_bytes=32 << 30, max_chunk=10):
-    """Run a request stream in real time; returns (per-request dicts, stats)."""
+def request_inputs(req):
+    """The request's material, source and dt as bfamx_request_build sets
+    them up (bfamx_request_inputs), without a device."""
+    K, Np = req.n ** 3, (req.N + 1) ** 3
+    rho, cp, cs, w = np.zeros(K), np.zeros(K), np.zeros(K), np.zeros(Np)
+    e = C.c_int32()
+    f0, t0, dt = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().bfamx_request_inputs(C.byref(req), rho.ctypes.data, cp.ctypes.data,
+                                      cs.ctypes.data, C.byref(e), w.ctypes.data, C.byref(f0),
+                                      C.byref(t0), C.byref(dt)))
+    return {"rho": rho, "cp": cp, "cs": cs, "src_elem": e.value, "src_weights": w,
+            "amp": [0, 0, 0, 1, 1, 1, 0, 0, 0], "f0": f0.value, "t0": t0.value, "dt": dt.value}
+
+
+SERVE_DONE = C.CFUNCTYPE(None, C.c_int, C.c_void_p, C.c_void_p)
+
+
+def serve(dev, reqs, precision=FP32_RHS, budget_bytes=32 << 30, max_chunk=10, on_done=None):
+    """Run a request stream in real time; returns (per-request dicts, stats).
+    on_done(index, subdomain), if given, sees each finished request's
+    subdomain at its final state (bfamx_serve_cb; valid during the call)."""
     n = len(reqs)
     arr = (Request * max(n, 1))(*reqs)
     res = (RequestResult * max(n, 1))()
     st = ServeStats()
-    _check(lib().bfamx_serve(dev.h, arr, n, precision, budget_bytes, max_chunk, res,
-                             C.byref(st)))
+    cb = None
+    if on_done is not None:
+        def _done(i, h, _user):
+            q = reqs[i]
+            on_done(i, Subdomain._adopt(dev, C.c_void_p(h), ELASTIC3D, precision, q.N,
+                                        q.n ** 3))
+        cb = SERVE_DONE(_done)
+    _check(lib().bfamx_serve_cb(dev.h, arr, n, precision, budget_bytes, max_chunk,
+                                C.cast(cb, C.c_void_p) if cb else None, None, res,
+                                C.byref(st)))
     out = [{k: getattr(res[i], k) for k, _ in RequestResult._fields_} for i in range(n)]
     stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
     return out, stats

@@ -59,7 +59,8 @@ def build(jobs=None, verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
+    _run(["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-lz", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -584,11 +584,29 @@ size_t bfamx_request_bytes(const bfamx_request_t *rq, bfamx_precision_t prec);
 int bfamx_request_build(bfamx_device_t *dev, const bfamx_request_t *rq,
                         bfamx_precision_t prec, bfamx_subdomain_t **out,
                         double *dt);
+/* the request's inputs as bfamx_request_build sets them up, without a
+ * device: per-element material (K = n^3, Morton order of bfamx_brick), the
+ * source element and its (N+1)^3 weights (amplitudes 1 on the three normal
+ * stresses), the Ricker f0 and t0, and dt; any output may be NULL */
+int bfamx_request_inputs(const bfamx_request_t *rq, double *rho, double *cp,
+                         double *cs, int32_t *src_elem, double *src_weights,
+                         double *f0, double *t0, double *dt);
 /* run the stream in real time (arrivals sorted), advancing at most
- * max_chunk steps between admissions; res[i] for reqs[i] */
+ * max_chunk steps between admissions; res[i] for reqs[i].  A host thread
+ * prepares each arrived request's mesh, material and source while the
+ * device steps, so that admission costs only the upload. */
 int bfamx_serve(bfamx_device_t *dev, const bfamx_request_t *reqs, int nreq,
                 bfamx_precision_t prec, size_t budget_bytes, int max_chunk,
                 bfamx_request_result_t *res, bfamx_serve_stats_t *stats);
+/* called on the serving thread when request `index` has finished, with its
+ * subdomain at the final state (valid during the call only: e.g. download
+ * its fields); the energy in res is already set */
+typedef void (*bfamx_serve_done_t)(int index, bfamx_subdomain_t *sub, void *user);
+/* bfamx_serve handing each finished request to done (may be NULL) */
+int bfamx_serve_cb(bfamx_device_t *dev, const bfamx_request_t *reqs, int nreq,
+                   bfamx_precision_t prec, size_t budget_bytes, int max_chunk,
+                   bfamx_serve_done_t done, void *user,
+                   bfamx_request_result_t *res, bfamx_serve_stats_t *stats);
 
 #ifdef __cplusplus
 }

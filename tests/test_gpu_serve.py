@@ -6,6 +6,8 @@ import numpy as np
 import pytest
 
 import bfam_amd as bx
+import bfam_amd.workloads as W
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -60,3 +62,57 @@ def test_serve_rejects_oversized_request(dev):
     with pytest.raises(bx.BfamxError) as e:
         bx.serve(dev, reqs, bx.FP32_RHS, budget_bytes=1000)
     assert e.value.code == bx.EBUDGET
+
+
+def alone_state(dev, req, prec):
+    sub, _ = bx.request_build(dev, req, prec)
+    sub.step(req.nsteps)
+    q = sub.download()
+    sub.close()
+    return q
+
+
+@pytest.mark.parametrize("prec", [bx.FP32_RHS, bx.FP32])
+def test_served_states_equal_alone_with_an_8cube_request(dev, prec):
+    """The final state of every served request, handed over by the
+    completion callback, equals the same request run alone bit for bit, in a
+    mix that holds an 8^3 request beside smaller ones of other orders."""
+    print(f"serve: final states alone vs mix with an 8^3 request, prec={prec}", flush=True)
+    reqs = bx.serve_generate(seed=5, nreq=8, rate_per_s=500.0, n=(3, 5), N=(3, 7),
+                             steps=(8, 20))
+    reqs[2].n = 8
+    states = {}
+    res, st = bx.serve(dev, reqs, prec, budget_bytes=1 << 30, max_chunk=4,
+                       on_done=lambda i, sub: states.__setitem__(i, sub.download()))
+    assert st["max_active"] >= 3 and sorted(states) == list(range(len(reqs)))
+    for i, q in enumerate(reqs):
+        assert np.array_equal(states[i], alone_state(dev, q, prec)), i
+
+
+@pytest.mark.parametrize("prec", [bx.FP32_RHS, bx.FP32])
+def test_served_request_matches_the_oracle(dev, prec):
+    """A served request's final state against the oracle's run of the same
+    problem (the request's mesh, layered material, Ricker source and dt, as
+    bfamx_request_inputs reports them): FP32 arithmetic to 2e-5 of the
+    largest value."""
+    print(f"serve: served request vs oracle, prec={prec}", flush=True)
+    reqs = bx.serve_generate(seed=11, nreq=3, rate_per_s=500.0, n=(3, 4), N=(3, 5),
+                             steps=(15, 25))
+    states = {}
+    bx.serve(dev, reqs, prec, budget_bytes=1 << 30, max_chunk=5,
+             on_done=lambda i, sub: states.__setitem__(i, sub.download()))
+    for i, rq in enumerate(reqs):
+        inp = bx.request_inputs(rq)
+        mesh = W.Mesh(3, [rq.n] * 3, rq.N)
+        rho, cp, cs = inp["rho"], inp["cp"], inp["cs"]
+        lam, mu = rho * (cp * cp - 2 * cs * cs), rho * cs * cs
+        src = {"elem": inp["src_elem"], "weights": inp["src_weights"], "amp": inp["amp"],
+               "f0": inp["f0"], "t0": inp["t0"]}
+        P = oracle.OracleProblem(oracle.ELASTIC3D, rq.N, mesh.vmapM, mesh.vmapP, mesh.h, rho,
+                                 lam, mu, x0=mesh.x0, src=src,
+                                 ops=(mesh.r, mesh.w, 1 / mesh.w, mesh.Dr))
+        want = P.run(np.zeros((9, mesh.K * mesh.Np)), inp["dt"], rq.nsteps)
+        scale = np.abs(want).max()
+        assert scale > 0
+        assert np.abs(states[i] - want).max() < 2e-5 * scale, i
+
