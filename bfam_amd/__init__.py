@@ -17,8 +17,9 @@ LIB_PATH = os.path.join(_HERE, "libbfamx.so")
 
 ACOUSTIC2D, ELASTIC3D = 0, 1
 BC_NONE, BC_FREE, BC_RIGID, BC_ABSORB = 0, 1, 2, 3
-FP64, FP64_PLAIN, FP32_RHS, FP32 = 0, 1, 2, 3
-PRECISIONS = {"fp64": FP64, "fp64_plain": FP64_PLAIN, "fp32_rhs": FP32_RHS, "fp32": FP32}
+FP64, FP64_PLAIN, FP32_RHS, FP32, FP32_BF16X3 = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": FP64, "fp64_plain": FP64_PLAIN, "fp32_rhs": FP32_RHS, "fp32": FP32,
+              "fp32_bf16x3": FP32_BF16X3}
 LSRK_KC54, LSRK_FE, LSRK_HEUN, LSRK_W33, LSRK_NOOP = 0, 1, 2, 3, 4
 OK, EINVAL, EBUDGET, EHIP, ESTATE, EUNSUPPORTED = 0, 1, 2, 3, 4, 5
 FLAG_SAME, FLAG_COARSEN, FLAG_REFINE = 0, 1, 2

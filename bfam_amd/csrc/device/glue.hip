@@ -255,6 +255,7 @@ hipError_t launch_mortar(int prec, const GlueArgs &a, hipStream_t s)
                        dim3(64 * GLUE_WAVES), 0, s, a);
     break;
   case PREC_FP32:
+  case PREC_FP32_BF16X3:
     hipLaunchKernelGGL((mortar_kernel<float, float>), dim3(a.ngroups),
                        dim3(64 * GLUE_WAVES), 0, s, a);
     break;

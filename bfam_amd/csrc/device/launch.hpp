@@ -40,8 +40,10 @@ enum
   PREC_FP64 = 0,
   PREC_FP64_PLAIN = 1,
   PREC_FP32_RHS = 2,
-  PREC_FP32 = 3
+  PREC_FP32 = 3,
+  PREC_FP32_BF16X3 = 4
 };
+constexpr int NPREC = 5;
 
 /* fused stage or RHS-only launch over a task table.  N < 0: mixed orders
  * (one kernel handles every N in 3..7).  grid: number of workgroups. */

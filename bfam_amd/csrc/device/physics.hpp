@@ -8,15 +8,18 @@
 namespace bfamx
 {
 
-template <class S, class R, class Cc> struct Prec
+template <class S, class R, class Cc, bool M = false> struct Prec
 {
   using TS = S; /* state */
   using TR = R; /* rate register */
   using TC = Cc; /* arithmetic */
+  /* derivative contractions in split-BF16x3 on the matrix cores */
+  static constexpr bool MF = M;
 };
 using P64 = Prec<double, double, double>;
 using P32R = Prec<double, float, float>;
 using P32 = Prec<float, float, float>;
+using P32M = Prec<float, float, float, true>;
 
 /* per-precision copies of the element tables, as global pointers; SD is
  * SubDev, possibly qualified with the constant address space */

@@ -37,7 +37,7 @@ struct bfamx_device
   /* mixed-order launch strategy chosen by bfamx_batch_autotune, per
    * precision and set of orders present (bit N): -1 unknown, 0 one mixed
    * launch, 1 one launch per order */
-  signed char mixed_choice[4][256];
+  signed char mixed_choice[NPREC][256];
   /* elastic stage launches with at most the resident wavefronts, each
    * looping over tasks and loading the next task's state during the
    * current task's output.  Off by default: on gfx950 the wait for the
@@ -45,12 +45,12 @@ struct bfamx_device
    * loads and stores), and the FP32 48^3 N=7 stage measured 2.75 ms against
    * 2.34 ms with one wavefront per task */
   int persistent;
-  int line_bpc[4][9]; /* cached blocks per CU, 0 unknown */
+  int line_bpc[NPREC][9]; /* cached blocks per CU, 0 unknown */
   /* elastic stage kernel per precision and order, chosen by
    * bfamx_batch_autotune: -1 unknown (line), 0 line (one wavefront per
    * element, lanes own lines), 1 node-parallel (512-thread blocks, threads
    * own nodes) */
-  signed char kchoice[4][9];
+  signed char kchoice[NPREC][9];
   /* the per-order launches of a grouped batch run on side streams forked
    * from and joined back to `stream` (parallel branches when captured) */
   hipStream_t side[8];
@@ -356,7 +356,7 @@ static int order_supported(int physics, int N)
  * dq_{s-1} = (q_s - q_{s-1}) / B_{s-1}) */
 static int rebuilds_dq(int prec, const bfamx_lsrk_t &t)
 {
-  if (prec != BFAMX_FP32 && prec != BFAMX_FP64)
+  if (prec != BFAMX_FP32 && prec != BFAMX_FP64 && prec != BFAMX_FP32_BF16X3)
     return 0;
   if (t.A[0] != 0.0)
     return 0;
@@ -387,7 +387,8 @@ extern "C" size_t bfamx_subdomain_bytes_lsrk(int dim, int N, int32_t K,
     t = *lsrk;
   else
     bfamx_lsrk_init(&t, BFAMX_LSRK_KC54);
-  const int dq_stored = (prec == BFAMX_FP32 || prec == BFAMX_FP64) && !rebuilds_dq(prec, t);
+  const int dq_stored = (prec == BFAMX_FP32 || prec == BFAMX_FP64 || prec == BFAMX_FP32_BF16X3) &&
+                        !rebuilds_dq(prec, t);
   return subdomain_bytes(dim, N, K, ph, prec, dq_stored);
 }
 
@@ -460,8 +461,10 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   if (!order_supported(physics, N))
     return bfamx_fail(BFAMX_EUNSUPPORTED, "order N=%d not built for physics %d",
                       N, physics);
-  if (prec < BFAMX_FP64 || prec > BFAMX_FP32)
+  if (prec < BFAMX_FP64 || prec > BFAMX_FP32_BF16X3)
     return bfamx_fail(BFAMX_EINVAL, "unknown precision %d", prec);
+  if (prec == BFAMX_FP32_BF16X3 && physics != BFAMX_ELASTIC3D)
+    return bfamx_fail(BFAMX_EUNSUPPORTED, "split-BF16x3 derivatives: elastic only");
   const int n1 = N + 1;
   const int Np = dim == 2 ? n1 * n1 : n1 * n1 * n1;
   if (sub->Np != Np || sub->K <= 0 || !sub->r || !sub->w || !sub->Dr ||
@@ -554,8 +557,8 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
   s->Np = Np;
   s->nf = nf;
   s->K = K;
-  s->state_fp32 = prec == BFAMX_FP32;
-  s->rate_fp32 = prec == BFAMX_FP32 || prec == BFAMX_FP32_RHS;
+  s->state_fp32 = prec == BFAMX_FP32 || prec == BFAMX_FP32_BF16X3;
+  s->rate_fp32 = s->state_fp32 || prec == BFAMX_FP32_RHS;
   s->bytes = 0;
   s->d_rhs = NULL;
   s->fields_set = 0;
@@ -995,7 +998,10 @@ static int node_kernel(const bfamx_device_t *d, int prec, int N)
 {
   if (N < 3 || N > 7)
     return 0;
-  const signed char k = d->kchoice[prec & 3][N];
+  /* the matrix-core derivatives exist in the node-parallel kernel only */
+  if (prec == PREC_FP32_BF16X3)
+    return 1;
+  const signed char k = d->kchoice[prec][N];
   return k < 0 ? default_node(prec, N) : k == 1;
 }
 
@@ -2049,8 +2055,8 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   if (b->exec)
     return bfamx_fail(BFAMX_ESTATE, "bfamx_batch_autotune: batch already captured");
   bfamx_device_t *d = b->dev;
-  signed char *kc = d->kchoice[b->prec & 3];
-  signed char &mc = d->mixed_choice[b->prec & 3][b->order_mask & 255];
+  signed char *kc = d->kchoice[b->prec];
+  signed char &mc = d->mixed_choice[b->prec][b->order_mask & 255];
   int need = 0;
   for (int n = 0; n < 9; ++n)
     if (b->grp_cnt[n] && (kc[n] < 0 || force))
@@ -2145,6 +2151,11 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
   {
     if (!b->grp_cnt[n] || (kc[n] >= 0 && !force))
       continue;
+    if (b->prec == PREC_FP32_BF16X3)
+    {
+      kc[n] = 1; /* the only kernel with matrix-core derivatives */
+      continue;
+    }
     float ms[2] = {0, 0};
     for (int k = 0; k < 2 && !rc; ++k)
     {
@@ -2189,15 +2200,18 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
 
 extern "C" int bfamx_set_kernel(bfamx_device_t *d, int prec, int N, int kind)
 {
-  if (!d || prec < 0 || prec > 3 || N < 0 || N > 8 || kind < -1 || kind > 1)
+  if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || kind < -1 || kind > 1)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_set_kernel: bad arguments");
+  if (prec == PREC_FP32_BF16X3 && kind == 0)
+    return bfamx_fail(BFAMX_EUNSUPPORTED,
+                      "bfamx_set_kernel: split-BF16x3 runs the node-parallel kernel only");
   d->kchoice[prec][N] = (signed char)kind;
   return BFAMX_OK;
 }
 
 extern "C" int bfamx_get_kernel(const bfamx_device_t *d, int prec, int N, int *kind)
 {
-  if (!d || prec < 0 || prec > 3 || N < 0 || N > 8 || !kind)
+  if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || !kind)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_get_kernel: bad arguments");
   *kind = node_kernel(d, prec, N);
   return BFAMX_OK;
@@ -2210,7 +2224,7 @@ static int grid_line(bfamx_device_t *d, int prec, int N, int ntasks)
   if (!d->persistent)
     return g;
   const int ni = N < 0 ? 8 : N;
-  int &bpc = d->line_bpc[prec & 3][ni];
+  int &bpc = d->line_bpc[prec][ni];
   if (bpc == 0)
   {
     int b = 0;

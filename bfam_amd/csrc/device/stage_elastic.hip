@@ -33,6 +33,7 @@
  * tables are read through the constant address space (scalar loads).
  */
 #include "elastic_axis.hpp"
+#include "mfma_deriv.hpp"
 #include "physics.hpp"
 
 namespace bfamx
@@ -49,6 +50,13 @@ template <int N, int EPB, class P> struct NodeDq
 };
 template <int N, int EPB, class P>
 using NodeSmem = ElasticSmem<N, EPB, typename P::TC, NodeDq<N, EPB, P>::value>;
+/* LDS of a block: the regions above, and for the matrix-core derivatives
+ * the six derivative fields of one axis (FP32) */
+template <int N, int EPB, class P> struct NodeLds
+{
+  static constexpr int MFW = P::MF ? 6 * EPB * (N + 1) * (N + 1) * (N + 1) : 0;
+  static constexpr size_t bytes = NodeSmem<N, EPB, P>::bytes + sizeof(float) * MFW;
+};
 
 /*
  * Block-wide copy of `count` contiguous 32-bit words from global memory to
@@ -258,10 +266,75 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     }
     __syncthreads();
   }
+
+  TC rate[NF];
+  /* matrix-core derivatives: every thread of the block takes part (the
+   * tiles go to the full wavefronts), one axis at a time through a six-field
+   * LDS buffer; each node then folds its six values of the axis into the
+   * rates exactly as the contraction below does */
+  if constexpr (P::MF && PART == PART_FUSED)
+  {
+    float *der = reinterpret_cast<float *>(smem + NodeSmem<N, EPB, P>::bytes);
+    EP M0 = (EP)mat_tab<TC>(sd) + (size_t)(act ? e : tk.e0) * MAT_STRIDE;
+    EP G0 = (EP)geo_tab<TC>(sd) + (size_t)(act ? e : tk.e0) * GEO_STRIDE;
+    const TC lam = M0[MAT_LAM], mu = M0[MAT_MU], l2m = lam + TC(2) * mu;
+    const float *dr = der + le * Np + n;
+    constexpr int SW = EPB * Np;
+    {
+      constexpr int fx[6] = {0, 1, 2, 3, 6, 7};
+      mf_axis<N, EPB, 0>(sq, sD, der, fx);
+      __syncthreads();
+      const TC g = G0[GEO_GX];
+      const TC v1 = g * dr[0], v2 = g * dr[SW], v3 = g * dr[2 * SW], s11 = g * dr[3 * SW],
+               s12 = g * dr[4 * SW], s13 = g * dr[5 * SW];
+      rate[0] = s11;
+      rate[1] = s12;
+      rate[2] = s13;
+      rate[3] = l2m * v1;
+      rate[4] = lam * v1;
+      rate[5] = lam * v1;
+      rate[6] = mu * v2;
+      rate[7] = mu * v3;
+      rate[8] = 0;
+      __syncthreads();
+    }
+    {
+      constexpr int fy[6] = {0, 1, 2, 4, 6, 8};
+      mf_axis<N, EPB, 1>(sq, sD, der, fy);
+      __syncthreads();
+      const TC g = G0[GEO_GY];
+      const TC v1 = g * dr[0], v2 = g * dr[SW], v3 = g * dr[2 * SW], s22 = g * dr[3 * SW],
+               s12 = g * dr[4 * SW], s23 = g * dr[5 * SW];
+      rate[0] += s12;
+      rate[1] += s22;
+      rate[2] += s23;
+      rate[3] += lam * v2;
+      rate[4] += l2m * v2;
+      rate[5] += lam * v2;
+      rate[6] += mu * v1;
+      rate[8] += mu * v3;
+      __syncthreads();
+    }
+    {
+      constexpr int fz[6] = {0, 1, 2, 5, 7, 8};
+      mf_axis<N, EPB, 2>(sq, sD, der, fz);
+      __syncthreads();
+      const TC g = G0[GEO_GZ];
+      const TC v1 = g * dr[0], v2 = g * dr[SW], v3 = g * dr[2 * SW], s33 = g * dr[3 * SW],
+               s13 = g * dr[4 * SW], s23 = g * dr[5 * SW];
+      rate[0] += s13;
+      rate[1] += s23;
+      rate[2] += s33;
+      rate[3] += lam * v3;
+      rate[4] += lam * v3;
+      rate[5] += l2m * v3;
+      rate[7] += mu * v1;
+      rate[8] += mu * v2;
+    }
+  }
   if (!act)
     return;
 
-  TC rate[NF];
   EP M = (EP)mat_tab<TC>(sd) + (size_t)e * MAT_STRIDE;
   const TC rhoi = M[MAT_RHOI], lam = M[MAT_LAM], mu = M[MAT_MU];
   EP G = (EP)geo_tab<TC>(sd) + (size_t)e * GEO_STRIDE;
@@ -271,6 +344,12 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
 #pragma unroll
     for (int f = 0; f < NF; ++f)
       rate[f] = (TC)rh[f * Np];
+  }
+  else if constexpr (P::MF)
+  {
+    rate[0] *= rhoi;
+    rate[1] *= rhoi;
+    rate[2] *= rhoi;
   }
   else
   {
@@ -453,8 +532,9 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
 template <int N, int EPB, class P> struct NodeWaves
 {
   static constexpr int value =
-      EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4)
-               : ((N == 5 && sizeof(typename P::TC) == 8) ? 4 : 1);
+      P::MF ? 4
+            : (EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4)
+                        : ((N == 5 && sizeof(typename P::TC) == 8) ? 4 : 1));
 };
 
 template <int N, int EPB, class P>
@@ -462,7 +542,7 @@ __global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1))
 __attribute__((amdgpu_waves_per_eu(NodeWaves<N, EPB, P>::value))) void
 elastic_stage_kernel(LaunchArgs a)
 {
-  __shared__ __align__(16) char smem[NodeSmem<N, EPB, P>::bytes];
+  __shared__ __align__(16) char smem[NodeLds<N, EPB, P>::bytes];
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
@@ -502,6 +582,7 @@ hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
   {
   case PREC_FP32: return launch_n<P32>(N, a, grid, s);
   case PREC_FP32_RHS: return launch_n<P32R>(N, a, grid, s);
+  case PREC_FP32_BF16X3: return launch_n<P32M>(N, a, grid, s);
   default: return launch_n<P64>(N, a, grid, s);
   }
 }

@@ -950,6 +950,7 @@ hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,
 {
   switch (prec)
   {
+  case PREC_FP32_BF16X3: return hipErrorInvalidValue; /* node-parallel kernel only */
   case PREC_FP32: return launch_line_n<P32>(N, a, grid, s);
   case PREC_FP32_RHS: return launch_line_n<P32R>(N, a, grid, s);
   default: return launch_line_n<P64>(N, a, grid, s);

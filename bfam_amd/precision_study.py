@@ -7,9 +7,12 @@ driven by the matching forcing. It runs for `steps` LSRK4(5) steps (10^4 by
 default) at CFL 0.5.
 
 Forms compared:
-  fp64      FP64 state, rates and arithmetic
-  fp32_rhs  FP64 state, FP32 rates and arithmetic
-  fp32      FP32 state, rates and arithmetic
+  fp64         FP64 state, rates and arithmetic
+  fp32_rhs     FP64 state, FP32 rates and arithmetic
+  fp32         FP32 state, rates and arithmetic
+  fp32_bf16x3  as fp32, with the derivative contractions on the matrix
+               cores in split-BF16x3 (two-term bf16 operands, three
+               products, FP32 accumulation)
 
 Every `every` steps the script prints one JSON line per form with:
   - the relative L2 error against the exact solution;
@@ -18,6 +21,9 @@ Every `every` steps the script prints one JSON line per form with:
 All of these are FP64 device reductions with compensated sums.
 
     python -m bfam_amd.precision_study [--n 32] [--order 7] [--steps 10000]
+                                       [--out record.jsonl]
+
+--out also writes the JSON lines to a file (the committed record of a run).
 """
 import argparse
 import json
@@ -37,7 +43,8 @@ def main(argv=None):
     ap.add_argument("--order", type=int, default=7)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--every", type=int, default=1000)
-    ap.add_argument("--forms", default="fp64,fp32_rhs,fp32")
+    ap.add_argument("--forms", default="fp64,fp32_rhs,fp32,fp32_bf16x3")
+    ap.add_argument("--out", default=None, help="also write the JSON lines to this file")
     a = ap.parse_args(argv)
     forms = a.forms.split(",")
     if "fp64" not in forms:
@@ -55,8 +62,17 @@ def main(argv=None):
     del q0
     clock = {f: 0.0 for f in forms}
     done = 0
-    print(json.dumps({"config": f"elastic3d_mms {a.n}^3 N={a.order}", "dof": dof,
-                      "dt": cfg["dt"], "steps": a.steps}), flush=True)
+    out = open(a.out, "w") if a.out else None
+
+    def emit(rec):
+        line = json.dumps(rec)
+        print(line, flush=True)
+        if out:
+            out.write(line + "\n")
+            out.flush()
+
+    emit({"config": f"elastic3d_mms {a.n}^3 N={a.order}", "dof": dof, "dt": cfg["dt"],
+          "steps": a.steps, "forms": forms})
     while done < a.steps:
         k = min(a.every, a.steps - done)
         for f in forms:
@@ -77,10 +93,12 @@ def main(argv=None):
             if f != "fp64":
                 x2, y2 = s.l2_diff(subs["fp64"])
                 rec["rel_l2_vs_fp64"] = math.sqrt(x2 / y2)
-            print(json.dumps(rec), flush=True)
+            emit(rec)
     for s in subs.values():
         s.close()
     dev.close()
+    if out:
+        out.close()
 
 
 if __name__ == "__main__":

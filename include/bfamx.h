@@ -181,7 +181,11 @@ typedef enum
   BFAMX_FP64 = 0,       /* fused stage, FP64 state, rates and arithmetic */
   BFAMX_FP64_PLAIN = 1, /* unfused volume / surface / update kernels, FP64 */
   BFAMX_FP32_RHS = 2,   /* fused, FP64 state, FP32 rates and arithmetic */
-  BFAMX_FP32 = 3        /* fused, FP32 state, rates and arithmetic */
+  BFAMX_FP32 = 3,       /* fused, FP32 state, rates and arithmetic */
+  /* as FP32, with the volume derivative contractions on the matrix cores
+   * in split-BF16x3 (operands split into two bf16 terms, three products,
+   * FP32 accumulation); elastic, node-parallel stage kernel */
+  BFAMX_FP32_BF16X3 = 4
 } bfamx_precision_t;
 
 /* the hot-path data of bfam_subdomain_dgx_t (bfam.h:1364-1420) */

@@ -247,6 +247,57 @@ def test_mms_order_7(dev, kernel, prec):
     assert np.sqrt(d2 / n2) < 1e-3
 
 
+@pytest.mark.parametrize("N", [3, 5, 7])
+def test_bf16x3_matrix_core_derivatives(dev, N):
+    """Split-BF16x3 derivatives on the matrix cores: the RHS and a few steps
+    against the oracle within the accuracy two-term bf16 operands give
+    (2e-4 of the largest value), and visibly less accurate than FP32 (so
+    that the matrix-core path is the one that ran)."""
+    print(f"bf16x3 N={N}", flush=True)
+    cfg, q0 = random_elastic((3, 2, 2), N, seed=40 + N)
+    P = oracle_problem(cfg)
+    t = 0.2 * cfg["src"]["t0"]
+    want = P.rhs(q0, t)
+    errs = {}
+    for prec in (bx.FP32, bx.FP32_BF16X3):
+        sub = W.upload(dev, cfg, prec)
+        sub.set_fields(q0, 0.0, cfg["dt"])
+        errs[prec] = rel_max(sub.rhs(t), want)
+        sub.step(3)
+        q = sub.download()
+        sub.close()
+        assert rel_max(q, P.run(q0, cfg["dt"], 3)) < 2e-4
+    assert errs[bx.FP32_BF16X3] < 2e-4
+    assert errs[bx.FP32_BF16X3] > 4 * errs[bx.FP32]
+
+
+def test_bf16x3_manufactured_solution_order_7(dev):
+    print("bf16x3 mms 3^3 N=7", flush=True)
+    cfg = W.elastic3d_mms(n=3, N=7)
+    m = cfg["mesh"]
+    q0 = W.mms_exact(m, cfg["A"], cfg["k"], cfg["omega"], 0.0)
+    P = oracle_problem(cfg)
+    sub = W.upload(dev, cfg, bx.FP32_BF16X3)
+    sub.set_fields(q0, 0.0, cfg["dt"])
+    sub.step(10)
+    d2, n2 = sub.mms_error(cfg["A"])
+    q = sub.download()
+    sub.close()
+    assert rel_max(q, P.run(q0, cfg["dt"], 10)) < 2e-4
+    assert np.sqrt(d2 / n2) < 1e-3
+
+
+def test_bf16x3_runs_the_node_kernel_only(dev):
+    with pytest.raises(bx.BfamxError) as e:
+        dev.set_kernel(bx.FP32_BF16X3, 7, 0)
+    assert e.value.code == bx.EUNSUPPORTED
+    assert dev.get_kernel(bx.FP32_BF16X3, 5) == 1
+    cfg, _ = random_acoustic((3, 3), 4)
+    with pytest.raises(bx.BfamxError) as e:
+        W.upload(dev, cfg, bx.FP32_BF16X3)
+    assert e.value.code == bx.EUNSUPPORTED
+
+
 def test_full_size_rhs_against_oracle(dev):
     """The 48^3 N=7 elastic brick at full size: one FP64 right-hand side
     evaluation (with the Ricker source active) against the oracle on the
