@@ -48,15 +48,63 @@ enum
   PART_SURFACE = 2  /* rhs += lifted face fluctuations (unfused path) */
 };
 
+/* values a face node's lift adds to the rates of its volume node: the
+ * velocities (0..2), the normal stresses (3..5) and the two shear stresses
+ * that involve the face's axis (face_lift) */
+constexpr int FACE_LIFT_N = 8;
+
 template <int N, int EPB, class T, bool DQ = false> struct ElasticSmem
 {
   static constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
   static constexpr int QW = EPB * 9 * Np;   /* staged fields */
   static constexpr int DW = DQ ? QW : 0;    /* staged dq rows */
-  static constexpr int FW = EPB * 36 * Nfp; /* face fluctuations */
-  /* LDS: fields, dq, face fluctuations, D, sin/cos tables */
+  static constexpr int FW = EPB * 6 * FACE_LIFT_N * Nfp; /* face lifts */
+  /* LDS: fields, dq, face lifts, D, sin/cos tables */
   static constexpr size_t bytes = sizeof(T) * (QW + DW + FW + n1 * n1 + EPB * 6 * n1);
 };
+
+/* rate index of lift value 6 and 7 for a face of axis A */
+__host__ __device__ constexpr int lift_shear0(int A) { return A == 2 ? 7 : 6; }
+__host__ __device__ constexpr int lift_shear1(int A) { return A == 0 ? 7 : 8; }
+
+/* The lift of one face node's fluctuations dv, dT, as the FACE_LIFT_N
+ * values it adds to the rates (those elastic_lift adds; its other rate is
+ * untouched):
+ *   L[0..2] = lift dT / rho,
+ *   L[3..5] = lift (lambda dvn + 2 mu s dv_c [c = axis]),
+ *   L[6], L[7] = lift mu s dv_t for the shear stresses of the axis
+ *                (rates lift_shear0(axis), lift_shear1(axis)). */
+template <int AX, class T>
+__device__ __forceinline__ void face_lift_axis(T s, T lift, T rhoi, T lam, T mu, const T *dv,
+                                               const T *dT, T *L)
+{
+  L[0] = lift * dT[0] * rhoi;
+  L[1] = lift * dT[1] * rhoi;
+  L[2] = lift * dT[2] * rhoi;
+  const T dvn = s * dv[AX];
+  const T ld = lam * dvn;
+  const T m2 = T(2) * mu * s;
+  L[3] = lift * (ld + (AX == 0 ? m2 * dv[0] : T(0)));
+  L[4] = lift * (ld + (AX == 1 ? m2 * dv[1] : T(0)));
+  L[5] = lift * (ld + (AX == 2 ? m2 * dv[2] : T(0)));
+  const T ms = mu * s;
+  L[6] = lift * (ms * dv[AX == 0 ? 1 : 0]);
+  L[7] = lift * (ms * dv[AX == 2 ? 1 : 2]);
+}
+/* face_lift_axis for an axis known only at run time (compile-time
+ * indices in each branch: a run-time index into dv would put it in
+ * scratch memory) */
+template <class T>
+__device__ __forceinline__ void face_lift(int axis, T s, T lift, T rhoi, T lam, T mu,
+                                          const T *dv, const T *dT, T *L)
+{
+  if (axis == 0)
+    face_lift_axis<0, T>(s, lift, rhoi, lam, mu, dv, dT, L);
+  else if (axis == 1)
+    face_lift_axis<1, T>(s, lift, rhoi, lam, mu, dv, dT, L);
+  else
+    face_lift_axis<2, T>(s, lift, rhoi, lam, mu, dv, dT, L);
+}
 
 /* rate += lift * (dT/rho, lambda dvn I + mu (dv n^T + n dv^T)) */
 template <class T>

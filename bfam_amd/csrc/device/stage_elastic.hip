@@ -22,10 +22,11 @@
  *   2. face tasks (one thread per face node): minus trace from LDS, plus trace
  *      (velocity and the stress column of the face's axis) gathered from the
  *      neighbour's q through nbr_elem/nbr_code (the on-device replacement of
- *      the glue trace exchange), upwind fluctuations -> LDS
+ *      the glue trace exchange), upwind fluctuations, lifted (the eight rate
+ *      increments of face_lift) -> LDS
  *   3. 18 directional derivatives per node from LDS (2(N+1) flop each),
  *      folded into the rates one axis at a time
- *   4. node owners lift the fluctuations of their faces (fixed face order)
+ *   4. node owners add the lift values of their faces (x, y, z)
  *   5. q_out (and dq, FP32 rates on FP64 state only) written: q ping-pongs
  *      so that neighbours read the stage input; nothing but the loop state
  *      is held across a barrier
@@ -41,12 +42,14 @@ namespace bfamx
 
 /* the dq rows of a task are staged in LDS by LDS-DMA (no registers held
  * while they load) for FP32 rates, where the larger LDS footprint still
- * lets three 512-thread blocks share a CU */
+ * lets as many blocks share a CU as the registers allow: three one-element
+ * blocks (N = 6, 7), two of the multi-element blocks of N = 3..5, whose
+ * register needs allow no more */
 template <int N, int EPB, class P> struct NodeDq
 {
   static constexpr bool value =
       sizeof(typename P::TR) == 4 &&
-      3 * ElasticSmem<N, EPB, typename P::TC, true>::bytes <= 160 * 1024;
+      (EPB == 1 ? 3 : 2) * ElasticSmem<N, EPB, typename P::TC, true>::bytes <= 160 * 1024;
 };
 template <int N, int EPB, class P>
 using NodeSmem = ElasticSmem<N, EPB, typename P::TC, NodeDq<N, EPB, P>::value>;
@@ -121,7 +124,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
 
   TC *sq = reinterpret_cast<TC *>(smem); /* [EPB][9][Np] */
   TC *sdq = sq + SM::QW;                 /* [EPB][9][Np] when DMA_DQ */
-  TC *sfl = sdq + SM::DW;                /* [EPB][6 faces][6 comps][Nfp] */
+  TC *sfl = sdq + SM::DW;                /* [EPB][6 faces][FACE_LIFT_N][Nfp] */
   TC *sD = sfl + SM::FW;
   TC *smms = sD + n1 * n1;
 
@@ -201,7 +204,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
-  /* ---- 2. faces: one task per face node, fluctuations straight to LDS ---- */
+  /* ---- 2. faces: one task per face node, its lift values straight to LDS ---- */
   if constexpr (PART != PART_VOLUME)
   {
     gptr<const int32_t> nbe = glob(sd.nbr_elem);
@@ -221,47 +224,56 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
         const int nm = face_node<3>(f, n1, a, b);
         const int32_t ep = nbe[(size_t)et * 6 + f];
         const int code = nbc[(size_t)et * 6 + f];
-        TC *o = sfl + (lt * 6 + f) * 6 * Nfp + fn;
+        TC *o = sfl + (lt * 6 + f) * FACE_LIFT_N * Nfp + fn;
+        const int A = f >> 1;
+        EP Mm = (EP)mtab + (size_t)et * MAT_STRIDE;
+        const TC lf = (EP(geo_tab<TC>(sd)) + (size_t)et * GEO_STRIDE)[GEO_LX + A];
+        TC dv[3], dT[3];
         if (code & NBR_GLUED)
         {
           /* fluctuations already on this face node (mortar kernel) */
           gptr<const TC> gb = glob(static_cast<const TC *>(sd.glue_buf)) + ep + fn;
 #pragma unroll
-          for (int c = 0; c < 6; ++c)
-            o[c * Nfp] = gb[c * Nfp];
-          continue;
+          for (int c = 0; c < 3; ++c)
+          {
+            dv[c] = gb[c * Nfp];
+            dT[c] = gb[(3 + c) * Nfp];
+          }
         }
-        const int oi = perm_apply(nbr_perm(code), n1, a, b);
-        const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
-        gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
-        const TC *u = sq + lt * NF * Np + nm;
-        TC sv, ss;
-        nbr_mirror<TC>(code, &sv, &ss);
-        /* only the velocity and the stress column of the face's axis enter
-         * the flux: six of the nine neighbour values are gathered */
-        const int A = f >> 1;
-        TC vm[3], Sm[3], vp[3], Sp[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
+        else
         {
-          const int col = sfield(c, A);
-          vm[c] = u[c * Np];
-          Sm[c] = u[col * Np];
-          vp[c] = sv * (TC)qp[c * Np];
-          Sp[c] = ss * (TC)qp[col * Np];
-        }
-        EP Mm = (EP)mtab + (size_t)et * MAT_STRIDE;
-        gptr<const TC> F = fmat_tab<TC>(sd) + ((size_t)et * 6 + f) * FMAT_STRIDE;
-        TC dv[3], dT[3];
-        riemann_dispatch<TC>(A, (f & 1) ? TC(1) : TC(-1), vm, Sm, vp, Sp, Mm[MAT_ZP],
-                             Mm[MAT_ZS], F[FMAT_ZP], F[FMAT_ZS], F[FMAT_IZP],
-                             F[FMAT_IZS], dv, dT);
+          const int oi = perm_apply(nbr_perm(code), n1, a, b);
+          const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
+          gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
+          const TC *u = sq + lt * NF * Np + nm;
+          TC sv, ss;
+          nbr_mirror<TC>(code, &sv, &ss);
+          /* only the velocity and the stress column of the face's axis enter
+           * the flux: six of the nine neighbour values are gathered */
+          TC vm[3], Sm[3], vp[3], Sp[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-        {
-          o[c * Nfp] = dv[c];
-          o[(3 + c) * Nfp] = dT[c];
+          for (int c = 0; c < 3; ++c)
+          {
+            const int col = sfield(c, A);
+            vm[c] = u[c * Np];
+            Sm[c] = u[col * Np];
+            vp[c] = sv * (TC)qp[c * Np];
+            Sp[c] = ss * (TC)qp[col * Np];
+          }
+          gptr<const TC> F = fmat_tab<TC>(sd) + ((size_t)et * 6 + f) * FMAT_STRIDE;
+          riemann_dispatch<TC>(A, (f & 1) ? TC(1) : TC(-1), vm, Sm, vp, Sp, Mm[MAT_ZP],
+                               Mm[MAT_ZS], F[FMAT_ZP], F[FMAT_ZS], F[FMAT_IZP],
+                               F[FMAT_IZS], dv, dT);
         }
+        /* the face node's lift, computed here once instead of by its
+         * volume node, whose wave would run the code of every face any of
+         * its lanes lies on */
+        TC L[FACE_LIFT_N];
+        face_lift<TC>(A, (f & 1) ? TC(1) : TC(-1), lf, Mm[MAT_RHOI], Mm[MAT_LAM], Mm[MAT_MU],
+                      dv, dT, L);
+#pragma unroll
+        for (int c = 0; c < FACE_LIFT_N; ++c)
+          o[c * Nfp] = L[c];
       }
     }
     __syncthreads();
@@ -438,25 +450,22 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     rate[2] *= rhoi;
   }
 
-  /* ---- 4. lift: faces in the fixed order -x,+x,-y,+y,-z,+z ---- */
+  /* ---- 4. lift: the lift values of the node's faces, x then y then z
+   * (a node lies on at most one face per axis) ---- */
   if constexpr (PART != PART_VOLUME)
   {
-    const TC *F = sfl + le * 36 * Nfp;
-    TC flv[6];
-#define BFAMX_LIFT(face, cond, fn_)                                            \
-  if (cond)                                                                    \
+    const TC *F = sfl + le * 6 * FACE_LIFT_N * Nfp;
+#define BFAMX_LIFT(A, c_, fn_)                                                 \
+  if ((c_) == 0 || (c_) == N)                                                  \
   {                                                                            \
-    _Pragma("unroll") for (int c = 0; c < 6; ++c) flv[c] =                     \
-        F[((face)*6 + c) * Nfp + (fn_)];                                       \
-    elastic_lift<TC>((face) >> 1, ((face)&1) ? TC(1) : TC(-1),                 \
-                     G[GEO_LX + ((face) >> 1)], rhoi, lam, mu, flv, rate);     \
+    const TC *Ff = F + (2 * (A) + ((c_) == N)) * FACE_LIFT_N * Nfp + (fn_);     \
+    _Pragma("unroll") for (int c = 0; c < 6; ++c) rate[c] += Ff[c * Nfp];      \
+    rate[lift_shear0(A)] += Ff[6 * Nfp];                                       \
+    rate[lift_shear1(A)] += Ff[7 * Nfp];                                       \
   }
-    BFAMX_LIFT(0, i == 0, j + n1 * k)
-    BFAMX_LIFT(1, i == N, j + n1 * k)
-    BFAMX_LIFT(2, j == 0, i + n1 * k)
-    BFAMX_LIFT(3, j == N, i + n1 * k)
-    BFAMX_LIFT(4, k == 0, i + n1 * j)
-    BFAMX_LIFT(5, k == N, i + n1 * j)
+    BFAMX_LIFT(0, i, j + n1 * k)
+    BFAMX_LIFT(1, j, i + n1 * k)
+    BFAMX_LIFT(2, k, i + n1 * j)
 #undef BFAMX_LIFT
   }
 

@@ -23,7 +23,8 @@ from bfam_amd import build as B  # noqa: E402
 
 PREC = {"fff": "Prec<float, float, float>", "dff": "Prec<double, float, float>",
         "ddd": "Prec<double, double, double>"}
-MANGLED = {"fff": "NS_4PrecIfffEE", "dff": "NS_4PrecIdffEE", "ddd": "NS_4PrecIdddEE"}
+# Prec<TS, TR, TC, MF = false>
+MANGLED = {"fff": "NS_4PrecIfffLb0EEE", "dff": "NS_4PrecIdffLb0EEE", "ddd": "NS_4PrecIdddLb0EEE"}
 
 
 @pytest.fixture(scope="module")
