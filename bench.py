@@ -38,7 +38,12 @@ The JSON line also carries:
                      admitted under an HBM budget and advanced together, one
                      batched launch per stage, FP32 rates on FP64 state:
                      p50/p99 latency and aggregate DOF-stage-updates/s;
-  serving_fp32       the same stream with FP32 state and rates.
+  serving_fp32       the same stream with FP32 state and rates;
+  serving_saturated  100 such requests arriving at 1000/s (the GPU completes
+                     about 2/s) served in FP32: p50/p99 and the aggregate
+                     rate, against the headline and against the same 100
+                     requests stepped as one batch (no arrivals, joins or
+                     leaves).
 
 --copy-calibration runs only a 1 GiB device-to-device copy kernel (torch)
 so that the counters of a kernel of known traffic can be recorded.
@@ -166,6 +171,38 @@ def run_cpu_baseline(cfg, budget_s):
                                    f"{el1:.1f} s"}}
 
 
+def run_saturated_serving(dev, nreq, budget, headline):
+    """nreq requests (8^3..32^3 hexes, N = 3..7, 200..2000 steps) arriving at
+    1000/s, far above what one GPU completes (about 2/s), served in FP32
+    under the budget; then the same requests stepped as one batch with no
+    arrivals, joins or leaves (the rate the serving loop could reach on
+    this mix)."""
+    import bfam_amd as bx
+    reqs = bx.serve_generate(seed=31, nreq=nreq, rate_per_s=1000.0)
+    _, st = bx.serve(dev, reqs, bx.FP32, budget_bytes=budget, max_chunk=10)
+    subs = [bx.request_build(dev, r, bx.FP32)[0] for r in reqs]
+    b = bx.Batch(dev, subs)
+    b.autotune(trials=2)
+    b.step(1)
+    dev.sync()
+    ks = 4
+    dev.timer_start()
+    b.step(ks)
+    ms = dev.timer_stop()
+    b.close()
+    for s_ in subs:
+        s_.close()
+    dof = sum(9 * r.n ** 3 * (r.N + 1) ** 3 for r in reqs)
+    mix = dof * 5 * ks / (ms * 1e-3)
+    return {"requests": nreq, "rate_per_s": 1000.0, "budget_bytes": budget,
+            "precision": "fp32 state and rates", "p50_s": st["p50_s"], "p99_s": st["p99_s"],
+            "max_s": st["max_s"], "dof_stage_per_s": st["dof_stage_per_s"],
+            "wall_s": st["wall_s"], "max_active": st["max_active"],
+            "batch_rebuilds": st["nbatches"], "batched_mix_dof_stage_per_s": mix,
+            "vs_batched_mix": st["dof_stage_per_s"] / mix,
+            "vs_headline": st["dof_stage_per_s"] / headline if headline else None}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +215,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adapted", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-saturated", type=int, default=100,
+                    help="requests of the saturated serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=0.6, help="arrivals per second")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--only", choices=["fp32", "fp64", "fp64_plain", "fp32_rhs", "adapted"],
@@ -323,6 +362,11 @@ def main():
                 "max_active": st["max_active"], "peak_bytes": st["peak_bytes"],
                 "batch_rebuilds": st["nbatches"]}
             log(f"{key}: {serving[key]}")
+    if not args.only and args.serve_saturated > 0:
+        serving["serving_saturated"] = run_saturated_serving(
+            dev, args.serve_saturated, int(args.serve_budget_gb * (1 << 30)),
+            res["fp32"]["dof_stage_per_s"] if "fp32" in res else None)
+        log(f"serving_saturated: {serving['serving_saturated']}")
     dev.close()
     if only_adapted:
         print(json.dumps({"adapted_mesh": adapted}), flush=True)
@@ -373,6 +417,7 @@ def main():
         "adapted_mesh": adapted,
         "serving": serving.get("serving"),
         "serving_fp32": serving.get("serving_fp32"),
+        "serving_saturated": serving.get("serving_saturated"),
     }
     print(json.dumps(out), flush=True)
 
