@@ -411,6 +411,18 @@ __device__ __forceinline__ StageCtx stage_ctx(const SD &sd, const LaunchArgs &a)
 }
 
 
+/* the neighbour's node facing node nm of face f (non-glued faces): the
+ * upload admits only axis-aligned connectivity, a neighbour meeting face f
+ * with its opposite face f ^ 1 and unpermuted nodes, and boundary faces,
+ * whose plus trace is the face's own */
+__device__ __forceinline__ int nbr_node(int code, int f, int n1, int nm)
+{
+  if (code & NBR_BOUNDARY)
+    return nm;
+  const int stride = f < 2 ? 1 : (f < 4 ? n1 : n1 * n1);
+  return (f & 1) ? nm - (n1 - 1) * stride : nm + (n1 - 1) * stride;
+}
+
 /* face permutation (swap / reverse bits), same as bfamx_perm_apply */
 __device__ __forceinline__ int perm_apply(int code, int n1, int a, int b)
 {

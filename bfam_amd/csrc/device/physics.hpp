@@ -53,13 +53,15 @@ enum
  * that involve the face's axis (face_lift) */
 constexpr int FACE_LIFT_N = 8;
 
-template <int N, int EPB, class T, bool DQ = false> struct ElasticSmem
+/* FC: values per face node, 6 (the fluctuations dv, dT) or FACE_LIFT_N
+ * (their lift, face_lift) */
+template <int N, int EPB, class T, bool DQ = false, int FC = FACE_LIFT_N> struct ElasticSmem
 {
   static constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
   static constexpr int QW = EPB * 9 * Np;   /* staged fields */
   static constexpr int DW = DQ ? QW : 0;    /* staged dq rows */
-  static constexpr int FW = EPB * 6 * FACE_LIFT_N * Nfp; /* face lifts */
-  /* LDS: fields, dq, face lifts, D, sin/cos tables */
+  static constexpr int FW = EPB * 6 * FC * Nfp; /* face values */
+  /* LDS: fields, dq, face values, D, sin/cos tables */
   static constexpr size_t bytes = sizeof(T) * (QW + DW + FW + n1 * n1 + EPB * 6 * n1);
 };
 

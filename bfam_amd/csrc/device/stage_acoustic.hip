@@ -122,8 +122,7 @@ __device__ __forceinline__ void acoustic_task(const SD &sd, const Task &tk,
       const int nm = face_node<2>(f, n1, a, 0);
       const int32_t ep = glob(sd.nbr_elem)[(size_t)et * 4 + f];
       const int code = glob(sd.nbr_code)[(size_t)et * 4 + f];
-      const int oi = perm_apply(nbr_perm(code), n1, a, 0);
-      const int np = face_node<2>(code & 7, n1, oi, 0);
+      const int np = nbr_node(code, f, n1, nm);
       const TC *u = sq + lt * NF * Np;
       const int axis = f >> 1;
       const TC s = (f & 1) ? TC(1) : TC(-1);

@@ -22,11 +22,12 @@
  *   2. face tasks (one thread per face node): minus trace from LDS, plus trace
  *      (velocity and the stress column of the face's axis) gathered from the
  *      neighbour's q through nbr_elem/nbr_code (the on-device replacement of
- *      the glue trace exchange), upwind fluctuations, lifted (the eight rate
- *      increments of face_lift) -> LDS
+ *      the glue trace exchange), upwind fluctuations (or, at the orders of
+ *      NodeFaceLift, their lift: the eight rate increments of face_lift) ->
+ *      LDS
  *   3. 18 directional derivatives per node from LDS (2(N+1) flop each),
  *      folded into the rates one axis at a time
- *   4. node owners add the lift values of their faces (x, y, z)
+ *   4. node owners lift their faces' fluctuations (or add the lift values)
  *   5. q_out (and dq, FP32 rates on FP64 state only) written: q ping-pongs
  *      so that neighbours read the stage input; nothing but the loop state
  *      is held across a barrier
@@ -40,6 +41,21 @@
 namespace bfamx
 {
 
+/* Where the LGL lift of a face node is computed.  Face-side: the face task
+ * turns its fluctuations into the FACE_LIFT_N rate increments (face_lift)
+ * and the volume node adds those of the at most one face per axis it lies
+ * on.  Node-side: the face task leaves its six fluctuations and the volume
+ * node lifts each of its faces (elastic_lift); its wave then runs the lift
+ * of every face any of its lanes lies on.  Face-side measured faster at
+ * N = 4 and 7 (30^3 N=7 FP32: 0.398 -> 0.389 ms per stage, 48^3: 6.27 ->
+ * 6.18 ms per step), slower at N = 3 and 6 (0.510 -> 0.529, 0.538 -> 0.572
+ * ms), the same at N = 5. */
+template <int N> struct NodeFaceLift
+{
+  static constexpr bool value = N == 4 || N == 5 || N == 7;
+  static constexpr int FC = value ? FACE_LIFT_N : 6;
+};
+
 /* the dq rows of a task are staged in LDS by LDS-DMA (no registers held
  * while they load) for FP32 rates, where the larger LDS footprint still
  * lets as many blocks share a CU as the registers allow: three one-element
@@ -49,10 +65,13 @@ template <int N, int EPB, class P> struct NodeDq
 {
   static constexpr bool value =
       sizeof(typename P::TR) == 4 &&
-      (EPB == 1 ? 3 : 2) * ElasticSmem<N, EPB, typename P::TC, true>::bytes <= 160 * 1024;
+      (EPB == 1 ? 3 : 2) *
+              ElasticSmem<N, EPB, typename P::TC, true, NodeFaceLift<N>::FC>::bytes <=
+          160 * 1024;
 };
 template <int N, int EPB, class P>
-using NodeSmem = ElasticSmem<N, EPB, typename P::TC, NodeDq<N, EPB, P>::value>;
+using NodeSmem =
+    ElasticSmem<N, EPB, typename P::TC, NodeDq<N, EPB, P>::value, NodeFaceLift<N>::FC>;
 /* LDS of a block: the regions above, and for the matrix-core derivatives
  * the six derivative fields of one axis (FP32) */
 template <int N, int EPB, class P> struct NodeLds
@@ -124,7 +143,10 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
 
   TC *sq = reinterpret_cast<TC *>(smem); /* [EPB][9][Np] */
   TC *sdq = sq + SM::QW;                 /* [EPB][9][Np] when DMA_DQ */
-  TC *sfl = sdq + SM::DW;                /* [EPB][6 faces][FACE_LIFT_N][Nfp] */
+  /* face values: [EPB][6 faces][FC][Nfp] */
+  constexpr bool FL = NodeFaceLift<N>::value;
+  constexpr int FC = NodeFaceLift<N>::FC;
+  TC *sfl = sdq + SM::DW;
   TC *sD = sfl + SM::FW;
   TC *smms = sD + n1 * n1;
 
@@ -210,22 +232,27 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     gptr<const int32_t> nbe = glob(sd.nbr_elem);
     gptr<const uint8_t> nbc = glob(sd.nbr_code);
     gptr<const TC> mtab = mat_tab<TC>(sd);
+    /* tasks axis-major (axis, element, side, face node), so that the
+     * waves of a block mostly see faces of one axis: the flux and lift of
+     * an axis run once per wave, not once per axis present in it */
+    constexpr int TPA = EPB * 2 * Nfp;
 #pragma unroll
     for (int r = 0; r < TPT; ++r)
     {
       const int t = tid + r * BS;
-      const int lt = EPB == 1 ? 0 : t / (6 * Nfp);
+      const int A = t / TPA, ra = t - A * TPA;
+      const int lt = EPB == 1 ? 0 : ra / (2 * Nfp);
       if (tid < BS && t < NT && lt < tk.ne)
       {
-        const int rem = t - lt * 6 * Nfp;
-        const int f = rem / Nfp, fn = rem - (rem / Nfp) * Nfp;
+        const int rs = ra - lt * 2 * Nfp;
+        const int side = rs / Nfp, fn = rs - side * Nfp;
+        const int f = 2 * A + side;
         const int et = tk.e0 + lt;
         const int a = fn % n1, b = fn / n1;
         const int nm = face_node<3>(f, n1, a, b);
         const int32_t ep = nbe[(size_t)et * 6 + f];
         const int code = nbc[(size_t)et * 6 + f];
-        TC *o = sfl + (lt * 6 + f) * FACE_LIFT_N * Nfp + fn;
-        const int A = f >> 1;
+        TC *o = sfl + (lt * 6 + f) * FC * Nfp + fn;
         EP Mm = (EP)mtab + (size_t)et * MAT_STRIDE;
         const TC lf = (EP(geo_tab<TC>(sd)) + (size_t)et * GEO_STRIDE)[GEO_LX + A];
         TC dv[3], dT[3];
@@ -242,9 +269,7 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
         }
         else
         {
-          const int oi = perm_apply(nbr_perm(code), n1, a, b);
-          const int np = face_node<3>(code & 7, n1, oi % n1, oi / n1);
-          gptr<const TS> qp = qin + (size_t)ep * NF * Np + np;
+          gptr<const TS> qp = qin + (size_t)ep * NF * Np + nbr_node(code, f, n1, nm);
           const TC *u = sq + lt * NF * Np + nm;
           TC sv, ss;
           nbr_mirror<TC>(code, &sv, &ss);
@@ -265,15 +290,24 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
                                Mm[MAT_ZS], F[FMAT_ZP], F[FMAT_ZS], F[FMAT_IZP],
                                F[FMAT_IZS], dv, dT);
         }
-        /* the face node's lift, computed here once instead of by its
-         * volume node, whose wave would run the code of every face any of
-         * its lanes lies on */
-        TC L[FACE_LIFT_N];
-        face_lift<TC>(A, (f & 1) ? TC(1) : TC(-1), lf, Mm[MAT_RHOI], Mm[MAT_LAM], Mm[MAT_MU],
-                      dv, dT, L);
+        if constexpr (FL)
+        {
+          TC L[FACE_LIFT_N];
+          face_lift<TC>(A, (f & 1) ? TC(1) : TC(-1), lf, Mm[MAT_RHOI], Mm[MAT_LAM],
+                        Mm[MAT_MU], dv, dT, L);
 #pragma unroll
-        for (int c = 0; c < FACE_LIFT_N; ++c)
-          o[c * Nfp] = L[c];
+          for (int c = 0; c < FACE_LIFT_N; ++c)
+            o[c * Nfp] = L[c];
+        }
+        else
+        {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+          {
+            o[c * Nfp] = dv[c];
+            o[(3 + c) * Nfp] = dT[c];
+          }
+        }
       }
     }
     __syncthreads();
@@ -450,15 +484,16 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     rate[2] *= rhoi;
   }
 
-  /* ---- 4. lift: the lift values of the node's faces, x then y then z
-   * (a node lies on at most one face per axis) ---- */
-  if constexpr (PART != PART_VOLUME)
+  /* ---- 4. lift: face-side, the lift values of the node's faces (at
+   * most one per axis); node-side, the node lifts its faces' fluctuations
+   * in the fixed order -x, +x, -y, +y, -z, +z ---- */
+  if constexpr (PART != PART_VOLUME && FL)
   {
-    const TC *F = sfl + le * 6 * FACE_LIFT_N * Nfp;
+    const TC *F = sfl + le * 6 * FC * Nfp;
 #define BFAMX_LIFT(A, c_, fn_)                                                 \
   if ((c_) == 0 || (c_) == N)                                                  \
   {                                                                            \
-    const TC *Ff = F + (2 * (A) + ((c_) == N)) * FACE_LIFT_N * Nfp + (fn_);     \
+    const TC *Ff = F + (2 * (A) + ((c_) == N)) * FC * Nfp + (fn_);              \
     _Pragma("unroll") for (int c = 0; c < 6; ++c) rate[c] += Ff[c * Nfp];      \
     rate[lift_shear0(A)] += Ff[6 * Nfp];                                       \
     rate[lift_shear1(A)] += Ff[7 * Nfp];                                       \
@@ -466,6 +501,26 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
     BFAMX_LIFT(0, i, j + n1 * k)
     BFAMX_LIFT(1, j, i + n1 * k)
     BFAMX_LIFT(2, k, i + n1 * j)
+#undef BFAMX_LIFT
+  }
+  else if constexpr (PART != PART_VOLUME)
+  {
+    const TC *F = sfl + le * 6 * FC * Nfp;
+    TC flv[6];
+#define BFAMX_LIFT(face, cond, fn_)                                            \
+  if (cond)                                                                    \
+  {                                                                            \
+    _Pragma("unroll") for (int c = 0; c < 6; ++c) flv[c] =                     \
+        F[((face)*FC + c) * Nfp + (fn_)];                                      \
+    elastic_lift<TC>((face) >> 1, ((face)&1) ? TC(1) : TC(-1),                 \
+                     G[GEO_LX + ((face) >> 1)], rhoi, lam, mu, flv, rate);     \
+  }
+    BFAMX_LIFT(0, i == 0, j + n1 * k)
+    BFAMX_LIFT(1, i == N, j + n1 * k)
+    BFAMX_LIFT(2, j == 0, i + n1 * k)
+    BFAMX_LIFT(3, j == N, i + n1 * k)
+    BFAMX_LIFT(4, k == 0, i + n1 * j)
+    BFAMX_LIFT(5, k == N, i + n1 * j)
 #undef BFAMX_LIFT
   }
 
@@ -602,7 +657,7 @@ template <int N, int EPB, int PART>
 __global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1)) void
 elastic_part_kernel(LaunchArgs a)
 {
-  __shared__ __align__(16) char smem[ElasticSmem<N, EPB, double>::bytes];
+  __shared__ __align__(16) char smem[NodeLds<N, EPB, P64>::bytes];
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
