@@ -163,7 +163,8 @@ __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
     Zps = Ms[MAT_ZP];
     Zss = Ms[MAT_ZS];
     izp = TC(1) / (Zps + Zpb);
-    izs = TC(1) / (Zss + Zsb);
+    /* two fluids: no shear waves, and no tangential fluctuation */
+    izs = Zss + Zsb > TC(0) ? TC(1) / (Zss + Zsb) : TC(0);
     /* interpolation to the mortar grid, first axis: lane (g, j) = (lo, hi) */
 #pragma unroll
     for (int c = 0; c < 6; ++c)

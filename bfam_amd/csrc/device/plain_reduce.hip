@@ -96,6 +96,18 @@ __device__ __forceinline__ double ld(const void *p, size_t i)
   return (double)static_cast<const T *>(p)[i];
 }
 
+/* sigma : C^-1 : sigma of an isotropic medium (twice the strain energy
+ * density) for the stress entries f[3..8], split into its deviatoric part,
+ * dev : dev / (2 mu), and its volumetric part, tr^2 / (3 (3 lambda + 2 mu)).
+ * A fluid (mu = 0) carries no deviatoric stress, so its part is 0 there */
+__device__ __forceinline__ double stress_energy(const double *f, double lam, double mu)
+{
+  const double tr = f[3] + f[4] + f[5], m = tr / 3;
+  const double dev = (f[3] - m) * (f[3] - m) + (f[4] - m) * (f[4] - m) +
+                     (f[5] - m) * (f[5] - m) + 2 * (f[6] * f[6] + f[7] * f[7] + f[8] * f[8]);
+  return (mu > 0 ? dev / (2 * mu) : 0.0) + tr * tr / (3 * (3 * lam + 2 * mu));
+}
+
 template <class TS>
 __global__ __launch_bounds__(256) void energy_kernel(SubDev sd, int parity,
                                                      double *partial)
@@ -125,11 +137,7 @@ __global__ __launch_bounds__(256) void energy_kernel(SubDev sd, int parity,
       double f[9];
       for (int c = 0; c < 9; ++c)
         f[c] = ld<TS>(q, b + (size_t)c * Np);
-      const double tr = f[3] + f[4] + f[5];
-      const double ss = f[3] * f[3] + f[4] * f[4] + f[5] * f[5] +
-                        2 * (f[6] * f[6] + f[7] * f[7] + f[8] * f[8]);
-      dens = rho * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) +
-             (ss - lam / (3 * lam + 2 * mu) * tr * tr) / (2 * mu);
+      dens = rho * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) + stress_energy(f, lam, mu);
     }
     acc += 0.5 * quad_weight(sd, dim, n1, e, n) * dens;
   }
@@ -261,11 +269,8 @@ __global__ __launch_bounds__(256) void mms_error_kernel(SubDev sd, int parity,
     for (int w2 = 0; w2 < 2; ++w2)
     {
       const double *f = w2 ? xv : dv;
-      const double tr = f[3] + f[4] + f[5];
-      const double ss = f[3] * f[3] + f[4] * f[4] + f[5] * f[5] +
-                        2 * (f[6] * f[6] + f[7] * f[7] + f[8] * f[8]);
-      const double dens = rho * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) +
-                          (ss - lam / (3 * lam + 2 * mu) * tr * tr) / (2 * mu);
+      const double dens =
+          rho * (f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) + stress_energy(f, lam, mu);
       (w2 ? en : ed) += 0.5 * wq * dens;
     }
   }

@@ -521,7 +521,11 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
     double *M = &mat[(size_t)e * MAT_STRIDE];
     const double rho = phys->rho[e], cp = phys->cp[e];
     const double cs = physics == BFAMX_ELASTIC3D ? phys->cs[e] : 0.0;
-    if (!(rho > 0) || !(cp > 0) || cs < 0 || (physics == BFAMX_ELASTIC3D && !(cs > 0)))
+    /* cs = 0: a fluid element (acoustic medium in the velocity-stress
+     * form); the exact Riemann flux then couples it to solid neighbours
+     * (no shear traction, tangential slip).  The bulk modulus
+     * rho (cp^2 - 4/3 cs^2) must be positive */
+    if (!(rho > 0) || !(cp > 0) || !(cs >= 0) || !(3 * cp * cp > 4 * cs * cs))
       return bfamx_fail(BFAMX_EINVAL, "element %d has invalid material", e);
     M[MAT_RHOI] = 1.0 / rho;
     M[MAT_LAM] = rho * (cp * cp - 2 * cs * cs);

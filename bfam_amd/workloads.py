@@ -109,6 +109,31 @@ def elastic3d_ricker(n=48, N=7, seed=2024, cfl=0.5, nsteps=200, L=1.0):
             "physics": bx.ELASTIC3D}
 
 
+def elastic3d_fluid_layer(n=16, N=5, seed=2024, cfl=0.5, nsteps=200, depth=0.375,
+                          fluid=(1.0, 1.5), L=1.0):
+    """Config: elastic3d_ricker's brick with a fluid layer: the elements whose
+    centre lies above z = L (1 - depth) hold a fluid (rho, cp = `fluid`, cs =
+    0), the rest the PCG32 solid material.  The brick is periodic, so the
+    layer meets the solid at two horizontal interfaces.  Explosive Ricker
+    source at a seeded point inside the fluid layer."""
+    cfg = elastic3d_ricker(n=n, N=N, seed=seed, cfl=cfl, nsteps=nsteps, L=L)
+    mesh = cfg["mesh"]
+    zc = mesh.x0[:, 2] + 0.5 * mesh.h[:, 2]
+    wet = zc > L * (1.0 - depth)
+    cfg["rho"][wet] = fluid[0]
+    cfg["cp"][wet] = fluid[1]
+    cfg["cs"][wet] = 0.0
+    g = bx.PCG32(seed, 11)
+    xs = [L * g.uniform(0.25, 0.75), L * g.uniform(0.25, 0.75),
+          L * (1.0 - depth * g.uniform(0.3, 0.7))]
+    e, rs = mesh.locate(xs)
+    T = nsteps * cfg["dt"]
+    cfg["src"] = {"elem": e, "weights": point_source_weights(mesh, e, rs),
+                  "amp": [0, 0, 0, 1, 1, 1, 0, 0, 0], "t0": 0.5 * T, "f0": 2.4 / T, "x": xs}
+    cfg["wet"] = wet
+    return cfg
+
+
 def acoustic2d_plane_wave(n=16, N=4, cfl=0.5, nsteps=100, rho=1.0, c=1.0):
     """Config: 2-D acoustic on the periodic unit square, p = cos(2 pi (x+y)),
     velocity of the plane wave travelling along (1,1)/sqrt(2)."""

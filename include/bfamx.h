@@ -211,7 +211,9 @@ typedef struct
   const double *elem_x0; /* K*dim lower corners (needed for MMS forcing) */
   const double *rho;     /* K */
   const double *cp;      /* K: P-wave (acoustic: sound) speed */
-  const double *cs;      /* K: S-wave speed (ignored for acoustics) */
+  const double *cs;      /* K: S-wave speed, >= 0; 0 makes the element a
+                            fluid coupled to solid neighbours by the flux
+                            (ignored for acoustics) */
 } bfamx_physics_desc_t;
 
 typedef struct bfamx_subdomain bfamx_subdomain_t;

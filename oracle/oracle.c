@@ -1279,11 +1279,15 @@ double orc_energy(const orc_problem_t *p, const double *q)
         const double s11 = q[3 * KNp + id], s22 = q[4 * KNp + id],
                      s33 = q[5 * KNp + id], s12 = q[6 * KNp + id],
                      s13 = q[7 * KNp + id], s23 = q[8 * KNp + id];
-        const double tr = s11 + s22 + s33;
-        const double ss = s11 * s11 + s22 * s22 + s33 * s33 +
-                          2 * (s12 * s12 + s13 * s13 + s23 * s23);
-        dens = rho * (v1 * v1 + v2 * v2 + v3 * v3) +
-               (ss - lam / (3 * lam + 2 * mu) * tr * tr) / (2 * mu);
+        /* sigma : C^-1 : sigma as deviatoric part dev : dev / (2 mu) plus
+         * volumetric part tr^2 / (3 (3 lambda + 2 mu)); a fluid (mu = 0)
+         * carries no deviatoric stress */
+        const double tr = s11 + s22 + s33, m = tr / 3;
+        const double dev = (s11 - m) * (s11 - m) + (s22 - m) * (s22 - m) +
+                           (s33 - m) * (s33 - m) +
+                           2 * (s12 * s12 + s13 * s13 + s23 * s23);
+        dens = rho * (v1 * v1 + v2 * v2 + v3 * v3) + (mu > 0 ? dev / (2 * mu) : 0.0) +
+               tr * tr / (3 * (3 * lam + 2 * mu));
       }
       const double y = 0.5 * J * wq * dens - comp;
       const double tt = sum + y;
