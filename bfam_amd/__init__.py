@@ -172,6 +172,8 @@ SIGNATURES = {
     "bfamx_batch_set_strategy": (C.c_int, [_vp, C.c_int]),
     "bfamx_set_kernel": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "bfamx_get_kernel": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "bfamx_set_occupancy": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "bfamx_get_occupancy": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "bfamx_timer_start": (C.c_int, [_vp]),
     "bfamx_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "bfamx_set_tuning": (C.c_int, [_vp, C.c_int]),
@@ -388,6 +390,17 @@ class Device:
         k = C.c_int()
         _check(lib().bfamx_get_kernel(self.h, int(precision), int(N), C.byref(k)))
         return k.value
+
+    def set_occupancy(self, precision, N, kind, level):
+        """resident-block cap of stage kernel `kind` (0 line, 1 node) for
+        (precision, N): 0 none, 1 three quarters, 2 half"""
+        _check(lib().bfamx_set_occupancy(self.h, int(precision), int(N), int(kind), int(level)))
+
+    def get_occupancy(self, precision, N, kind):
+        lv = C.c_int()
+        _check(lib().bfamx_get_occupancy(self.h, int(precision), int(N), int(kind),
+                                         C.byref(lv)))
+        return lv.value
 
     def set_persistent(self, on=1):
         _check(lib().bfamx_set_persistent(self.h, int(on)))

@@ -22,10 +22,13 @@ inline int elastic_line_epw(int N)
 }
 /* workgroups of the line stage for ntasks tasks (one task per wave) */
 constexpr int LINE_TASKS_PER_BLOCK = 1;
-/* resident one-wave blocks per CU of the elastic stage kernel */
-hipError_t line_blocks_per_cu(int prec, int N, int *blocks);
+/* resident one-wave blocks per CU of the elastic stage kernel, with dyn
+ * bytes of dynamic LDS per block */
+hipError_t line_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn = 0);
+/* dyn: dynamic LDS bytes per block that the kernel does not use, requested
+ * only to cap how many blocks share a CU */
 hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,
-                               hipStream_t s);
+                               hipStream_t s, unsigned dyn = 0);
 
 inline int acoustic_epb(int N)
 {
@@ -48,7 +51,9 @@ constexpr int NPREC = 5;
 /* fused stage or RHS-only launch over a task table.  N < 0: mixed orders
  * (one kernel handles every N in 3..7).  grid: number of workgroups. */
 hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
-                                hipStream_t s);
+                                hipStream_t s, unsigned dyn = 0);
+/* resident blocks per CU of the node-parallel stage kernel (N = 3..7) */
+hipError_t node_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn = 0);
 hipError_t launch_acoustic_stage(int prec, int N, const LaunchArgs &a,
                                  int grid, hipStream_t s);
 

@@ -51,6 +51,14 @@ struct bfamx_device
    * element, lanes own lines), 1 node-parallel (512-thread blocks, threads
    * own nodes) */
   signed char kchoice[NPREC][9];
+  /* resident-block cap per precision, order and kernel (0 line, 1
+   * node-parallel), chosen by bfamx_batch_autotune or bfamx_set_occupancy:
+   * 0 as many blocks as registers and LDS allow, 1 three quarters of that,
+   * 2 half.  A cap is a request of dynamic LDS the kernel does not use
+   * (occ_dyn, bytes per block, found once by the occupancy calculator;
+   * ~0u unknown) */
+  signed char occ[NPREC][9][2];
+  unsigned occ_dyn[NPREC][9][2][3];
   /* the per-order launches of a grouped batch run on side streams forked
    * from and joined back to `stream` (parallel branches when captured) */
   hipStream_t side[8];
@@ -186,6 +194,8 @@ extern "C" int bfamx_device_create(int hip_device, size_t hbm_budget_bytes,
   d->persistent = 0;
   memset(d->line_bpc, 0, sizeof(d->line_bpc));
   memset(d->kchoice, -1, sizeof(d->kchoice));
+  memset(d->occ, 0, sizeof(d->occ));
+  memset(d->occ_dyn, 0xff, sizeof(d->occ_dyn));
   /* this device's copy of the no-allocation staging buffer, looked up once
    * with the device current */
   d->scratch = NULL;
@@ -1009,6 +1019,47 @@ static int node_kernel(const bfamx_device_t *d, int prec, int N)
   return k < 0 ? default_node(prec, N) : k == 1;
 }
 
+/* dynamic LDS bytes per block that cap the stage kernel of (prec, N,
+ * node) at level lv (0: no cap, 1: 3/4 of the uncapped resident blocks per
+ * CU, 2: 1/2): the smallest request at which the occupancy calculator
+ * allows no more than that many blocks; 0 if the cap is not below the
+ * uncapped count or cannot be computed */
+static unsigned occ_dyn(bfamx_device_t *d, int prec, int N, int node, int lv)
+{
+  if (lv <= 0 || N < 3 || N > 7)
+    return 0;
+  unsigned &c = d->occ_dyn[prec][N][node][lv];
+  if (c != ~0u)
+    return c;
+  c = 0;
+  auto blocks = [&](unsigned dyn, int *nb) {
+    return node ? node_blocks_per_cu(prec, N, nb, dyn) : line_blocks_per_cu(prec, N, nb, dyn);
+  };
+  int b0 = 0;
+  if (blocks(0, &b0) != hipSuccess || b0 < 2)
+    return c;
+  const int target = lv == 1 ? (3 * b0) / 4 : b0 / 2;
+  if (target < 1 || target >= b0)
+    return c;
+  unsigned lo = 0, hi = 160 * 1024; /* blocks(hi) <= target: at most one block */
+  while (hi - lo > 256)
+  {
+    const unsigned mid = (lo + hi) / 2;
+    int nb = 0;
+    if (blocks(mid, &nb) != hipSuccess)
+      return c;
+    (nb <= target ? hi : lo) = mid;
+  }
+  int nb = 0;
+  if (blocks(hi, &nb) == hipSuccess && nb >= 1 && nb <= target)
+    c = hi;
+  return c;
+}
+static unsigned kernel_dyn(bfamx_device_t *d, int prec, int N, int node)
+{
+  return N >= 3 && N <= 7 ? occ_dyn(d, prec, N, node, d->occ[prec][N][node]) : 0;
+}
+
 /* does any order of the batch run the node-parallel kernel */
 static int batch_any_node(const bfamx_batch_t *b);
 
@@ -1022,11 +1073,12 @@ static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
     LaunchArgs b = a;
     b.tasks = s->d_tasks_node;
     b.ntasks = s->ntasks_node;
-    e = launch_elastic_stage(s->prec, s->N, b, grid_for(s->dev, b.ntasks), s->dev->stream);
+    e = launch_elastic_stage(s->prec, s->N, b, grid_for(s->dev, b.ntasks), s->dev->stream,
+                             kernel_dyn(s->dev, s->prec, s->N, 1));
   }
   else
     e = launch_elastic_line(s->prec, s->N, a, grid_line(s->dev, s->prec, s->N, a.ntasks),
-                            s->dev->stream);
+                            s->dev->stream, kernel_dyn(s->dev, s->prec, s->N, 0));
   if (e == hipSuccess && a.mode == MODE_STAGE)
     e = launch_clock_advance(s->d_sub, 1, s->dev->stream);
   if (e != hipSuccess)
@@ -1525,11 +1577,13 @@ static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n,
   {
     g.tasks = b->d_tasks_node + b->grpn_off[n];
     g.ntasks = b->grpn_cnt[n];
-    return launch_elastic_stage(b->prec, n, g, grid_for(b->dev, g.ntasks), st);
+    return launch_elastic_stage(b->prec, n, g, grid_for(b->dev, g.ntasks), st,
+                                kernel_dyn(b->dev, b->prec, n, 1));
   }
   g.tasks = a.tasks + b->grp_off[n];
   g.ntasks = b->grp_cnt[n];
-  return launch_elastic_line(b->prec, n, g, grid_line(b->dev, b->prec, n, g.ntasks), st);
+  return launch_elastic_line(b->prec, n, g, grid_line(b->dev, b->prec, n, g.ntasks), st,
+                             kernel_dyn(b->dev, b->prec, n, 0));
 }
 static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n)
 {
@@ -2150,23 +2204,37 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     return e == hipSuccess ? BFAMX_OK
                            : bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
   };
-  /* (a) kernel per order */
+  /* (a) kernel and resident-block cap per order: the line and the
+   * node-parallel kernel, each uncapped and capped at 3/4 and 1/2 of its
+   * resident blocks per CU (split-BF16x3 has the node-parallel kernel
+   * only); caps the occupancy calculator cannot place are skipped */
   for (int n = 0; n < 9 && !rc; ++n)
   {
     if (!b->grp_cnt[n] || (kc[n] >= 0 && !force))
       continue;
-    if (b->prec == PREC_FP32_BF16X3)
+    float best = 0.f;
+    int bk = -1, bl = 0;
+    for (int k = b->prec == PREC_FP32_BF16X3 ? 1 : 0; k < 2 && !rc; ++k)
+      for (int lv = 0; lv < 3 && !rc; ++lv)
+      {
+        if (lv > 0 && occ_dyn(d, b->prec, n, k, lv) == 0)
+          continue;
+        kc[n] = (signed char)k;
+        d->occ[b->prec][n][k] = (signed char)lv;
+        float ms = 0.f;
+        rc = time_it([&]() { return launch_order(b, a, n); }, &ms);
+        if (!rc && (bk < 0 || ms < best))
+          best = ms, bk = k, bl = lv;
+      }
+    if (rc)
     {
-      kc[n] = 1; /* the only kernel with matrix-core derivatives */
-      continue;
+      kc[n] = -1;
+      d->occ[b->prec][n][0] = d->occ[b->prec][n][1] = 0;
+      break;
     }
-    float ms[2] = {0, 0};
-    for (int k = 0; k < 2 && !rc; ++k)
-    {
-      kc[n] = (signed char)k;
-      rc = time_it([&]() { return launch_order(b, a, n); }, &ms[k]);
-    }
-    kc[n] = rc ? -1 : (ms[1] < ms[0] ? 1 : 0);
+    kc[n] = (signed char)bk;
+    d->occ[b->prec][n][0] = d->occ[b->prec][n][1] = 0;
+    d->occ[b->prec][n][bk] = (signed char)bl;
   }
   /* (b) one order-switching launch or one launch per order.  The
    * order-switching launch is the line kernel: it is a candidate only when
@@ -2210,6 +2278,24 @@ extern "C" int bfamx_set_kernel(bfamx_device_t *d, int prec, int N, int kind)
     return bfamx_fail(BFAMX_EUNSUPPORTED,
                       "bfamx_set_kernel: split-BF16x3 runs the node-parallel kernel only");
   d->kchoice[prec][N] = (signed char)kind;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_set_occupancy(bfamx_device_t *d, int prec, int N, int kind, int level)
+{
+  if (!d || prec < 0 || prec >= NPREC || N < 3 || N > 7 || kind < 0 || kind > 1 ||
+      level < 0 || level > 2)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_set_occupancy: bad arguments");
+  d->occ[prec][N][kind] = (signed char)level;
+  return BFAMX_OK;
+}
+
+extern "C" int bfamx_get_occupancy(const bfamx_device_t *d, int prec, int N, int kind,
+                                   int *level)
+{
+  if (!d || prec < 0 || prec >= NPREC || N < 3 || N > 7 || kind < 0 || kind > 1 || !level)
+    return bfamx_fail(BFAMX_EINVAL, "bfamx_get_occupancy: bad arguments");
+  *level = d->occ[prec][N][kind];
   return BFAMX_OK;
 }
 

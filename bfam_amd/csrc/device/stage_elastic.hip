@@ -619,14 +619,15 @@ elastic_stage_kernel(LaunchArgs a)
 }
 
 template <class P>
-static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s)
+static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s,
+                           unsigned dyn)
 {
   switch (N)
   {
 #define BFAMX_CASE(NN, EE)                                                     \
   case NN:                                                                     \
     hipLaunchKernelGGL((elastic_stage_kernel<NN, EE, P>), dim3(grid),          \
-                       dim3(EE * (NN + 1) * (NN + 1) * (NN + 1)), 0, s, a);    \
+                       dim3(EE * (NN + 1) * (NN + 1) * (NN + 1)), dyn, s, a);  \
     break;
     BFAMX_CASE(3, 8)
     BFAMX_CASE(4, 4)
@@ -639,15 +640,45 @@ static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
-                                hipStream_t s)
+template <class P>
+static hipError_t occupancy_node(int N, int *blocks, unsigned dyn)
+{
+  switch (N)
+  {
+#define BFAMX_CASE(NN, EE)                                                     \
+  case NN:                                                                     \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
+        blocks, elastic_stage_kernel<NN, EE, P>, EE * (NN + 1) * (NN + 1) * (NN + 1), dyn);
+    BFAMX_CASE(3, 8)
+    BFAMX_CASE(4, 4)
+    BFAMX_CASE(5, 2)
+    BFAMX_CASE(6, 1)
+    BFAMX_CASE(7, 1)
+#undef BFAMX_CASE
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t node_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn)
 {
   switch (prec)
   {
-  case PREC_FP32: return launch_n<P32>(N, a, grid, s);
-  case PREC_FP32_RHS: return launch_n<P32R>(N, a, grid, s);
-  case PREC_FP32_BF16X3: return launch_n<P32M>(N, a, grid, s);
-  default: return launch_n<P64>(N, a, grid, s);
+  case PREC_FP32: return occupancy_node<P32>(N, blocks, dyn);
+  case PREC_FP32_RHS: return occupancy_node<P32R>(N, blocks, dyn);
+  case PREC_FP32_BF16X3: return occupancy_node<P32M>(N, blocks, dyn);
+  default: return occupancy_node<P64>(N, blocks, dyn);
+  }
+}
+
+hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
+                                hipStream_t s, unsigned dyn)
+{
+  switch (prec)
+  {
+  case PREC_FP32: return launch_n<P32>(N, a, grid, s, dyn);
+  case PREC_FP32_RHS: return launch_n<P32R>(N, a, grid, s, dyn);
+  case PREC_FP32_BF16X3: return launch_n<P32M>(N, a, grid, s, dyn);
+  default: return launch_n<P64>(N, a, grid, s, dyn);
   }
 }
 

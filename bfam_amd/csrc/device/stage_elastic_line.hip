@@ -891,14 +891,14 @@ __global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_mixed(LaunchArgs a
 
 template <class P>
 static hipError_t launch_line_n(int N, const LaunchArgs &a, int grid,
-                                hipStream_t s)
+                                hipStream_t s, unsigned dyn)
 {
   switch (N)
   {
 #define BFAMX_CASE(NN)                                                         \
   case NN:                                                                     \
     hipLaunchKernelGGL((elastic_line_kernel<NN, P, PART_FUSED>), dim3(grid),   \
-                       dim3(64 * LINE_WPB), 0, s, a);                          \
+                       dim3(64 * LINE_WPB), dyn, s, a);                        \
     break;
     BFAMX_CASE(3)
     BFAMX_CASE(4)
@@ -915,14 +915,14 @@ static hipError_t launch_line_n(int N, const LaunchArgs &a, int grid,
 }
 
 template <class P>
-static hipError_t occupancy_n(int N, int *blocks)
+static hipError_t occupancy_n(int N, int *blocks, unsigned dyn)
 {
   switch (N)
   {
 #define BFAMX_CASE(NN)                                                         \
   case NN:                                                                     \
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
-        blocks, elastic_line_kernel<NN, P, PART_FUSED>, 64 * LINE_WPB, 0);
+        blocks, elastic_line_kernel<NN, P, PART_FUSED>, 64 * LINE_WPB, dyn);
     BFAMX_CASE(3)
     BFAMX_CASE(4)
     BFAMX_CASE(5)
@@ -931,29 +931,30 @@ static hipError_t occupancy_n(int N, int *blocks)
 #undef BFAMX_CASE
   default:
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, elastic_line_mixed<P>,
-                                                        64 * LINE_WPB, 0);
+                                                        64 * LINE_WPB, dyn);
   }
 }
 
-hipError_t line_blocks_per_cu(int prec, int N, int *blocks)
+hipError_t line_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn)
 {
   switch (prec)
   {
-  case PREC_FP32: return occupancy_n<P32>(N, blocks);
-  case PREC_FP32_RHS: return occupancy_n<P32R>(N, blocks);
-  default: return occupancy_n<P64>(N, blocks);
+  case PREC_FP32_BF16X3: return hipErrorInvalidValue;
+  case PREC_FP32: return occupancy_n<P32>(N, blocks, dyn);
+  case PREC_FP32_RHS: return occupancy_n<P32R>(N, blocks, dyn);
+  default: return occupancy_n<P64>(N, blocks, dyn);
   }
 }
 
 hipError_t launch_elastic_line(int prec, int N, const LaunchArgs &a, int grid,
-                               hipStream_t s)
+                               hipStream_t s, unsigned dyn)
 {
   switch (prec)
   {
   case PREC_FP32_BF16X3: return hipErrorInvalidValue; /* node-parallel kernel only */
-  case PREC_FP32: return launch_line_n<P32>(N, a, grid, s);
-  case PREC_FP32_RHS: return launch_line_n<P32R>(N, a, grid, s);
-  default: return launch_line_n<P64>(N, a, grid, s);
+  case PREC_FP32: return launch_line_n<P32>(N, a, grid, s, dyn);
+  case PREC_FP32_RHS: return launch_line_n<P32R>(N, a, grid, s, dyn);
+  default: return launch_line_n<P64>(N, a, grid, s, dyn);
   }
 }
 

@@ -309,9 +309,11 @@ int bfamx_batch_capture(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_launch_graph(bfamx_batch_t *b);
 /*
  * Autotuning on the batch's own data, for elastic batches.  For every order
- * present it times `trials` stages of two stage kernels and keeps the
- * faster: the line kernel (one wavefront per element, lanes own node lines)
- * and the node-parallel kernel (512-thread blocks, threads own nodes).  With
+ * present it times `trials` stages of two stage kernels, the line kernel
+ * (one wavefront per element, lanes own node lines) and the node-parallel
+ * kernel (512-thread blocks, threads own nodes), each with as many resident
+ * blocks per CU as registers and LDS allow and capped at 3/4 and 1/2 of
+ * that, and keeps the fastest kernel and cap.  With
  * mixed orders it then chooses between one launch of the order-switching
  * kernel and one launch per order.  The state and results are left
  * unchanged.  Choices are cached on the device, per precision and order and
@@ -327,6 +329,14 @@ int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order);
  * at N >= 5, the line kernel otherwise); get returns the kernel that runs */
 int bfamx_set_kernel(bfamx_device_t *dev, int precision, int N, int kind);
 int bfamx_get_kernel(const bfamx_device_t *dev, int precision, int N, int *kind);
+/* resident-block cap of a stage kernel (kind 0 line, 1 node-parallel) for
+ * (precision, order N = 3..7) by hand: level 0 none, 1 three quarters, 2
+ * half of the blocks per CU that registers and LDS allow (a request of
+ * dynamic LDS the kernel does not use).  No counterpart in the reference:
+ * the tile and occupancy choice its OCCA kernels left to the caller. */
+int bfamx_set_occupancy(bfamx_device_t *dev, int precision, int N, int kind, int level);
+int bfamx_get_occupancy(const bfamx_device_t *dev, int precision, int N, int kind,
+                        int *level);
 
 /* RHS L(q, t) of every member into host arrays rhs[i][field] (parity
  * checks); a glued batch evaluates its mortars first */

@@ -176,6 +176,28 @@ def test_graph_replay_matches_eager(dev):
     a.close()
 
 
+@pytest.mark.parametrize("N", [3, 7])
+def test_occupancy_caps_are_bit_identical(dev, kernel, N):
+    """Capping the resident blocks per CU (dynamic LDS the kernel does not
+    use) changes residency only: every level gives the same bits, and the
+    autotuner leaves a valid kernel and level for the order."""
+    print(f"occupancy caps N={N} kernel={kernel}", flush=True)
+    cfg, q0 = random_elastic((4, 3, 3), N, seed=50 + N)
+    res = []
+    for lv in (0, 1, 2):
+        dev.set_occupancy(bx.FP32, N, kernel, lv)
+        assert dev.get_occupancy(bx.FP32, N, kernel) == lv
+        s = W.upload(dev, cfg, bx.FP32)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(2)
+        res.append(s.download())
+        s.close()
+    dev.set_occupancy(bx.FP32, N, kernel, 0)
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+    with pytest.raises(bx.BfamxError):
+        dev.set_occupancy(bx.FP32, N, kernel, 3)
+
+
 def test_xcd_remap_invariant(dev):
     """Every block order gives the same bits (at N=7 the 5x5x3 brick has one
     whole group of 64 elements and a partial one in mode 2, the 8x8x2 brick
