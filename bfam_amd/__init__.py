@@ -381,12 +381,13 @@ class Device:
         _check(lib().bfamx_set_tuning(self.h, int(xcd_remap)))
 
     def set_kernel(self, precision, N, kind):
-        """elastic stage kernel for (precision, N): 0 line, 1 node, -1 unknown
-        (the per-precision default until autotuned)"""
+        """elastic stage kernel for (precision, N): 0 line, 1 node, 2 node on a
+        resident grid, -1 unknown (the per-precision default until autotuned)"""
         _check(lib().bfamx_set_kernel(self.h, int(precision), int(N), int(kind)))
 
     def get_kernel(self, precision, N):
-        """the elastic stage kernel that runs for (precision, N): 0 line, 1 node"""
+        """the elastic stage kernel that runs for (precision, N): 0 line, 1 node,
+        2 node on a resident grid"""
         k = C.c_int()
         _check(lib().bfamx_get_kernel(self.h, int(precision), int(N), C.byref(k)))
         return k.value

@@ -49,7 +49,8 @@ struct bfamx_device
   /* elastic stage kernel per precision and order, chosen by
    * bfamx_batch_autotune: -1 unknown (line), 0 line (one wavefront per
    * element, lanes own lines), 1 node-parallel (512-thread blocks, threads
-   * own nodes) */
+   * own nodes), 2 node-parallel on a resident grid (as many blocks as fit
+   * the CUs at once, each looping over tasks) */
   signed char kchoice[NPREC][9];
   /* resident-block cap per precision, order and kernel (0 line, 1
    * node-parallel), chosen by bfamx_batch_autotune or bfamx_set_occupancy:
@@ -1016,7 +1017,12 @@ static int node_kernel(const bfamx_device_t *d, int prec, int N)
   if (prec == PREC_FP32_BF16X3)
     return 1;
   const signed char k = d->kchoice[prec][N];
-  return k < 0 ? default_node(prec, N) : k == 1;
+  return k < 0 ? default_node(prec, N) : k >= 1;
+}
+/* the node-parallel kernel of (prec, N) runs on a resident grid */
+static int node_resident(const bfamx_device_t *d, int prec, int N)
+{
+  return N >= 3 && N <= 7 && d->kchoice[prec][N] == 2;
 }
 
 /* dynamic LDS bytes per block that cap the stage kernel of (prec, N,
@@ -1059,6 +1065,21 @@ static unsigned kernel_dyn(bfamx_device_t *d, int prec, int N, int node)
 {
   return N >= 3 && N <= 7 ? occ_dyn(d, prec, N, node, d->occ[prec][N][node]) : 0;
 }
+/* grid of a node-parallel launch over ntasks tasks: one block per task, or,
+ * on a resident grid, the blocks that fit the CUs at once (each then takes
+ * every grid-th task; its setup and block launch are paid once per CU slot
+ * instead of once per task) */
+static int grid_node(bfamx_device_t *d, int prec, int N, int ntasks)
+{
+  int g = grid_for(d, ntasks);
+  if (!node_resident(d, prec, N))
+    return g;
+  int bpc = 0;
+  if (node_blocks_per_cu(prec, N, &bpc, kernel_dyn(d, prec, N, 1)) == hipSuccess && bpc > 0 &&
+      (long long)bpc * d->num_cu < g)
+    g = bpc * d->num_cu;
+  return g;
+}
 
 /* does any order of the batch run the node-parallel kernel */
 static int batch_any_node(const bfamx_batch_t *b);
@@ -1073,8 +1094,8 @@ static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
     LaunchArgs b = a;
     b.tasks = s->d_tasks_node;
     b.ntasks = s->ntasks_node;
-    e = launch_elastic_stage(s->prec, s->N, b, grid_for(s->dev, b.ntasks), s->dev->stream,
-                             kernel_dyn(s->dev, s->prec, s->N, 1));
+    e = launch_elastic_stage(s->prec, s->N, b, grid_node(s->dev, s->prec, s->N, b.ntasks),
+                             s->dev->stream, kernel_dyn(s->dev, s->prec, s->N, 1));
   }
   else
     e = launch_elastic_line(s->prec, s->N, a, grid_line(s->dev, s->prec, s->N, a.ntasks),
@@ -1577,7 +1598,7 @@ static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n,
   {
     g.tasks = b->d_tasks_node + b->grpn_off[n];
     g.ntasks = b->grpn_cnt[n];
-    return launch_elastic_stage(b->prec, n, g, grid_for(b->dev, g.ntasks), st,
+    return launch_elastic_stage(b->prec, n, g, grid_node(b->dev, b->prec, n, g.ntasks), st,
                                 kernel_dyn(b->dev, b->prec, n, 1));
   }
   g.tasks = a.tasks + b->grp_off[n];
@@ -2204,23 +2225,25 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     return e == hipSuccess ? BFAMX_OK
                            : bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
   };
-  /* (a) kernel and resident-block cap per order: the line and the
-   * node-parallel kernel, each uncapped and capped at 3/4 and 1/2 of its
-   * resident blocks per CU (split-BF16x3 has the node-parallel kernel
-   * only); caps the occupancy calculator cannot place are skipped */
+  /* (a) kernel and resident-block cap per order: the line kernel, the
+   * node-parallel kernel with one block per task and on a resident grid,
+   * each uncapped and capped at 3/4 and 1/2 of its resident blocks per CU
+   * (split-BF16x3 has the node-parallel kernel only); caps the occupancy
+   * calculator cannot place are skipped */
   for (int n = 0; n < 9 && !rc; ++n)
   {
     if (!b->grp_cnt[n] || (kc[n] >= 0 && !force))
       continue;
     float best = 0.f;
     int bk = -1, bl = 0;
-    for (int k = b->prec == PREC_FP32_BF16X3 ? 1 : 0; k < 2 && !rc; ++k)
+    for (int k = b->prec == PREC_FP32_BF16X3 ? 1 : 0; k < 3 && !rc; ++k)
       for (int lv = 0; lv < 3 && !rc; ++lv)
       {
-        if (lv > 0 && occ_dyn(d, b->prec, n, k, lv) == 0)
+        const int node = k > 0;
+        if (lv > 0 && occ_dyn(d, b->prec, n, node, lv) == 0)
           continue;
         kc[n] = (signed char)k;
-        d->occ[b->prec][n][k] = (signed char)lv;
+        d->occ[b->prec][n][node] = (signed char)lv;
         float ms = 0.f;
         rc = time_it([&]() { return launch_order(b, a, n); }, &ms);
         if (!rc && (bk < 0 || ms < best))
@@ -2234,7 +2257,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     }
     kc[n] = (signed char)bk;
     d->occ[b->prec][n][0] = d->occ[b->prec][n][1] = 0;
-    d->occ[b->prec][n][bk] = (signed char)bl;
+    d->occ[b->prec][n][bk > 0] = (signed char)bl;
   }
   /* (b) one order-switching launch or one launch per order.  The
    * order-switching launch is the line kernel: it is a candidate only when
@@ -2272,7 +2295,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
 
 extern "C" int bfamx_set_kernel(bfamx_device_t *d, int prec, int N, int kind)
 {
-  if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || kind < -1 || kind > 1)
+  if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || kind < -1 || kind > 2)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_set_kernel: bad arguments");
   if (prec == PREC_FP32_BF16X3 && kind == 0)
     return bfamx_fail(BFAMX_EUNSUPPORTED,
@@ -2303,7 +2326,7 @@ extern "C" int bfamx_get_kernel(const bfamx_device_t *d, int prec, int N, int *k
 {
   if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || !kind)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_get_kernel: bad arguments");
-  *kind = node_kernel(d, prec, N);
+  *kind = node_kernel(d, prec, N) ? 1 + node_resident(d, prec, N) : 0;
   return BFAMX_OK;
 }
 

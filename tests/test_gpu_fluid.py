@@ -24,7 +24,7 @@ def dev():
     d.close()
 
 
-@pytest.fixture(params=[0, 1], ids=["line", "node"])
+@pytest.fixture(params=[0, 1, 2], ids=["line", "node", "node_resident"])
 def kernel(request, dev):
     for prec in PRECS:
         for N in range(3, 8):

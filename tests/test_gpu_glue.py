@@ -50,7 +50,7 @@ def teardown(subs, glue, b):
         s.close()
 
 
-@pytest.fixture(params=[0, 1], ids=["line", "node"])
+@pytest.fixture(params=[0, 1, 2], ids=["line", "node", "node_resident"])
 def kernel(request, dev):
     for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
         for N in range(3, 8):

@@ -22,7 +22,7 @@ def dev():
     d.close()
 
 
-@pytest.fixture(params=[0, 1], ids=["line", "node"])
+@pytest.fixture(params=[0, 1, 2], ids=["line", "node", "node_resident"])
 def kernel(request, dev):
     """run with the line (0) or the node-parallel (1) elastic stage kernel"""
     for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
@@ -184,18 +184,42 @@ def test_occupancy_caps_are_bit_identical(dev, kernel, N):
     print(f"occupancy caps N={N} kernel={kernel}", flush=True)
     cfg, q0 = random_elastic((4, 3, 3), N, seed=50 + N)
     res = []
+    kind = min(kernel, 1)  # the resident grid shares the node kernel's cap
     for lv in (0, 1, 2):
-        dev.set_occupancy(bx.FP32, N, kernel, lv)
-        assert dev.get_occupancy(bx.FP32, N, kernel) == lv
+        dev.set_occupancy(bx.FP32, N, kind, lv)
+        assert dev.get_occupancy(bx.FP32, N, kind) == lv
         s = W.upload(dev, cfg, bx.FP32)
         s.set_fields(q0, 0.0, cfg["dt"])
         s.step(2)
         res.append(s.download())
         s.close()
-    dev.set_occupancy(bx.FP32, N, kernel, 0)
+    dev.set_occupancy(bx.FP32, N, kind, 0)
     assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
     with pytest.raises(bx.BfamxError):
-        dev.set_occupancy(bx.FP32, N, kernel, 3)
+        dev.set_occupancy(bx.FP32, N, kind, 3)
+
+
+@pytest.mark.parametrize("N,n", [(3, 18), (5, 12), (7, 12)])
+@pytest.mark.parametrize("prec", [bx.FP32, bx.FP64])
+def test_resident_grid_is_bit_identical(dev, N, n, prec):
+    """The node-parallel kernel on a resident grid (each block looping over
+    tasks; these bricks have more tasks than blocks fit the CUs at once)
+    gives the bits of one block per task"""
+    print(f"resident grid N={N} n={n} prec={prec}", flush=True)
+    cfg = W.elastic3d_ricker(n=n, N=N, nsteps=20)
+    m = cfg["mesh"]
+    q0 = np.random.default_rng(N).standard_normal((9, m.K * m.Np))
+    res = []
+    for k in (1, 2):
+        dev.set_kernel(prec, N, k)
+        assert dev.get_kernel(prec, N) == k
+        s = W.upload(dev, cfg, prec)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(2)
+        res.append(s.download())
+        s.close()
+    dev.set_kernel(prec, N, -1)
+    assert np.array_equal(res[0], res[1])
 
 
 def test_xcd_remap_invariant(dev):
