@@ -994,35 +994,38 @@ static int grid_for(const bfamx_device_t *d, int ntasks)
   return d->num_xcd * ((ntasks + d->num_xcd - 1) / d->num_xcd);
 }
 
-/* the stage kernel of an order not (yet) autotuned: the faster one in the
- * per-order measurements of both kernels on bricks of about 124 M DOF
- * (node-parallel for FP32_RHS at every order, for FP32 at N = 4, 6, 7 and
- * for FP64 from N = 5) */
-static int default_node(int prec, int N)
+/* the stage kernel of an order not (yet) autotuned (0 line, 1
+ * node-parallel, 2 node-parallel on a resident grid): what the autotuner
+ * kept on bricks of about 124 M DOF (FP32: line at N = 3, resident grid at
+ * N = 4..6, node-parallel at N = 7; FP64: line at N = 3, 4, resident grid
+ * from N = 5), and the node-parallel kernel for FP32_RHS and split-BF16x3 */
+static int default_kind(int prec, int N)
 {
   switch (prec)
   {
-  case PREC_FP32: return N != 3 && N != 5;
+  case PREC_FP32: return N == 3 ? 0 : (N == 7 ? 1 : 2);
   case PREC_FP32_RHS: return 1;
-  case PREC_FP64: return N >= 5;
+  case PREC_FP32_BF16X3: return 1;
+  case PREC_FP64: return N >= 5 ? 2 : 0;
   default: return 0;
   }
 }
 
-static int node_kernel(const bfamx_device_t *d, int prec, int N)
+static int kernel_kind(const bfamx_device_t *d, int prec, int N)
 {
   if (N < 3 || N > 7)
     return 0;
-  /* the matrix-core derivatives exist in the node-parallel kernel only */
-  if (prec == PREC_FP32_BF16X3)
-    return 1;
   const signed char k = d->kchoice[prec][N];
-  return k < 0 ? default_node(prec, N) : k >= 1;
+  return k < 0 ? default_kind(prec, N) : k;
+}
+static int node_kernel(const bfamx_device_t *d, int prec, int N)
+{
+  return kernel_kind(d, prec, N) >= 1;
 }
 /* the node-parallel kernel of (prec, N) runs on a resident grid */
 static int node_resident(const bfamx_device_t *d, int prec, int N)
 {
-  return N >= 3 && N <= 7 && d->kchoice[prec][N] == 2;
+  return kernel_kind(d, prec, N) == 2;
 }
 
 /* dynamic LDS bytes per block that cap the stage kernel of (prec, N,
@@ -2326,7 +2329,7 @@ extern "C" int bfamx_get_kernel(const bfamx_device_t *d, int prec, int N, int *k
 {
   if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || !kind)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_get_kernel: bad arguments");
-  *kind = node_kernel(d, prec, N) ? 1 + node_resident(d, prec, N) : 0;
+  *kind = kernel_kind(d, prec, N);
   return BFAMX_OK;
 }
 
