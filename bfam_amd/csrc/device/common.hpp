@@ -210,7 +210,7 @@ struct GlueArgs
   const SubDev *subs; /* the batch's descriptors */
   const MortarGroup *groups;
   int ngroups;
-  const void *ops; /* glue_ops_index table, arithmetic type */
+  const void *ops; /* glue_ops8_index table, arithmetic type */
   void *buf;       /* face fluctuation blocks */
 };
 
@@ -222,6 +222,15 @@ __host__ __device__ constexpr int glue_ops_index(int kind, int Na, int Nb, int k
   return ((((Na - 1) * 8 + (Nb - 1)) * 2 + kind) * 5 + k) * 81;
 }
 constexpr int GLUE_OPS_SIZE = 64 * 2 * 5 * 81;
+
+/* the operators between orders 1..7 (the orders a face of the mortar
+ * kernel can have), each padded to 8 x 8: entry (row, col) at row + 8 col,
+ * zero outside the actual size; the mortar kernel's table */
+__host__ __device__ constexpr int glue_ops8_index(int kind, int Na, int Nb, int k)
+{
+  return ((((Na - 1) * 7 + (Nb - 1)) * 2 + kind) * 5 + k) * 64;
+}
+constexpr int GLUE_OPS8_SIZE = 49 * 2 * 5 * 64;
 
 /* one work item of a stage launch: elements [e0, e0+ne) of subdomain sub */
 struct Task

@@ -25,18 +25,21 @@
 namespace bfamx
 {
 
-/* Face data and 1-D operators of the mortar kernel live in LDS padded to
- * order 7: node (a, b) of a face at a + 8 b, operator entry (row, col) at
- * row + 8 col, zero outside the actual size.  Every contraction then runs a
- * fixed eight terms, fully unrolled; the padded terms add exact zeros after
- * the actual ones, so the sums are those of the actual size. */
+/* Face data of the mortar kernel live in LDS padded to order 7: node
+ * (a, b) of a face at a + 8 b, zero outside the actual size; the six fields
+ * of a face lie GS words apart (72 rather than 64: in a column step the
+ * lanes of different fields then read different banks).  The 1-D operators come padded the same way
+ * (entry (row, col) at row + 8 col, zero outside the actual size) from the
+ * glue_ops8_index table.  Every contraction runs a fixed eight terms; the
+ * padded terms add exact zeros after the actual ones, so the sums are those
+ * of the actual size. */
 constexpr int GP = 8;
+constexpr int GS = 72;
 
 /* face trace (v, sigma[.][A]) of one element face into LDS, lane = padded
  * node */
 template <class TS, class TC, class SD>
-__device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
-                                           TC (*tr)[GLUE_MAXFP], int lane)
+__device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f, TC *tr, int lane)
 {
   const int n1 = sd.N + 1, np = n1 * n1 * n1, A = f >> 1;
   const int a = lane % GP, b = lane / GP;
@@ -44,7 +47,7 @@ __device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
   {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-      tr[c][lane] = TC(0);
+      tr[c * GS + lane] = TC(0);
     return;
   }
   const int parity = (int)(cglob((const Clock *)sd.clock)->stage & 1);
@@ -53,57 +56,86 @@ __device__ __forceinline__ void load_trace(const SD &sd, int32_t e, int f,
 #pragma unroll
   for (int c = 0; c < 3; ++c)
   {
-    tr[c][lane] = (TC)q[c * np];
-    tr[3 + c][lane] = (TC)q[sfield(c, A) * np];
+    tr[c * GS + lane] = (TC)q[c * np];
+    tr[(3 + c) * GS + lane] = (TC)q[sfield(c, A) * np];
   }
 }
 
-/* the six 1-D operators of one mortar: interpolation of the s face (Is) and
- * of the b face along each axis (Ia, Ib) to the mortar grid, and the mass
- * projections back (Ws; Wa, Wb) */
-enum
-{
-  OP_IS = 0,
-  OP_IA = 1,
-  OP_IB = 2,
-  OP_WS = 3,
-  OP_WA = 4,
-  OP_WB = 5
-};
-
-/* sum_{t < 8} M[r + 8 t] x[t * sx + o], fully unrolled */
+/* One line of a tensor step: the eight values X[xi + sx t] (t < 8) of a
+ * padded face, times the wave-uniform operator M (entry (r, t) at r + 8 t,
+ * read with scalar loads): y[r] = sum_t M[r + 8 t] x[t], each sum in
+ * ascending t. */
 template <class TC>
-__device__ __forceinline__ TC contract(const TC *M, int r, const TC *x, int o, int sx)
+__device__ __forceinline__ void line_apply(cptr<TC> M, const TC *X, int xi, int sx, TC (&y)[GP])
 {
-  TC acc = 0;
+  TC x[GP];
 #pragma unroll
   for (int t = 0; t < GP; ++t)
-    acc += M[r + GP * t] * x[o + sx * t];
-  return acc;
+    x[t] = X[xi + sx * t];
+#pragma unroll
+  for (int r = 0; r < GP; ++r)
+  {
+    TC acc = 0;
+#pragma unroll
+    for (int t = 0; t < GP; ++t)
+      acc += M[r + GP * t] * x[t];
+    y[r] = acc;
+  }
+}
+
+/* the same line, written back in place (Y[xi + sx r] = y[r]) */
+template <class TC>
+__device__ __forceinline__ void line_step(cptr<TC> M, TC *X, int xi, int sx)
+{
+  TC y[GP];
+  line_apply<TC>(M, X, xi, sx, y);
+#pragma unroll
+  for (int r = 0; r < GP; ++r)
+    X[xi + sx * r] = y[r];
+}
+
+/* order the LDS accesses of one wavefront: the compiler keeps them on
+ * their side of this point, and the hardware runs them in issue order */
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 /*
  * One workgroup per interface group, one wavefront per mortar (wave m of
  * four takes mortar m; a conforming group uses wave 0 only).  Every global
- * read of the group (the b trace, the traces of its mortars' s faces and
- * their operators) is issued up front into LDS, so the group waits on
- * memory once.  The tensor products run sum-factorised, one axis per step
- * with the partial result in LDS: lane (g, j) first contracts along i, then
- * lane (g, h) along j.  Each step sums in the same order as the nested form,
- * so the results are those of the direct double sum.  The big face's four
- * contributions are added in mortar order by wave 0, as a single wave
- * walking the mortars in turn would add them.
+ * read of the group (the b trace and the traces of its mortars' s faces)
+ * is issued up front into LDS, so the group waits on memory once; the
+ * operators are read with scalar loads as they are used.
+ *
+ * The tensor products run sum-factorised, one axis per step, and each step
+ * is a set of line tasks: lane (c, l) = (lane / 8, lane % 8), lanes 0..47,
+ * applies the step's operator to line l of field c (a row of eight nodes
+ * for the first axis, a column for the second) and writes the eight results
+ * back over the line it read, so no lane reads what another writes within
+ * a step.  The s side runs first, then the b side, so the operator is the
+ * same for every lane.  Between the steps the Riemann solves run one lane
+ * per mortar node.  Each sum runs in the same order as the direct double
+ * sum.  The big face's four contributions are added in mortar order by
+ * wave 0, as a single wave walking the mortars in turn would add them.
+ *
+ * Between its steps a wave reads only its own LDS, so the steps are
+ * ordered by a wavefront barrier (LDS operations of one wave execute in
+ * issue order); the workgroup synchronises twice, after the loads (every
+ * wave reads the b trace) and before the big face's sum.
  */
 constexpr int GLUE_WAVES = 4;
 
 template <class TS, class TC>
 __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
 {
-  __shared__ TC tb[6][GLUE_MAXFP];                     /* b trace */
-  __shared__ TC ts[GLUE_WAVES][6][GLUE_MAXFP];         /* s traces */
-  __shared__ TC sop[GLUE_WAVES][6][GP * GP];           /* operators */
-  __shared__ TC t1[GLUE_WAVES][6][GLUE_MAXFP], t2[GLUE_WAVES][6][GLUE_MAXFP];
-  __shared__ TC fs[GLUE_WAVES][6][GLUE_MAXFP], fb[GLUE_WAVES][6][GLUE_MAXFP];
+  /* b trace, s traces, and per wave the s- and b-side values of each step
+   * (interpolated traces, then the fluctuations, then their projections) */
+  __shared__ __attribute__((aligned(16))) TC tb[6 * GS];
+  __shared__ __attribute__((aligned(16))) TC ts[GLUE_WAVES][6 * GS];
+  __shared__ __attribute__((aligned(16))) TC tw[GLUE_WAVES][2][6 * GS];
   const int gi = blockIdx.x;
   if (gi >= a.ngroups)
     return;
@@ -111,8 +143,8 @@ __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
   /* the wave index is uniform in the wave */
   const int m = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lo = lane % GP, hi = lane / GP; /* padded (row, column) */
-  /* group, descriptors, clocks and materials are read-only here: constant
-   * address space, scalar loads */
+  /* group, descriptors, clocks, materials and operators are read-only
+   * here: constant address space, scalar loads */
   cptr<MortarGroup> gp = cglob(a.groups) + gi;
   const int32_t b_sub = gp->b_sub, b_elem = gp->b_elem, b_off = gp->b_off;
   const int b_face = gp->b_face, nm = gp->nm, quad0 = gp->quad0;
@@ -120,14 +152,20 @@ __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
   const int Nb = B.N, nb1 = Nb + 1, nfb = nb1 * nb1;
   const int A = b_face >> 1;
   const TC sgs = (b_face & 1) ? TC(-1) : TC(1); /* normal sign of the s side */
-  gptr<const TC> ops = glob(static_cast<const TC *>(a.ops));
   gptr<TC> buf = glob(static_cast<TC *>(a.buf));
   const bool mine = m < nm; /* this wave has a mortar */
 
   /* ---- every load of the group ---- */
   if (m == 0)
     load_trace<TS, TC>(B, b_elem, b_face, tb, lane);
-  int Ns = 0;
+  /* the mortar's orders and its six operators (glue_ops8_index blocks):
+   * interpolation of the s face (Is) and of the b face along each axis
+   * (Ia, Ib) to the mortar grid, and the mass projections back (Ws; Wa,
+   * Wb).  Taken before the barrier below: computed after it, the waves
+   * m > 0 of hanging groups were measured to use wrong operators, a
+   * compiler effect found by bisection and not understood further; the
+   * hanging-mortar parity tests cover it. */
+  int Ns = 0, oIs = 0, oIa = 0, oIb = 0, oWs = 0, oWa = 0, oWb = 0;
   if (mine)
   {
     const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
@@ -137,94 +175,105 @@ __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
     const int quad = nm == 1 ? quad0 : m;
     const int kI = quad < 0 ? 0 : 3 + (quad & 1), kJ = quad < 0 ? 0 : 3 + ((quad >> 1) & 1);
     const int kA = quad < 0 ? 0 : 1 + (quad & 1), kB = quad < 0 ? 0 : 1 + ((quad >> 1) & 1);
-    /* block offset, rows and columns of each operator (column-major) */
-    const int base[6] = {glue_ops_index(0, Ns, Ng, 0), glue_ops_index(0, Nb, Ng, kI),
-                         glue_ops_index(0, Nb, Ng, kJ), glue_ops_index(1, Ng, Ns, 0),
-                         glue_ops_index(1, Ng, Nb, kA), glue_ops_index(1, Ng, Nb, kB)};
-    const int rows[6] = {Ng + 1, Ng + 1, Ng + 1, Ns + 1, Nb + 1, Nb + 1};
-    const int cols[6] = {Ns + 1, Nb + 1, Nb + 1, Ng + 1, Ng + 1, Ng + 1};
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-      sop[m][k][lane] = (lo < rows[k] && hi < cols[k]) ? ops[base[k] + lo + rows[k] * hi]
-                                                        : TC(0);
+    oIs = glue_ops8_index(0, Ns, Ng, 0);
+    oIa = glue_ops8_index(0, Nb, Ng, kI);
+    oIb = glue_ops8_index(0, Nb, Ng, kJ);
+    oWs = glue_ops8_index(1, Ng, Ns, 0);
+    oWa = glue_ops8_index(1, Ng, Nb, kA);
+    oWb = glue_ops8_index(1, Ng, Nb, kB);
   }
   cptr<TC> Mb = (cptr<TC>)mat_tab<TC>(B) + (size_t)b_elem * MAT_STRIDE;
   const TC Zpb = Mb[MAT_ZP], Zsb = Mb[MAT_ZS];
   __syncthreads();
 
-  const int ns1 = Ns + 1, nfs = ns1 * ns1;
-  const TC *Is = sop[m][OP_IS], *Ia = sop[m][OP_IA], *Ib = sop[m][OP_IB];
-  const TC *Ws = sop[m][OP_WS], *Wa = sop[m][OP_WA], *Wb = sop[m][OP_WB];
-  TC Zps = 0, Zss = 0, izp = 0, izs = 0;
   if (mine)
   {
     const auto &S = *(cglob(a.subs) + gp->s_sub[m]);
+    const int ns1 = Ns + 1, nfs = ns1 * ns1;
+    cptr<TC> ops = cglob(static_cast<const TC *>(a.ops));
+    cptr<TC> Is = ops + oIs, Ia = ops + oIa, Ib = ops + oIb, Ws = ops + oWs, Wa = ops + oWa,
+             Wb = ops + oWb;
     cptr<TC> Ms = (cptr<TC>)mat_tab<TC>(S) + (size_t)gp->s_elem[m] * MAT_STRIDE;
-    Zps = Ms[MAT_ZP];
-    Zss = Ms[MAT_ZS];
-    izp = TC(1) / (Zps + Zpb);
+    const TC Zps = Ms[MAT_ZP], Zss = Ms[MAT_ZS];
+    const TC izp = TC(1) / (Zps + Zpb);
     /* two fluids: no shear waves, and no tangential fluctuation */
-    izs = Zss + Zsb > TC(0) ? TC(1) / (Zss + Zsb) : TC(0);
-    /* interpolation to the mortar grid, first axis: lane (g, j) = (lo, hi) */
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
+    const TC izs = Zss + Zsb > TC(0) ? TC(1) / (Zss + Zsb) : TC(0);
+    TC *ws = tw[m][0], *wb = tw[m][1];
+    const int c = hi, l = lo; /* line task: field c, line l */
+    const bool task = lane < 6 * GP;
+
+    /* interpolation to the mortar grid, first axis: row l of each face */
+    if (task)
     {
-      t1[m][c][lane] = contract(Is, lo, ts[m][c], GP * hi, 1);
-      t2[m][c][lane] = contract(Ia, lo, tb[c], GP * hi, 1);
+      TC y[GP];
+      line_apply<TC>(Is, ts[m], c * GS + GP * l, 1, y);
+#pragma unroll
+      for (int r = 0; r < GP; ++r)
+        ws[c * GS + GP * l + r] = y[r];
+      line_apply<TC>(Ia, tb, c * GS + GP * l, 1, y);
+#pragma unroll
+      for (int r = 0; r < GP; ++r)
+        wb[c * GS + GP * l + r] = y[r];
     }
-  }
-  __syncthreads();
-  if (mine)
-  {
-    /* second axis and the Riemann solves: lane (g, h) = (lo, hi) */
-    TC us[6], ub[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
+    wave_sync();
+    /* second axis: column l */
+    if (task)
     {
-      us[c] = contract(Is, hi, t1[m][c], lo, GP);
-      ub[c] = contract(Ib, hi, t2[m][c], lo, GP);
+      line_step<TC>(Is, ws, c * GS + l, GP);
+      line_step<TC>(Ib, wb, c * GS + l, GP);
     }
-    TC dv[3], dT[3];
-    riemann_dispatch<TC>(A, sgs, us, us + 3, ub, ub + 3, Zps, Zss, Zpb, Zsb, izp, izs, dv,
-                         dT);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
+    wave_sync();
+
+    /* the Riemann solves: lane (g, h) = (lo, hi) of the mortar grid */
     {
-      fs[m][c][lane] = dv[c];
-      fs[m][3 + c][lane] = dT[c];
+      TC us[6], ub[6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f)
+      {
+        us[f] = ws[f * GS + lane];
+        ub[f] = wb[f * GS + lane];
+      }
+      TC dv[3], dT[3];
+      riemann_dispatch<TC>(A, sgs, us, us + 3, ub, ub + 3, Zps, Zss, Zpb, Zsb, izp, izs, dv,
+                           dT);
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+      {
+        ws[f * GS + lane] = dv[f];
+        ws[(3 + f) * GS + lane] = dT[f];
+      }
+      riemann_dispatch<TC>(A, -sgs, ub, ub + 3, us, us + 3, Zpb, Zsb, Zps, Zss, izp, izs, dv,
+                           dT);
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+      {
+        wb[f * GS + lane] = dv[f];
+        wb[(3 + f) * GS + lane] = dT[f];
+      }
     }
-    riemann_dispatch<TC>(A, -sgs, ub, ub + 3, us, us + 3, Zpb, Zsb, Zps, Zss, izp, izs, dv,
-                         dT);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
+    wave_sync();
+
+    /* mass projection back to each face, first axis: row l */
+    if (task)
     {
-      fb[m][c][lane] = dv[c];
-      fb[m][3 + c][lane] = dT[c];
+      line_step<TC>(Ws, ws, c * GS + GP * l, 1);
+      line_step<TC>(Wa, wb, c * GS + GP * l, 1);
     }
-  }
-  __syncthreads();
-  if (mine)
-  {
-    /* mass projection back to each face, first axis: lane (i, h) = (lo, hi) */
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
+    wave_sync();
+    /* second axis: column l; the s face's values go to the glue buffer,
+     * the big face's contribution stays in LDS for the sum below */
+    if (task)
     {
-      t1[m][c][lane] = contract(Ws, lo, fs[m][c], GP * hi, 1);
-      t2[m][c][lane] = contract(Wa, lo, fb[m][c], GP * hi, 1);
-    }
-  }
-  __syncthreads();
-  if (mine)
-  {
-    /* second axis: lane (i, j) = (lo, hi) of each face; the big face's
-     * contribution goes to fs[m], free now, for the sum below */
+      TC y[GP];
+      line_apply<TC>(Ws, ws, c * GS + l, GP, y);
+      if (l < ns1)
+      {
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
-    {
-      const TC x = contract(Ws, hi, t1[m][c], lo, GP);
-      if (lo < ns1 && hi < ns1)
-        buf[gp->s_off[m] + c * nfs + lo + ns1 * hi] = x;
-      fs[m][c][lane] = contract(Wb, hi, t2[m][c], lo, GP);
+        for (int r = 0; r < GP; ++r)
+          if (r < ns1)
+            buf[gp->s_off[m] + c * nfs + l + ns1 * r] = y[r];
+      }
+      line_step<TC>(Wb, wb, c * GS + l, GP);
     }
   }
   __syncthreads();
@@ -235,7 +284,7 @@ __global__ __launch_bounds__(64 * GLUE_WAVES) void mortar_kernel(GlueArgs a)
     {
       TC acc = 0;
       for (int k = 0; k < nm; ++k)
-        acc += fs[k][c][lane];
+        acc += tw[k][1][c * GS + lane];
       buf[b_off + c * nfb + lo + nb1 * hi] = acc;
     }
   }

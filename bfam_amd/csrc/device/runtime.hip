@@ -1858,6 +1858,28 @@ static std::vector<T> glue_ops_table()
   return t;
 }
 
+/* the mortar kernel's operators: glue_ops_table's blocks between orders
+ * 1..7, padded to 8 x 8 */
+template <class T>
+static std::vector<T> glue_ops8_table()
+{
+  const std::vector<T> t = glue_ops_table<T>();
+  std::vector<T> p(GLUE_OPS8_SIZE, T(0));
+  for (int Na = 1; Na <= 7; ++Na)
+    for (int Nb = 1; Nb <= 7; ++Nb)
+      for (int kind = 0; kind < 2; ++kind)
+        for (int k = 0; k < 5; ++k)
+        {
+          /* (Nb + 1) x (Na + 1), column-major */
+          const T *src = t.data() + glue_ops_index(kind, Na, Nb, k);
+          T *dst = p.data() + glue_ops8_index(kind, Na, Nb, k);
+          for (int c = 0; c <= Na; ++c)
+            for (int r = 0; r <= Nb; ++r)
+              dst[r + 8 * c] = src[r + (Nb + 1) * c];
+        }
+  return p;
+}
+
 extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *subs,
                                  int nsubs, const bfamx_mortar_t *m, int32_t nm,
                                  bfamx_glue_t **out)
@@ -1983,7 +2005,7 @@ extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *
   if (!rc)
   {
     gl->d_groups = (MortarGroup *)p;
-    rc = dev_alloc(dev, gl->allocs, &gl->bytes, (fp32 ? 4 : 8) * (size_t)GLUE_OPS_SIZE, &p);
+    rc = dev_alloc(dev, gl->allocs, &gl->bytes, (fp32 ? 4 : 8) * (size_t)GLUE_OPS8_SIZE, &p);
   }
   if (!rc)
   {
@@ -2003,12 +2025,12 @@ extern "C" int bfamx_glue_upload(bfamx_device_t *dev, bfamx_subdomain_t *const *
   {
     if (fp32)
     {
-      auto t = glue_ops_table<float>();
+      auto t = glue_ops8_table<float>();
       he = hipMemcpy(gl->d_ops, t.data(), 4 * t.size(), hipMemcpyHostToDevice);
     }
     else
     {
-      auto t = glue_ops_table<double>();
+      auto t = glue_ops8_table<double>();
       he = hipMemcpy(gl->d_ops, t.data(), 8 * t.size(), hipMemcpyHostToDevice);
     }
   }
