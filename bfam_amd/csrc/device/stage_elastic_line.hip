@@ -817,8 +817,24 @@ __device__ __forceinline__ int first_wave_task(const LaunchArgs &a, int *stride,
   return lo + s * LINE_WPB + w;
 }
 
+/* wavefronts per SIMD the register allocation of the line kernel aims at
+ * (amdgpu_waves_per_eu; 1: no limit).  FP32 at N = 4 and 6 and FP32 rates
+ * at N = 4 need 177-264 registers unconstrained, one or two wavefronts
+ * per SIMD; capped for one more wavefront they spill 28-60 bytes per lane
+ * and run faster (124 M DOF bricks, ms per stage: FP32 N=4 0.658 -> 0.573,
+ * N=6 0.950 -> 0.618; FP32 rates N=4 2.03 -> 1.69).  Elsewhere the cap
+ * measured slower (FP32 N=3 0.454 -> 0.604, N=5 0.566 -> 0.580, FP64 N=4
+ * 1.18 -> 1.93) or is moot (LDS already limits N=7 and FP64 N=3 to two). */
+template <int N, class P> struct LineWaves
+{
+  static constexpr int value = __is_same(P, P32) ? (N == 4 ? 3 : (N == 6 ? 2 : 1))
+                               : __is_same(P, P32R) ? (N == 4 ? 3 : 1)
+                                                    : 1;
+};
+
 template <int N, class P, int PART>
-__global__ __launch_bounds__(64 * LINE_WPB) void elastic_line_kernel(LaunchArgs a)
+__global__ __launch_bounds__(64 * LINE_WPB)
+__attribute__((amdgpu_waves_per_eu(LineWaves<N, P>::value))) void elastic_line_kernel(LaunchArgs a)
 {
   using CF = LineCfg<N, typename P::TC, typename P::TC>;
   __shared__ __align__(16) char lds[LINE_WPB * CF::bytes];
