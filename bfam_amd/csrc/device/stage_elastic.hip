@@ -592,13 +592,19 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
  * 2.88 ms per 48^3 stage).  FP64 at N = 5 (two elements per block): 4
  * keeps two blocks per CU at 13 spilled registers, 1.12 against 1.22 ms per
  * 40^3 stage; at N = 3, 4 the spills cost more than the second block.
- * Other cases are left to the compiler. */
+ * FP32 at N = 5: 6 (80 registers, 32 bytes spilled per lane) lets a third
+ * 432-thread block share the CU, which its LDS allows: 0.582 -> 0.542 ms per
+ * 42^3 stage on the resident grid (one block per task: 0.586 -> 0.616).
+ * The same for N = 4 (with the node-side lift, to fit the LDS) measured
+ * slower (0.594 -> 0.659), and 8 for N = 6 (five blocks per CU) too
+ * (0.432 -> 0.539).  Other cases are left to the compiler. */
 template <int N, int EPB, class P> struct NodeWaves
 {
   static constexpr int value =
       P::MF ? 4
             : (EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4)
-                        : ((N == 5 && sizeof(typename P::TC) == 8) ? 4 : 1));
+                        : ((N == 5 && sizeof(typename P::TC) == 8) ? 4
+                           : (N == 5 && __is_same(P, P32)) ? 6 : 1));
 };
 
 template <int N, int EPB, class P>

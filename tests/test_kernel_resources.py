@@ -3,10 +3,12 @@ the compiler's resource report for gfx950 (CPU only: hipcc cross-compiles).
 
 The node-parallel kernel at N = 6, 7 runs 512-thread blocks, and its speed
 rests on how many fit on a CU: three FP32 blocks need at most 80 VGPRs, two
-FP32_RHS or FP64 blocks at most 128.  None of these kernels may touch
-scratch memory: per-lane scratch is written through L2 to HBM by every
-thread (a 24-byte spill of the stage context once added 0.9 GB of writes to
-each 48^3 FP32 stage)."""
+FP32_RHS or FP64 blocks at most 128.  These kernels may not touch scratch
+memory: per-lane scratch is written through L2 to HBM by every thread (a
+24-byte spill of the stage context once added 0.9 GB of writes to each 48^3
+FP32 stage).  The one exception is FP32 at N = 5, capped at 80 VGPRs so
+that a third two-element block fits the CU: it spills at most 32 bytes per
+lane and measured faster that way."""
 import os
 import re
 import shutil
@@ -66,4 +68,8 @@ def test_node_kernel_fits_its_blocks_without_scratch(node_report, N, prec, max_v
 @pytest.mark.parametrize("N,epb", [(3, 8), (4, 4), (5, 2)])
 def test_node_kernel_fp32_no_scratch(node_report, N, epb):
     for prec in ("fff", "dff"):
-        assert node_kernel(node_report, N, epb, prec)["scratch"] == 0, (N, PREC[prec])
+        r = node_kernel(node_report, N, epb, prec)
+        if (N, prec) == (5, "fff"):
+            assert r["vgpr"] <= 80 and r["scratch"] <= 32, (N, PREC[prec], r)
+        else:
+            assert r["scratch"] == 0, (N, PREC[prec], r)
