@@ -586,25 +586,32 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
 }
 
 /* waves per SIMD the register allocation must leave room for.  For the
- * one-element blocks of N = 6, 7 (two waves per SIMD each): 6 keeps three
- * blocks per CU for the FP32 stage; FP32_RHS and FP64 keep two (forcing
- * FP32_RHS to three spilled registers and measured slower, 3.01 against
- * 2.88 ms per 48^3 stage).  FP64 at N = 5 (two elements per block): 4
- * keeps two blocks per CU at 13 spilled registers, 1.12 against 1.22 ms per
- * 40^3 stage; at N = 3, 4 the spills cost more than the second block.
- * FP32 at N = 5: 6 (80 registers, 32 bytes spilled per lane) lets a third
- * 432-thread block share the CU, which its LDS allows: 0.582 -> 0.542 ms per
- * 42^3 stage on the resident grid (one block per task: 0.586 -> 0.616).
- * The same for N = 4 (with the node-side lift, to fit the LDS) measured
- * slower (0.594 -> 0.659), and 8 for N = 6 (five blocks per CU) too
- * (0.432 -> 0.539).  Other cases are left to the compiler. */
+ * one-element blocks (N = 5, 6, 7): 6 keeps three 512-thread blocks per CU
+ * for the FP32 stage at N = 7 and six 216-thread ones at N = 5 (80
+ * registers; 16 bytes spilled per lane at N = 5); FP32_RHS and FP64 take
+ * 4 (forcing FP32_RHS to three N = 7 blocks spilled three registers and
+ * measured slower, 3.01 against 2.88 ms per 48^3 stage).  At N = 3, 4
+ * (four and eight elements per block) the compiler chooses: capping FP32
+ * at N = 4 (with the node-side lift, to fit a third block's LDS) measured
+ * slower (0.594 -> 0.659 ms per 50^3 stage), as did 8 for FP32 at N = 6
+ * (five blocks per CU, 0.432 -> 0.539) and, for the two-element N = 5
+ * blocks of earlier versions, FP64 caps beyond two blocks per CU. */
 template <int N, int EPB, class P> struct NodeWaves
 {
   static constexpr int value =
-      P::MF ? 4
-            : (EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4)
-                        : ((N == 5 && sizeof(typename P::TC) == 8) ? 4
-                           : (N == 5 && __is_same(P, P32)) ? 6 : 1));
+      P::MF ? 4 : (EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4) : 1);
+};
+
+/* elements per block of the fused node-parallel kernel.  A node task holds
+ * up to 512 / (N+1)^3 elements (elastic_epb); a block of fewer elements
+ * takes a task in passes.  N = 5 runs one-element blocks (216 threads; FP32:
+ * 80 registers and 22 KB of LDS, six blocks per CU against three of the
+ * two-element blocks), ms per 42^3 stage: FP32 0.541 -> 0.483, FP32 rates
+ * 0.852 -> 0.739, FP64 1.009 -> 0.795.  Halving the FP32 blocks at N = 3
+ * and 4 measured slower (0.482 -> 0.508, 0.594 -> 0.682). */
+template <int N> struct NodeEpb
+{
+  static constexpr int value = N == 3 ? 8 : (N == 4 ? 4 : 1);
 };
 
 template <int N, int EPB, class P>
@@ -613,14 +620,29 @@ __attribute__((amdgpu_waves_per_eu(NodeWaves<N, EPB, P>::value))) void
 elastic_stage_kernel(LaunchArgs a)
 {
   __shared__ __align__(16) char smem[NodeLds<N, EPB, P>::bytes];
+  /* elements of a node task */
+  constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
+  static_assert(TE % EPB == 0, "a block takes whole passes of a task");
   int stride, end;
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
     const auto &sd = *(cglob(a.subs) + tk.sub);
     const StageCtx cx = stage_ctx(sd, a);
-    elastic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
-    __syncthreads();
+    if constexpr (EPB == TE)
+    {
+      elastic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
+      __syncthreads();
+    }
+    else
+      for (int s0 = 0; s0 < tk.ne; s0 += EPB)
+      {
+        Task pt = tk;
+        pt.e0 = tk.e0 + s0;
+        pt.ne = tk.ne - s0 < EPB ? tk.ne - s0 : EPB;
+        elastic_task<N, EPB, P>(sd, pt, cx, a.mode, smem);
+        __syncthreads();
+      }
   }
 }
 
@@ -635,9 +657,9 @@ static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s,
     hipLaunchKernelGGL((elastic_stage_kernel<NN, EE, P>), dim3(grid),          \
                        dim3(EE * (NN + 1) * (NN + 1) * (NN + 1)), dyn, s, a);  \
     break;
-    BFAMX_CASE(3, 8)
-    BFAMX_CASE(4, 4)
-    BFAMX_CASE(5, 2)
+    BFAMX_CASE(3, NodeEpb<3>::value)
+    BFAMX_CASE(4, NodeEpb<4>::value)
+    BFAMX_CASE(5, NodeEpb<5>::value)
     BFAMX_CASE(6, 1)
     BFAMX_CASE(7, 1)
 #undef BFAMX_CASE
@@ -655,9 +677,9 @@ static hipError_t occupancy_node(int N, int *blocks, unsigned dyn)
   case NN:                                                                     \
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
         blocks, elastic_stage_kernel<NN, EE, P>, EE * (NN + 1) * (NN + 1) * (NN + 1), dyn);
-    BFAMX_CASE(3, 8)
-    BFAMX_CASE(4, 4)
-    BFAMX_CASE(5, 2)
+    BFAMX_CASE(3, NodeEpb<3>::value)
+    BFAMX_CASE(4, NodeEpb<4>::value)
+    BFAMX_CASE(5, NodeEpb<5>::value)
     BFAMX_CASE(6, 1)
     BFAMX_CASE(7, 1)
 #undef BFAMX_CASE
