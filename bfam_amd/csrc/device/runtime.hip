@@ -996,19 +996,19 @@ static int grid_for(const bfamx_device_t *d, int ntasks)
 
 /* the stage kernel of an order not (yet) autotuned (0 line, 1
  * node-parallel, 2 node-parallel on a resident grid): the one measured
- * fastest on bricks of about 124 M DOF (FP32: line at N = 3, resident grid
- * at N = 4 and 6, node-parallel at N = 5 and 7; FP64: line at N = 3, 4,
- * node-parallel at N = 5, resident grid at N = 6, 7; at N = 5 the two
- * node-parallel grids are within 5 % of each other), and the node-parallel
- * kernel for FP32_RHS and split-BF16x3 */
+ * fastest on bricks of about 124 M DOF (FP32: resident grid at N = 4 and
+ * 6, node-parallel at N = 3, 5 and 7; FP64: node-parallel at N = 3..5,
+ * resident grid at N = 6, 7; at N = 5 the two node-parallel grids are
+ * within 5 % of each other), and the node-parallel kernel for FP32_RHS and
+ * split-BF16x3 */
 static int default_kind(int prec, int N)
 {
   switch (prec)
   {
-  case PREC_FP32: return N == 3 ? 0 : ((N == 5 || N == 7) ? 1 : 2);
+  case PREC_FP32: return (N == 4 || N == 6) ? 2 : 1;
   case PREC_FP32_RHS: return 1;
   case PREC_FP32_BF16X3: return 1;
-  case PREC_FP64: return N <= 4 ? 0 : (N == 5 ? 1 : 2);
+  case PREC_FP64: return N <= 5 ? 1 : 2;
   default: return 0;
   }
 }

@@ -604,14 +604,22 @@ template <int N, int EPB, class P> struct NodeWaves
 
 /* elements per block of the fused node-parallel kernel.  A node task holds
  * up to 512 / (N+1)^3 elements (elastic_epb); a block of fewer elements
- * takes a task in passes.  N = 5 runs one-element blocks (216 threads; FP32:
- * 80 registers and 22 KB of LDS, six blocks per CU against three of the
- * two-element blocks), ms per 42^3 stage: FP32 0.541 -> 0.483, FP32 rates
- * 0.852 -> 0.739, FP64 1.009 -> 0.795.  Halving the FP32 blocks at N = 3
- * and 4 measured slower (0.482 -> 0.508, 0.594 -> 0.682). */
-template <int N> struct NodeEpb
+ * takes a task in passes.  Smaller blocks let more of them share a CU
+ * (the LDS of a block scales with its elements), ms per stage on bricks of
+ * about 124 M DOF, one-block-per-task grid:
+ *   N = 5, one element (216 threads): FP32 0.541 -> 0.483, FP32 rates
+ *   0.852 -> 0.739, FP64 1.009 -> 0.795;
+ *   N = 3: FP32 one element (one wavefront) 0.482 -> 0.444, FP64 two
+ *   elements 1.05 -> 0.806 (its line kernel: 1.017);
+ *   N = 4: FP64 one element 1.28 -> 0.948 (its line kernel: 1.188).
+ * Slower, and not used: FP32 at N = 4 with one or two elements (0.594 ->
+ * 0.607, 0.626), FP32 rates at N = 3, 4 with one or two (0.708 -> 0.759,
+ * 0.894 -> 1.007). */
+template <int N, class P> struct NodeEpb
 {
-  static constexpr int value = N == 3 ? 8 : (N == 4 ? 4 : 1);
+  static constexpr bool f32 = __is_same(P, P32), f64 = __is_same(P, P64);
+  static constexpr int value =
+      N == 3 ? (f32 ? 1 : (f64 ? 2 : 8)) : (N == 4 ? (f64 ? 1 : 4) : 1);
 };
 
 template <int N, int EPB, class P>
@@ -657,9 +665,9 @@ static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s,
     hipLaunchKernelGGL((elastic_stage_kernel<NN, EE, P>), dim3(grid),          \
                        dim3(EE * (NN + 1) * (NN + 1) * (NN + 1)), dyn, s, a);  \
     break;
-    BFAMX_CASE(3, NodeEpb<3>::value)
-    BFAMX_CASE(4, NodeEpb<4>::value)
-    BFAMX_CASE(5, NodeEpb<5>::value)
+    BFAMX_CASE(3, (NodeEpb<3, P>::value))
+    BFAMX_CASE(4, (NodeEpb<4, P>::value))
+    BFAMX_CASE(5, (NodeEpb<5, P>::value))
     BFAMX_CASE(6, 1)
     BFAMX_CASE(7, 1)
 #undef BFAMX_CASE
@@ -677,9 +685,9 @@ static hipError_t occupancy_node(int N, int *blocks, unsigned dyn)
   case NN:                                                                     \
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
         blocks, elastic_stage_kernel<NN, EE, P>, EE * (NN + 1) * (NN + 1) * (NN + 1), dyn);
-    BFAMX_CASE(3, NodeEpb<3>::value)
-    BFAMX_CASE(4, NodeEpb<4>::value)
-    BFAMX_CASE(5, NodeEpb<5>::value)
+    BFAMX_CASE(3, (NodeEpb<3, P>::value))
+    BFAMX_CASE(4, (NodeEpb<4, P>::value))
+    BFAMX_CASE(5, (NodeEpb<5, P>::value))
     BFAMX_CASE(6, 1)
     BFAMX_CASE(7, 1)
 #undef BFAMX_CASE

@@ -326,10 +326,10 @@ int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order);
 /* the elastic stage kernel for (precision, order) by hand: 0 line,
  * 1 node-parallel, 2 node-parallel on a resident grid (the blocks that fit
  * the CUs at once, each looping over tasks), -1 back to unknown (until
- * autotuned: FP32 line at N = 3, resident grid at N = 4 and 6,
- * node-parallel at N = 5 and 7; FP64 line at N = 3, 4, node-parallel at
- * N = 5, resident grid at N = 6, 7; FP32_RHS and split-BF16x3
- * node-parallel); get returns the kernel that runs */
+ * autotuned: FP32 resident grid at N = 4 and 6, node-parallel at N = 3, 5
+ * and 7; FP64 node-parallel at N = 3..5, resident grid at N = 6, 7;
+ * FP32_RHS and split-BF16x3 node-parallel); get returns the kernel that
+ * runs */
 int bfamx_set_kernel(bfamx_device_t *dev, int precision, int N, int kind);
 int bfamx_get_kernel(const bfamx_device_t *dev, int precision, int N, int *kind);
 /* resident-block cap of a stage kernel (kind 0 line, 1 node-parallel) for

@@ -586,9 +586,9 @@ def test_default_kernel_policy():
     back to that default."""
     d = bx.Device(0)
     try:
-        assert [d.get_kernel(bx.FP32, N) for N in range(3, 8)] == [0, 2, 1, 2, 1]
+        assert [d.get_kernel(bx.FP32, N) for N in range(3, 8)] == [1, 2, 1, 2, 1]
         assert all(d.get_kernel(bx.FP32_RHS, N) == 1 for N in range(3, 8))
-        assert [d.get_kernel(bx.FP64, N) for N in range(3, 8)] == [0, 0, 1, 2, 2]
+        assert [d.get_kernel(bx.FP64, N) for N in range(3, 8)] == [1, 1, 1, 2, 2]
         assert all(d.get_kernel(bx.FP32_BF16X3, N) == 1 for N in range(3, 8))
         d.set_kernel(bx.FP32, 7, 0)
         assert d.get_kernel(bx.FP32, 7) == 0
