@@ -586,20 +586,24 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
 }
 
 /* waves per SIMD the register allocation must leave room for.  For the
- * one-element blocks (N = 5, 6, 7): 6 keeps three 512-thread blocks per CU
- * for the FP32 stage at N = 7 and six 216-thread ones at N = 5 (80
- * registers; 16 bytes spilled per lane at N = 5); FP32_RHS and FP64 take
- * 4 (forcing FP32_RHS to three N = 7 blocks spilled three registers and
- * measured slower, 3.01 against 2.88 ms per 48^3 stage).  At N = 3, 4
- * (four and eight elements per block) the compiler chooses: capping FP32
- * at N = 4 (with the node-side lift, to fit a third block's LDS) measured
- * slower (0.594 -> 0.659 ms per 50^3 stage), as did 8 for FP32 at N = 6
- * (five blocks per CU, 0.432 -> 0.539) and, for the two-element N = 5
- * blocks of earlier versions, FP64 caps beyond two blocks per CU. */
+ * one-element blocks (FP32 at N = 3..7, FP64 at N = 4..7, FP32 rates at
+ * N = 5..7): 6 (80 registers) keeps three 512-thread blocks per CU for the
+ * FP32 stage at N = 7 and runs the FP32 one-wavefront blocks of N = 3
+ * (0.441 against 0.447 ms per 60^3 stage at 96 registers); 5 (96
+ * registers, no spill) suits FP32 at N = 4, 5 (N = 5: 0.459 against 0.479
+ * at 80 registers); FP32_RHS and FP64 take 4 (forcing FP32_RHS to three
+ * N = 7 blocks spilled three registers and measured slower, 3.01 against
+ * 2.88 ms per 48^3 stage).  Multi-element blocks are left to the
+ * compiler: capping the four-element FP32 blocks of N = 4 for a third block
+ * per CU measured slower (0.594 -> 0.659), as did 8 for FP32 at N = 6 (five
+ * blocks per CU, 0.432 -> 0.539). */
 template <int N, int EPB, class P> struct NodeWaves
 {
   static constexpr int value =
-      P::MF ? 4 : (EPB == 1 ? (sizeof(typename P::TS) == 4 ? 6 : 4) : 1);
+      P::MF ? 4
+            : (EPB == 1 ? (((N == 4 || N == 5) && __is_same(P, P32)) ? 5
+                           : (sizeof(typename P::TS) == 4 ? 6 : 4))
+                        : 1);
 };
 
 /* elements per block of the fused node-parallel kernel.  A node task holds
@@ -611,15 +615,16 @@ template <int N, int EPB, class P> struct NodeWaves
  *   0.852 -> 0.739, FP64 1.009 -> 0.795;
  *   N = 3: FP32 one element (one wavefront) 0.482 -> 0.444, FP64 two
  *   elements 1.05 -> 0.806 (its line kernel: 1.017);
- *   N = 4: FP64 one element 1.28 -> 0.948 (its line kernel: 1.188).
- * Slower, and not used: FP32 at N = 4 with one or two elements (0.594 ->
- * 0.607, 0.626), FP32 rates at N = 3, 4 with one or two (0.708 -> 0.759,
- * 0.894 -> 1.007). */
+ *   N = 4: FP64 one element 1.28 -> 0.948 (its line kernel: 1.188); FP32
+ *   one element at 96 registers 0.590 -> 0.572 (resident grid; at 80
+ *   registers 0.607, at 114 0.610; two elements 0.626).
+ * Slower, and not used: FP32 rates at N = 3, 4 with one or two elements
+ * (0.708 -> 0.759, 0.894 -> 1.007). */
 template <int N, class P> struct NodeEpb
 {
   static constexpr bool f32 = __is_same(P, P32), f64 = __is_same(P, P64);
   static constexpr int value =
-      N == 3 ? (f32 ? 1 : (f64 ? 2 : 8)) : (N == 4 ? (f64 ? 1 : 4) : 1);
+      N == 3 ? (f32 ? 1 : (f64 ? 2 : 8)) : (N == 4 ? ((f64 || f32) ? 1 : 4) : 1);
 };
 
 template <int N, int EPB, class P>
