@@ -382,19 +382,22 @@ class Device:
 
     def set_kernel(self, precision, N, kind):
         """elastic stage kernel for (precision, N): 0 line, 1 node, 2 node on a
-        resident grid, -1 unknown (the per-precision default until autotuned)"""
+        resident grid, 3 and 4 as 1 and 2 with the node kernel's other
+        elements-per-block (N = 3..5), -1 unknown (the per-precision default
+        until autotuned)"""
         _check(lib().bfamx_set_kernel(self.h, int(precision), int(N), int(kind)))
 
     def get_kernel(self, precision, N):
         """the elastic stage kernel that runs for (precision, N): 0 line, 1 node,
-        2 node on a resident grid"""
+        2 node on a resident grid, 3 and 4 the same with the other block size"""
         k = C.c_int()
         _check(lib().bfamx_get_kernel(self.h, int(precision), int(N), C.byref(k)))
         return k.value
 
     def set_occupancy(self, precision, N, kind, level):
-        """resident-block cap of stage kernel `kind` (0 line, 1 node) for
-        (precision, N): 0 none, 1 three quarters, 2 half"""
+        """resident-block cap of stage kernel `kind` (0 line, 1 node, 2 node with
+        its other block size) for (precision, N): 0 none, 1 three quarters,
+        2 half"""
         _check(lib().bfamx_set_occupancy(self.h, int(precision), int(N), int(kind), int(level)))
 
     def get_occupancy(self, precision, N, kind):

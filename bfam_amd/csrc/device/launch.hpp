@@ -51,9 +51,10 @@ constexpr int NPREC = 5;
 /* fused stage or RHS-only launch over a task table.  N < 0: mixed orders
  * (one kernel handles every N in 3..7).  grid: number of workgroups. */
 hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
-                                hipStream_t s, unsigned dyn = 0);
-/* resident blocks per CU of the node-parallel stage kernel (N = 3..7) */
-hipError_t node_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn = 0);
+                                hipStream_t s, unsigned dyn = 0, int alt = 0);
+/* resident blocks per CU of the node-parallel stage kernel (N = 3..7);
+ * alt: with its other block size (kernel kinds 3, 4) */
+hipError_t node_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn = 0, int alt = 0);
 hipError_t launch_acoustic_stage(int prec, int N, const LaunchArgs &a,
                                  int grid, hipStream_t s);
 

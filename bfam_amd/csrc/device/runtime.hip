@@ -47,19 +47,21 @@ struct bfamx_device
   int persistent;
   int line_bpc[NPREC][9]; /* cached blocks per CU, 0 unknown */
   /* elastic stage kernel per precision and order, chosen by
-   * bfamx_batch_autotune: -1 unknown (line), 0 line (one wavefront per
-   * element, lanes own lines), 1 node-parallel (512-thread blocks, threads
-   * own nodes), 2 node-parallel on a resident grid (as many blocks as fit
-   * the CUs at once, each looping over tasks) */
+   * bfamx_batch_autotune: -1 unknown (the default of the per-order
+   * measurements), 0 line (one wavefront per element, lanes own lines), 1
+   * node-parallel (threads own nodes), 2 node-parallel on a resident grid
+   * (as many blocks as fit the CUs at once, each looping over tasks), 3 and
+   * 4 as 1 and 2 with the kernel's other block size (N = 3..5) */
   signed char kchoice[NPREC][9];
-  /* resident-block cap per precision, order and kernel (0 line, 1
-   * node-parallel), chosen by bfamx_batch_autotune or bfamx_set_occupancy:
+  /* resident-block cap per precision, order and kernel slot (0 line, 1
+   * node-parallel, 2 node-parallel with its other block size), chosen by
+   * bfamx_batch_autotune or bfamx_set_occupancy:
    * 0 as many blocks as registers and LDS allow, 1 three quarters of that,
    * 2 half.  A cap is a request of dynamic LDS the kernel does not use
    * (occ_dyn, bytes per block, found once by the occupancy calculator;
    * ~0u unknown) */
-  signed char occ[NPREC][9][2];
-  unsigned occ_dyn[NPREC][9][2][3];
+  signed char occ[NPREC][9][3];
+  unsigned occ_dyn[NPREC][9][3][3];
   /* the per-order launches of a grouped batch run on side streams forked
    * from and joined back to `stream` (parallel branches when captured) */
   hipStream_t side[8];
@@ -1027,24 +1029,42 @@ static int node_kernel(const bfamx_device_t *d, int prec, int N)
 /* the node-parallel kernel of (prec, N) runs on a resident grid */
 static int node_resident(const bfamx_device_t *d, int prec, int N)
 {
-  return kernel_kind(d, prec, N) == 2;
+  const int k = kernel_kind(d, prec, N);
+  return k == 2 || k == 4;
+}
+/* the node-parallel kernel of (prec, N) runs its other block size */
+static int node_alt(const bfamx_device_t *d, int prec, int N)
+{
+  return kernel_kind(d, prec, N) >= 3;
+}
+/* occupancy-cap slot of a kernel kind: 0 line, 1 node-parallel, 2
+ * node-parallel with the other block size */
+static int kind_slot(int kind)
+{
+  return kind == 0 ? 0 : (kind <= 2 ? 1 : 2);
+}
+static int kernel_slot(const bfamx_device_t *d, int prec, int N)
+{
+  return kind_slot(kernel_kind(d, prec, N));
 }
 
 /* dynamic LDS bytes per block that cap the stage kernel of (prec, N,
- * node) at level lv (0: no cap, 1: 3/4 of the uncapped resident blocks per
- * CU, 2: 1/2): the smallest request at which the occupancy calculator
+ * slot: 0 line, 1 node-parallel, 2 node-parallel with its other block
+ * size) at level lv (0: no cap, 1: 3/4 of the uncapped resident blocks
+ * per CU, 2: 1/2): the smallest request at which the occupancy calculator
  * allows no more than that many blocks; 0 if the cap is not below the
  * uncapped count or cannot be computed */
-static unsigned occ_dyn(bfamx_device_t *d, int prec, int N, int node, int lv)
+static unsigned occ_dyn(bfamx_device_t *d, int prec, int N, int slot, int lv)
 {
   if (lv <= 0 || N < 3 || N > 7)
     return 0;
-  unsigned &c = d->occ_dyn[prec][N][node][lv];
+  unsigned &c = d->occ_dyn[prec][N][slot][lv];
   if (c != ~0u)
     return c;
   c = 0;
   auto blocks = [&](unsigned dyn, int *nb) {
-    return node ? node_blocks_per_cu(prec, N, nb, dyn) : line_blocks_per_cu(prec, N, nb, dyn);
+    return slot ? node_blocks_per_cu(prec, N, nb, dyn, slot == 2)
+                : line_blocks_per_cu(prec, N, nb, dyn);
   };
   int b0 = 0;
   if (blocks(0, &b0) != hipSuccess || b0 < 2)
@@ -1066,9 +1086,9 @@ static unsigned occ_dyn(bfamx_device_t *d, int prec, int N, int node, int lv)
     c = hi;
   return c;
 }
-static unsigned kernel_dyn(bfamx_device_t *d, int prec, int N, int node)
+static unsigned kernel_dyn(bfamx_device_t *d, int prec, int N, int slot)
 {
-  return N >= 3 && N <= 7 ? occ_dyn(d, prec, N, node, d->occ[prec][N][node]) : 0;
+  return N >= 3 && N <= 7 ? occ_dyn(d, prec, N, slot, d->occ[prec][N][slot]) : 0;
 }
 /* grid of a node-parallel launch over ntasks tasks: one block per task, or,
  * on a resident grid, the blocks that fit the CUs at once (each then takes
@@ -1080,8 +1100,10 @@ static int grid_node(bfamx_device_t *d, int prec, int N, int ntasks)
   if (!node_resident(d, prec, N))
     return g;
   int bpc = 0;
-  if (node_blocks_per_cu(prec, N, &bpc, kernel_dyn(d, prec, N, 1)) == hipSuccess && bpc > 0 &&
-      (long long)bpc * d->num_cu < g)
+  const int slot = kernel_slot(d, prec, N);
+  if (node_blocks_per_cu(prec, N, &bpc, kernel_dyn(d, prec, N, slot), slot == 2) ==
+          hipSuccess &&
+      bpc > 0 && (long long)bpc * d->num_cu < g)
     g = bpc * d->num_cu;
   return g;
 }
@@ -1099,8 +1121,9 @@ static int run_fused(bfamx_subdomain_t *s, const LaunchArgs &a)
     LaunchArgs b = a;
     b.tasks = s->d_tasks_node;
     b.ntasks = s->ntasks_node;
+    const int slot = kernel_slot(s->dev, s->prec, s->N);
     e = launch_elastic_stage(s->prec, s->N, b, grid_node(s->dev, s->prec, s->N, b.ntasks),
-                             s->dev->stream, kernel_dyn(s->dev, s->prec, s->N, 1));
+                             s->dev->stream, kernel_dyn(s->dev, s->prec, s->N, slot), slot == 2);
   }
   else
     e = launch_elastic_line(s->prec, s->N, a, grid_line(s->dev, s->prec, s->N, a.ntasks),
@@ -1603,8 +1626,9 @@ static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n,
   {
     g.tasks = b->d_tasks_node + b->grpn_off[n];
     g.ntasks = b->grpn_cnt[n];
+    const int slot = kernel_slot(b->dev, b->prec, n);
     return launch_elastic_stage(b->prec, n, g, grid_node(b->dev, b->prec, n, g.ntasks), st,
-                                kernel_dyn(b->dev, b->prec, n, 1));
+                                kernel_dyn(b->dev, b->prec, n, slot), slot == 2);
   }
   g.tasks = a.tasks + b->grp_off[n];
   g.ntasks = b->grp_cnt[n];
@@ -2252,8 +2276,9 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
     return e == hipSuccess ? BFAMX_OK
                            : bfamx_fail(BFAMX_EHIP, "autotune launch: %s", hipGetErrorString(e));
   };
-  /* (a) kernel and resident-block cap per order: the line kernel, the
-   * node-parallel kernel with one block per task and on a resident grid,
+  /* (a) kernel, block size and resident-block cap per order: the line
+   * kernel, the node-parallel kernel with one block per task and on a
+   * resident grid, each with both its block sizes at N = 3..5 (kinds 3, 4),
    * each uncapped and capped at 3/4 and 1/2 of its resident blocks per CU
    * (split-BF16x3 has the node-parallel kernel only); caps the occupancy
    * calculator cannot place are skipped */
@@ -2263,28 +2288,29 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
       continue;
     float best = 0.f;
     int bk = -1, bl = 0;
-    for (int k = b->prec == PREC_FP32_BF16X3 ? 1 : 0; k < 3 && !rc; ++k)
+    const int kmax = n >= 3 && n <= 5 ? 5 : 3;
+    for (int k = b->prec == PREC_FP32_BF16X3 ? 1 : 0; k < kmax && !rc; ++k)
       for (int lv = 0; lv < 3 && !rc; ++lv)
       {
-        const int node = k > 0;
-        if (lv > 0 && occ_dyn(d, b->prec, n, node, lv) == 0)
+        const int slot = kind_slot(k);
+        if (lv > 0 && occ_dyn(d, b->prec, n, slot, lv) == 0)
           continue;
         kc[n] = (signed char)k;
-        d->occ[b->prec][n][node] = (signed char)lv;
+        d->occ[b->prec][n][slot] = (signed char)lv;
         float ms = 0.f;
         rc = time_it([&]() { return launch_order(b, a, n); }, &ms);
         if (!rc && (bk < 0 || ms < best))
           best = ms, bk = k, bl = lv;
       }
+    for (int sl = 0; sl < 3; ++sl)
+      d->occ[b->prec][n][sl] = 0;
     if (rc)
     {
       kc[n] = -1;
-      d->occ[b->prec][n][0] = d->occ[b->prec][n][1] = 0;
       break;
     }
     kc[n] = (signed char)bk;
-    d->occ[b->prec][n][0] = d->occ[b->prec][n][1] = 0;
-    d->occ[b->prec][n][bk > 0] = (signed char)bl;
+    d->occ[b->prec][n][kind_slot(bk)] = (signed char)bl;
   }
   /* (b) one order-switching launch or one launch per order.  The
    * order-switching launch is the line kernel: it is a candidate only when
@@ -2322,7 +2348,7 @@ extern "C" int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force,
 
 extern "C" int bfamx_set_kernel(bfamx_device_t *d, int prec, int N, int kind)
 {
-  if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || kind < -1 || kind > 2)
+  if (!d || prec < 0 || prec >= NPREC || N < 0 || N > 8 || kind < -1 || kind > 4)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_set_kernel: bad arguments");
   if (prec == PREC_FP32_BF16X3 && kind == 0)
     return bfamx_fail(BFAMX_EUNSUPPORTED,
@@ -2333,7 +2359,7 @@ extern "C" int bfamx_set_kernel(bfamx_device_t *d, int prec, int N, int kind)
 
 extern "C" int bfamx_set_occupancy(bfamx_device_t *d, int prec, int N, int kind, int level)
 {
-  if (!d || prec < 0 || prec >= NPREC || N < 3 || N > 7 || kind < 0 || kind > 1 ||
+  if (!d || prec < 0 || prec >= NPREC || N < 3 || N > 7 || kind < 0 || kind > 2 ||
       level < 0 || level > 2)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_set_occupancy: bad arguments");
   d->occ[prec][N][kind] = (signed char)level;
@@ -2343,7 +2369,7 @@ extern "C" int bfamx_set_occupancy(bfamx_device_t *d, int prec, int N, int kind,
 extern "C" int bfamx_get_occupancy(const bfamx_device_t *d, int prec, int N, int kind,
                                    int *level)
 {
-  if (!d || prec < 0 || prec >= NPREC || N < 3 || N > 7 || kind < 0 || kind > 1 || !level)
+  if (!d || prec < 0 || prec >= NPREC || N < 3 || N > 7 || kind < 0 || kind > 2 || !level)
     return bfamx_fail(BFAMX_EINVAL, "bfamx_get_occupancy: bad arguments");
   *level = d->occ[prec][N][kind];
   return BFAMX_OK;

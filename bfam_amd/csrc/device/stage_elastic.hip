@@ -626,6 +626,15 @@ template <int N, class P> struct NodeEpb
   static constexpr int value =
       N == 3 ? (f32 ? 1 : (f64 ? 2 : 8)) : (N == 4 ? ((f64 || f32) ? 1 : 4) : 1);
 };
+/* the other block size the autotuner tries at N = 3..5 (kernel kinds 3
+ * and 4): the task-sized blocks where NodeEpb is smaller, one element
+ * where it is the task size, two elements at N = 5 */
+template <int N, class P> struct NodeEpbAlt
+{
+  static constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
+  static constexpr int value =
+      N == 5 ? 2 : (N > 5 ? 1 : (NodeEpb<N, P>::value < TE ? TE : 1));
+};
 
 template <int N, int EPB, class P>
 __global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1))
@@ -659,67 +668,82 @@ elastic_stage_kernel(LaunchArgs a)
   }
 }
 
+/* the launch of order N with EPB elements per block, or the occupancy
+ * calculator's resident blocks per CU for it */
+template <int N, int EPB, class P>
+static hipError_t launch_ne(const LaunchArgs &a, int grid, hipStream_t s, unsigned dyn)
+{
+  hipLaunchKernelGGL((elastic_stage_kernel<N, EPB, P>), dim3(grid),
+                     dim3(EPB * (N + 1) * (N + 1) * (N + 1)), dyn, s, a);
+  return hipGetLastError();
+}
+template <int N, int EPB, class P>
+static hipError_t occupancy_ne(int *blocks, unsigned dyn)
+{
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks, elastic_stage_kernel<N, EPB, P>, EPB * (N + 1) * (N + 1) * (N + 1), dyn);
+}
+
+/* alt: the block size of NodeEpbAlt instead of NodeEpb */
 template <class P>
-static hipError_t launch_n(int N, const LaunchArgs &a, int grid, hipStream_t s,
+static hipError_t launch_n(int N, int alt, const LaunchArgs &a, int grid, hipStream_t s,
                            unsigned dyn)
 {
   switch (N)
   {
-#define BFAMX_CASE(NN, EE)                                                     \
+#define BFAMX_CASE(NN)                                                         \
   case NN:                                                                     \
-    hipLaunchKernelGGL((elastic_stage_kernel<NN, EE, P>), dim3(grid),          \
-                       dim3(EE * (NN + 1) * (NN + 1) * (NN + 1)), dyn, s, a);  \
-    break;
-    BFAMX_CASE(3, (NodeEpb<3, P>::value))
-    BFAMX_CASE(4, (NodeEpb<4, P>::value))
-    BFAMX_CASE(5, (NodeEpb<5, P>::value))
-    BFAMX_CASE(6, 1)
-    BFAMX_CASE(7, 1)
+    return alt ? launch_ne<NN, NodeEpbAlt<NN, P>::value, P>(a, grid, s, dyn)    \
+               : launch_ne<NN, NodeEpb<NN, P>::value, P>(a, grid, s, dyn);
+    BFAMX_CASE(3)
+    BFAMX_CASE(4)
+    BFAMX_CASE(5)
+    BFAMX_CASE(6)
+    BFAMX_CASE(7)
 #undef BFAMX_CASE
   default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <class P>
-static hipError_t occupancy_node(int N, int *blocks, unsigned dyn)
+static hipError_t occupancy_node(int N, int alt, int *blocks, unsigned dyn)
 {
   switch (N)
   {
-#define BFAMX_CASE(NN, EE)                                                     \
+#define BFAMX_CASE(NN)                                                         \
   case NN:                                                                     \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(                       \
-        blocks, elastic_stage_kernel<NN, EE, P>, EE * (NN + 1) * (NN + 1) * (NN + 1), dyn);
-    BFAMX_CASE(3, (NodeEpb<3, P>::value))
-    BFAMX_CASE(4, (NodeEpb<4, P>::value))
-    BFAMX_CASE(5, (NodeEpb<5, P>::value))
-    BFAMX_CASE(6, 1)
-    BFAMX_CASE(7, 1)
+    return alt ? occupancy_ne<NN, NodeEpbAlt<NN, P>::value, P>(blocks, dyn)     \
+               : occupancy_ne<NN, NodeEpb<NN, P>::value, P>(blocks, dyn);
+    BFAMX_CASE(3)
+    BFAMX_CASE(4)
+    BFAMX_CASE(5)
+    BFAMX_CASE(6)
+    BFAMX_CASE(7)
 #undef BFAMX_CASE
   default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t node_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn)
+hipError_t node_blocks_per_cu(int prec, int N, int *blocks, unsigned dyn, int alt)
 {
   switch (prec)
   {
-  case PREC_FP32: return occupancy_node<P32>(N, blocks, dyn);
-  case PREC_FP32_RHS: return occupancy_node<P32R>(N, blocks, dyn);
-  case PREC_FP32_BF16X3: return occupancy_node<P32M>(N, blocks, dyn);
-  default: return occupancy_node<P64>(N, blocks, dyn);
+  case PREC_FP32: return occupancy_node<P32>(N, alt, blocks, dyn);
+  case PREC_FP32_RHS: return occupancy_node<P32R>(N, alt, blocks, dyn);
+  case PREC_FP32_BF16X3: return occupancy_node<P32M>(N, alt, blocks, dyn);
+  default: return occupancy_node<P64>(N, alt, blocks, dyn);
   }
 }
 
 hipError_t launch_elastic_stage(int prec, int N, const LaunchArgs &a, int grid,
-                                hipStream_t s, unsigned dyn)
+                                hipStream_t s, unsigned dyn, int alt)
 {
   switch (prec)
   {
-  case PREC_FP32: return launch_n<P32>(N, a, grid, s, dyn);
-  case PREC_FP32_RHS: return launch_n<P32R>(N, a, grid, s, dyn);
-  case PREC_FP32_BF16X3: return launch_n<P32M>(N, a, grid, s, dyn);
-  default: return launch_n<P64>(N, a, grid, s, dyn);
+  case PREC_FP32: return launch_n<P32>(N, alt, a, grid, s, dyn);
+  case PREC_FP32_RHS: return launch_n<P32R>(N, alt, a, grid, s, dyn);
+  case PREC_FP32_BF16X3: return launch_n<P32M>(N, alt, a, grid, s, dyn);
+  default: return launch_n<P64>(N, alt, a, grid, s, dyn);
   }
 }
 

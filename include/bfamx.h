@@ -309,11 +309,13 @@ int bfamx_batch_capture(bfamx_batch_t *b, int nsteps);
 int bfamx_batch_launch_graph(bfamx_batch_t *b);
 /*
  * Autotuning on the batch's own data, for elastic batches.  For every order
- * present it times `trials` stages of two stage kernels, the line kernel
+ * present it times `trials` stages of each stage kernel: the line kernel
  * (one wavefront per element, lanes own node lines) and the node-parallel
- * kernel (512-thread blocks, threads own nodes), each with as many resident
- * blocks per CU as registers and LDS allow and capped at 3/4 and 1/2 of
- * that, and keeps the fastest kernel and cap.  With
+ * kernel (threads own nodes) with one block per task or on a resident
+ * grid, at N = 3..5 with both its elements-per-block choices (the kinds of
+ * bfamx_set_kernel), each with as many resident blocks per CU as registers
+ * and LDS allow and capped at 3/4 and 1/2 of that, and keeps the fastest
+ * kernel, block size and cap.  With
  * mixed orders it then chooses between one launch of the order-switching
  * kernel and one launch per order.  The state and results are left
  * unchanged.  Choices are cached on the device, per precision and order and
@@ -325,14 +327,18 @@ int bfamx_batch_autotune(bfamx_batch_t *b, int trials, int force, int *choice);
 int bfamx_batch_set_strategy(bfamx_batch_t *b, int per_order);
 /* the elastic stage kernel for (precision, order) by hand: 0 line,
  * 1 node-parallel, 2 node-parallel on a resident grid (the blocks that fit
- * the CUs at once, each looping over tasks), -1 back to unknown (until
+ * the CUs at once, each looping over tasks), 3 and 4 as 1 and 2 with the
+ * node-parallel kernel's other elements-per-block at N = 3..5 (elsewhere
+ * the same as 1 and 2), -1 back to unknown (until
  * autotuned: FP32 resident grid at N = 4 and 6, node-parallel at N = 3, 5
  * and 7; FP64 node-parallel at N = 3..5, resident grid at N = 6, 7;
  * FP32_RHS and split-BF16x3 node-parallel); get returns the kernel that
  * runs */
 int bfamx_set_kernel(bfamx_device_t *dev, int precision, int N, int kind);
 int bfamx_get_kernel(const bfamx_device_t *dev, int precision, int N, int *kind);
-/* resident-block cap of a stage kernel (kind 0 line, 1 node-parallel) for
+/* resident-block cap of a stage kernel (kind 0 line, 1 node-parallel, 2
+ * node-parallel with its other elements-per-block, kinds 3 and 4 of
+ * bfamx_set_kernel) for
  * (precision, order N = 3..7) by hand: level 0 none, 1 three quarters, 2
  * half of the blocks per CU that registers and LDS allow (a request of
  * dynamic LDS the kernel does not use).  No counterpart in the reference:

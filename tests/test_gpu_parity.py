@@ -22,9 +22,12 @@ def dev():
     d.close()
 
 
-@pytest.fixture(params=[0, 1, 2], ids=["line", "node", "node_resident"])
+@pytest.fixture(params=[0, 1, 2, 3, 4],
+                ids=["line", "node", "node_resident", "node_alt", "node_alt_resident"])
 def kernel(request, dev):
-    """run with the line (0) or the node-parallel (1) elastic stage kernel"""
+    """run with the line (0) or the node-parallel elastic stage kernel: one
+    block per task (1) or on a resident grid (2), and the same with its
+    other elements-per-block at N = 3..5 (3, 4)"""
     for prec in (bx.FP64, bx.FP32_RHS, bx.FP32):
         for N in range(3, 8):
             dev.set_kernel(prec, N, request.param)
@@ -184,7 +187,8 @@ def test_occupancy_caps_are_bit_identical(dev, kernel, N):
     print(f"occupancy caps N={N} kernel={kernel}", flush=True)
     cfg, q0 = random_elastic((4, 3, 3), N, seed=50 + N)
     res = []
-    kind = min(kernel, 1)  # the resident grid shares the node kernel's cap
+    # the resident grid shares the cap of its kernel and block size
+    kind = {0: 0, 1: 1, 2: 1, 3: 2, 4: 2}[kernel]
     for lv in (0, 1, 2):
         dev.set_occupancy(bx.FP32, N, kind, lv)
         assert dev.get_occupancy(bx.FP32, N, kind) == lv
@@ -204,13 +208,15 @@ def test_occupancy_caps_are_bit_identical(dev, kernel, N):
 def test_resident_grid_is_bit_identical(dev, N, n, prec):
     """The node-parallel kernel on a resident grid (each block looping over
     tasks; these bricks have more tasks than blocks fit the CUs at once)
-    gives the bits of one block per task"""
+    gives the bits of one block per task, and so does either with the
+    other elements-per-block (a node's arithmetic does not depend on how
+    many elements share its block)"""
     print(f"resident grid N={N} n={n} prec={prec}", flush=True)
     cfg = W.elastic3d_ricker(n=n, N=N, nsteps=20)
     m = cfg["mesh"]
     q0 = np.random.default_rng(N).standard_normal((9, m.K * m.Np))
     res = []
-    for k in (1, 2):
+    for k in (1, 2, 3, 4):
         dev.set_kernel(prec, N, k)
         assert dev.get_kernel(prec, N) == k
         s = W.upload(dev, cfg, prec)
@@ -219,7 +225,8 @@ def test_resident_grid_is_bit_identical(dev, N, n, prec):
         res.append(s.download())
         s.close()
     dev.set_kernel(prec, N, -1)
-    assert np.array_equal(res[0], res[1])
+    for r in res[1:]:
+        assert np.array_equal(res[0], r)
 
 
 def test_xcd_remap_invariant(dev):
