@@ -54,15 +54,24 @@ enum
 constexpr int FACE_LIFT_N = 8;
 
 /* FC: values per face node, 6 (the fluctuations dv, dT) or FACE_LIFT_N
- * (their lift, face_lift) */
+ * (their lift, face_lift).  Where an element's nine fields are not a
+ * multiple of 16 bytes (odd Np: N = 4, 6), an element block starts at any
+ * 4-byte offset; its 16-byte LDS-DMA then starts at the 16-byte boundary
+ * below it and ends at the one above, so the regions of the staged fields
+ * and dq rows (QR, DR) hold up to 3 words before and after the QW values
+ * and start on 16-byte boundaries. */
 template <int N, int EPB, class T, bool DQ = false, int FC = FACE_LIFT_N> struct ElasticSmem
 {
   static constexpr int n1 = N + 1, Np = n1 * n1 * n1, Nfp = n1 * n1;
   static constexpr int QW = EPB * 9 * Np;   /* staged fields */
+  static constexpr bool SHIFT = (9 * Np) % 4 != 0;
+  static constexpr int QR =
+      SHIFT ? (QW * (int)sizeof(T) / 4 + 6) / 4 * 4 * 4 / (int)sizeof(T) : QW;
   static constexpr int DW = DQ ? QW : 0;    /* staged dq rows */
+  static constexpr int DR = DQ ? QR : 0;
   static constexpr int FW = EPB * 6 * FC * Nfp; /* face values */
   /* LDS: fields, dq, face values, D, sin/cos tables */
-  static constexpr size_t bytes = sizeof(T) * (QW + DW + FW + n1 * n1 + EPB * 6 * n1);
+  static constexpr size_t bytes = sizeof(T) * (QR + DR + FW + n1 * n1 + EPB * 6 * n1);
 };
 
 /* rate index of lift value 6 and 7 for a face of axis A */

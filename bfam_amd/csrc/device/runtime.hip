@@ -415,20 +415,22 @@ static size_t subdomain_bytes(int dim, int N, int32_t K, int ph, int prec, int d
   const size_t rnd = 255;
   size_t b = 0;
   auto add = [&](size_t x) { b += ((x ? x : 16) + rnd) & ~rnd; };
+  /* state and dq buffers carry 16 bytes past their values */
+  auto addq = [&](size_t x) { add(x + 16); };
   switch (prec)
   {
   /* FP64 and FP32 keep no dq unless the table needs one (rebuilds_dq) */
   case BFAMX_FP64:
-    add(8 * ndof), add(8 * ndof);
+    addq(8 * ndof), addq(8 * ndof);
     if (dq_stored)
-      add(8 * ndof);
+      addq(8 * ndof);
     break;
-  case BFAMX_FP64_PLAIN: add(8 * ndof), add(8 * ndof), add(8 * ndof); break;
-  case BFAMX_FP32_RHS: add(8 * ndof), add(8 * ndof), add(4 * ndof); break;
+  case BFAMX_FP64_PLAIN: addq(8 * ndof), add(8 * ndof), addq(8 * ndof); break;
+  case BFAMX_FP32_RHS: addq(8 * ndof), addq(8 * ndof), addq(4 * ndof); break;
   default:
-    add(4 * ndof), add(4 * ndof);
+    addq(4 * ndof), addq(4 * ndof);
     if (dq_stored)
-      add(4 * ndof);
+      addq(4 * ndof);
     break;
   }
   const size_t nfaces = 2 * dim;
@@ -649,18 +651,22 @@ extern "C" int bfamx_subdomain_upload(bfamx_device_t *dev,
       goto fail;                                                               \
     ptr = (decltype(ptr))p;                                                    \
   } while (0)
-  ALLOC(h.q[0], ssz * ndof);
+  /* the states and dq carry 16 bytes past their values: the node-parallel
+   * stage copies element blocks into LDS in 16-byte pieces from the
+   * 16-byte boundaries around each block, which at odd Np reach up to
+   * 12 bytes past the last element */
+  ALLOC(h.q[0], ssz * ndof + 16);
   if (prec == BFAMX_FP64_PLAIN)
   {
     h.q[1] = h.q[0];
     ALLOC(h.rhs, 8 * ndof);
   }
   else
-    ALLOC(h.q[1], ssz * ndof);
+    ALLOC(h.q[1], ssz * ndof + 16);
   /* no dq buffer where stages rebuild it from the states (h.dq stays
    * NULL: no kernel reads or writes it then) */
   if (!h.dq_rec)
-    ALLOC(h.dq, rsz * ndof);
+    ALLOC(h.dq, rsz * ndof + 16);
   {
     int32_t *ne;
     uint8_t *nc;

@@ -126,11 +126,20 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   constexpr int TPT = (NT + BS - 1) / BS;
   using SM = NodeSmem<N, EPB, P>;
   /* state staged exactly (FP32, FP64): LDS-DMA straight into the staged
-   * fields, 16 bytes per lane where every element block is 16-byte aligned */
+   * fields, 16 bytes per lane.  An element's words (QE state, DE dq) are
+   * not a multiple of 4 at odd Np: the copy then starts at the 16-byte
+   * boundary below the block, sh words before its first value, and the
+   * staged fields begin sh words into their region (SM::SHIFT).  The
+   * buffers are allocated 16 bytes longer than their values, so the
+   * rounded-up end of the last element stays inside them (FP32 N = 4:
+   * 1-element blocks 0.603 ms per 50^3 stage with 4-byte lanes) */
   constexpr bool DMA_Q = sizeof(TS) == sizeof(TC);
   constexpr bool DMA_DQ = SM::DW > 0;
-  constexpr int DMA_WQ = (9 * Np * (int)sizeof(TS) / 4) % 4 == 0 ? 16 : 4;
-  constexpr int DMA_WD = (9 * Np * (int)sizeof(TR) / 4) % 4 == 0 ? 16 : 4;
+  constexpr int QE = NF * Np * (int)sizeof(TS) / 4, DE = NF * Np;
+  static_assert(!DMA_Q || QE % 4 == 0 || SM::SHIFT, "state rows need the shifted copy");
+  static_assert(!DMA_DQ || DE % 4 == 0 || SM::SHIFT, "dq rows need the shifted copy");
+  const int shq = DMA_Q && QE % 4 != 0 ? ((tk.e0 & 3) * (QE & 3)) & 3 : 0;
+  const int shd = DMA_DQ && DE % 4 != 0 ? ((tk.e0 & 3) * (DE & 3)) & 3 : 0;
 
   const int tid = threadIdx.x;
   /* one element per block (N >= 6): the element index is uniform, and its
@@ -141,12 +150,13 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   const int e = tk.e0 + le;
   const int i = n % n1, j = (n / n1) % n1, k = n / (n1 * n1);
 
-  TC *sq = reinterpret_cast<TC *>(smem); /* [EPB][9][Np] */
-  TC *sdq = sq + SM::QW;                 /* [EPB][9][Np] when DMA_DQ */
+  TC *sq = reinterpret_cast<TC *>(smem + 4 * shq); /* [EPB][9][Np] */
+  /* [EPB][9][Np] when DMA_DQ (FP32 rows) */
+  TC *sdq = reinterpret_cast<TC *>(smem + sizeof(TC) * SM::QR + 4 * shd);
   /* face values: [EPB][6 faces][FC][Nfp] */
   constexpr bool FL = NodeFaceLift<N>::value;
   constexpr int FC = NodeFaceLift<N>::FC;
-  TC *sfl = sdq + SM::DW;
+  TC *sfl = reinterpret_cast<TC *>(smem) + SM::QR + SM::DR;
   TC *sD = sfl + SM::FW;
   TC *smms = sD + n1 * n1;
 
@@ -167,9 +177,8 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   TS qs[KEEP_Q ? NF : 1];
   if constexpr (DMA_Q)
   {
-    block_to_lds<DMA_WQ, BS>((gptr<const float>)(qin + (size_t)tk.e0 * NF * Np),
-                             reinterpret_cast<float *>(sq),
-                             tk.ne * NF * Np * (int)(sizeof(TS) / 4));
+    block_to_lds<16, BS>((gptr<const float>)(qin + (size_t)tk.e0 * NF * Np) - shq,
+                         reinterpret_cast<float *>(smem), (shq + tk.ne * QE + 3) & ~3);
   }
   else if (act)
   {
@@ -202,9 +211,9 @@ __device__ __forceinline__ void elastic_task(const SD &sd, const Task &tk,
   if constexpr (DMA_DQ)
   {
     if (rd_dq)
-      block_to_lds<DMA_WD, BS, CPOL_NT>(
-          (gptr<const float>)glob(static_cast<const float *>(dsrc)) + (size_t)tk.e0 * NF * Np,
-          reinterpret_cast<float *>(sdq), tk.ne * NF * Np);
+      block_to_lds<16, BS, CPOL_NT>(
+          (gptr<const float>)glob(static_cast<const float *>(dsrc)) + (size_t)tk.e0 * DE - shd,
+          reinterpret_cast<float *>(smem + sizeof(TC) * SM::QR), (shd + tk.ne * DE + 3) & ~3);
   }
   /* rates without room in LDS (FP64, and FP32 where a third block per CU
    * would not fit): this node's dq values load into registers instead, also
