@@ -293,37 +293,42 @@ __device__ __forceinline__ size_t qidx(int e, int f, int n, int NF, int Np)
  * (Morton-ordered) task list; mode 2 (the default) keeps one round-robin
  * front over the list but hands each XCD whole 2x2x2 Morton cubes.
  */
-__device__ __forceinline__ int first_task(const LaunchArgs &a, int *stride,
+__device__ __forceinline__ int first_task(const LaunchArgs &a, int ntasks, int *stride,
                                           int *end)
 {
   const int b = blockIdx.x, G = gridDim.x;
-  if (a.xcd_remap == 2 && a.nxcd == 8 && G >= a.ntasks)
+  if (a.xcd_remap == 2 && a.nxcd == 8 && G >= ntasks)
   {
     /* round-robin front, but each XCD's 8 blocks of a group of 64 take one
      * 2x2x2 Morton cube of tasks, whose inner faces then share an L2; a
      * partial last group keeps the identity */
     *stride = G;
-    *end = a.ntasks;
+    *end = ntasks;
     const int g0 = b & ~63;
-    if (g0 + 64 > a.ntasks)
+    if (g0 + 64 > ntasks)
       return b;
     return g0 + 8 * (b & 7) + ((b & 63) >> 3);
   }
   if (a.xcd_remap != 1 || G % a.nxcd)
   {
     *stride = G;
-    *end = a.ntasks;
+    *end = ntasks;
     return b;
   }
   const int x = b % a.nxcd, s = b / a.nxcd, per = G / a.nxcd;
-  const int C = (a.ntasks + a.nxcd - 1) / a.nxcd;
+  const int C = (ntasks + a.nxcd - 1) / a.nxcd;
   const int lo = x * C;
   int hi = lo + C;
-  if (hi > a.ntasks)
-    hi = a.ntasks;
+  if (hi > ntasks)
+    hi = ntasks;
   *stride = per;
   *end = hi;
   return lo + s;
+}
+
+__device__ __forceinline__ int first_task(const LaunchArgs &a, int *stride, int *end)
+{
+  return first_task(a, a.ntasks, stride, end);
 }
 
 /* Ricker wavelet (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2 */

@@ -645,7 +645,24 @@ template <int N, class P> struct NodeEpbAlt
       N == 5 ? 2 : (N > 5 ? 1 : (NodeEpb<N, P>::value < TE ? TE : 1));
 };
 
-template <int N, int EPB, class P>
+/* blocks per node task on a one-block-per-task grid: the task's elements
+ * (one per block) where one-element blocks measured faster taking one
+ * element each than a whole task in passes, 1 elsewhere.  Each XCD's
+ * blocks then cover 2x2x2 elements of a Morton group instead of 4x4x4
+ * (N = 3), and fewer elements are in flight per XCD.  ms per stage on
+ * bricks of about 124 M DOF: FP32 N = 3 0.443 -> 0.412-0.414, N = 4
+ * 0.548-0.552 -> 0.523-0.530.  FP32 at N = 5 measured slower with
+ * one-element tasks (0.455 -> 0.464), as did the resident grids; the FP64
+ * N = 4 form spilled 12 bytes per lane and took 0.866 -> 0.946 ms. */
+template <int N, int EPB, class P> struct NodeSplit
+{
+  static constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
+  static constexpr int value = EPB == 1 && __is_same(P, P32) && (N == 3 || N == 4) ? TE : 1;
+};
+
+/* SPLIT: the one-block-per-element launch of a NodeSplit order, block v
+ * taking element v % NS of task v / NS */
+template <int N, int EPB, class P, bool SPLIT = false>
 __global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1))
 __attribute__((amdgpu_waves_per_eu(NodeWaves<N, EPB, P>::value))) void
 elastic_stage_kernel(LaunchArgs a)
@@ -655,6 +672,26 @@ elastic_stage_kernel(LaunchArgs a)
   constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
   static_assert(TE % EPB == 0, "a block takes whole passes of a task");
   int stride, end;
+  if constexpr (SPLIT)
+  {
+    constexpr int NS = NodeSplit<N, EPB, P>::value;
+    static_assert(EPB == 1 && NS > 1, "split launches take one element per block");
+    for (int vi = first_task(a, a.ntasks * NS, &stride, &end); vi < end; vi += stride)
+    {
+      const int ti = vi / NS, s = vi - ti * NS;
+      Task tk = load_task(a.tasks, ti);
+      if (s < tk.ne)
+      {
+        const auto &sd = *(cglob(a.subs) + tk.sub);
+        const StageCtx cx = stage_ctx(sd, a);
+        tk.e0 += s;
+        tk.ne = 1;
+        elastic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
+        __syncthreads();
+      }
+    }
+    return;
+  }
   for (int ti = first_task(a, &stride, &end); ti < end; ti += stride)
   {
     const Task tk = load_task(a.tasks, ti);
@@ -682,6 +719,17 @@ elastic_stage_kernel(LaunchArgs a)
 template <int N, int EPB, class P>
 static hipError_t launch_ne(const LaunchArgs &a, int grid, hipStream_t s, unsigned dyn)
 {
+  constexpr int NS = NodeSplit<N, EPB, P>::value;
+  /* one block per task (not a resident grid): one block per element */
+  if constexpr (NS > 1)
+    if (grid >= a.ntasks)
+    {
+      const int vt = a.ntasks * NS;
+      grid = a.xcd_remap == 1 && a.nxcd > 1 ? a.nxcd * ((vt + a.nxcd - 1) / a.nxcd) : vt;
+      hipLaunchKernelGGL((elastic_stage_kernel<N, EPB, P, true>), dim3(grid),
+                         dim3(EPB * (N + 1) * (N + 1) * (N + 1)), dyn, s, a);
+      return hipGetLastError();
+    }
   hipLaunchKernelGGL((elastic_stage_kernel<N, EPB, P>), dim3(grid),
                      dim3(EPB * (N + 1) * (N + 1) * (N + 1)), dyn, s, a);
   return hipGetLastError();

@@ -203,14 +203,14 @@ def test_occupancy_caps_are_bit_identical(dev, kernel, N):
         dev.set_occupancy(bx.FP32, N, kind, 3)
 
 
-@pytest.mark.parametrize("N,n", [(3, 18), (5, 12), (7, 12)])
+@pytest.mark.parametrize("N,n", [(3, 18), (4, 14), (5, 12), (7, 12)])
 @pytest.mark.parametrize("prec", [bx.FP32, bx.FP64])
 def test_resident_grid_is_bit_identical(dev, N, n, prec):
     """The node-parallel kernel on a resident grid (each block looping over
     tasks; these bricks have more tasks than blocks fit the CUs at once)
     gives the bits of one block per task, and so does either with the
-    other elements-per-block (a node's arithmetic does not depend on how
-    many elements share its block)"""
+    other elements-per-block, except FP64 at N = 4, whose two block sizes
+    agree to the last bits"""
     print(f"resident grid N={N} n={n} prec={prec}", flush=True)
     cfg = W.elastic3d_ricker(n=n, N=N, nsteps=20)
     m = cfg["mesh"]
@@ -225,6 +225,38 @@ def test_resident_grid_is_bit_identical(dev, N, n, prec):
         res.append(s.download())
         s.close()
     dev.set_kernel(prec, N, -1)
+    if prec == bx.FP64 and N == 4:
+        # the four-element FP64 blocks of N = 4 (kinds 3, 4) differ from the
+        # one-element ones in the last bits only
+        assert np.array_equal(res[0], res[1])
+        d = [rel_max(r, res[0]) for r in res[2:]]
+        print(f"FP64 N=4 four-element blocks: rel max {d}", flush=True)
+        assert max(d) < 1e-13
+        return
+    for r in res[1:]:
+        assert np.array_equal(res[0], r)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_element_blocks_bit_identical_in_every_block_order(dev, N):
+    """FP32 at N = 3, 4 launches one block per element of each node task
+    (block v takes element v % T of task v / T) where a block per task takes
+    the task's elements in turn: every block order, and the resident grid
+    that takes whole tasks, give the same bits.  The 5x5x3 brick's last
+    task is partial, so some blocks find no element."""
+    print(f"element blocks N={N}", flush=True)
+    cfg, q0 = random_elastic((5, 5, 3), N, seed=60 + N)
+    res = []
+    for remap, kind in ((2, 1), (0, 1), (1, 1), (2, 2)):
+        dev.set_tuning(remap)
+        dev.set_kernel(bx.FP32, N, kind)
+        s = W.upload(dev, cfg, bx.FP32)
+        s.set_fields(q0, 0.0, cfg["dt"])
+        s.step(2)
+        res.append(s.download())
+        s.close()
+    dev.set_tuning(2)
+    dev.set_kernel(bx.FP32, N, -1)
     for r in res[1:]:
         assert np.array_equal(res[0], r)
 

@@ -50,8 +50,9 @@ def node_report(tmp_path_factory):
     return rep
 
 
-def node_kernel(rep, N, epb, prec):
-    key = f"elastic_stage_kernelILi{N}ELi{epb}E{MANGLED[prec]}"
+def node_kernel(rep, N, epb, prec, split=False):
+    """the stage kernel instance; split: its one-block-per-element form"""
+    key = f"elastic_stage_kernelILi{N}ELi{epb}E{MANGLED[prec]}Lb{int(split)}E"
     hits = [v for k, v in rep.items() if key in k]
     assert len(hits) == 1, key
     return hits[0]
@@ -66,10 +67,12 @@ def test_node_kernel_fits_its_blocks_without_scratch(node_report, N, prec, max_v
 
 
 # (N, precision) -> elements per block of the fused kernel, and the
-# registers and spilled bytes per lane it may use: the one-element FP32
-# blocks are capped at 80 (N = 3, which spills a few bytes per lane and
-# measured faster) or 96 registers (N = 4, 5)
-NODE = {(3, "fff"): (1, 80, 16), (4, "fff"): (1, 96, 0), (5, "fff"): (1, 96, 0),
+# registers and spilled bytes per lane its whole-task form may use: the
+# one-element FP32 blocks are capped at 80 (N = 3) or 96 registers (N = 4,
+# 5); at N = 3 and 4 the whole-task form runs only on the resident grid and
+# spills a few bytes per lane there (measured no slower), while the
+# one-block-per-element form that the other grid runs spills none
+NODE = {(3, "fff"): (1, 80, 16), (4, "fff"): (1, 96, 12), (5, "fff"): (1, 96, 0),
         (3, "dff"): (8, 256, 0), (4, "dff"): (4, 256, 0), (5, "dff"): (1, 128, 0)}
 
 
@@ -78,3 +81,13 @@ def test_node_kernel_small_orders(node_report, N, prec):
     epb, max_vgpr, max_scratch = NODE[(N, prec)]
     r = node_kernel(node_report, N, epb, prec)
     assert r["vgpr"] <= max_vgpr and r["scratch"] <= max_scratch, (N, PREC[prec], r)
+
+
+@pytest.mark.parametrize("N,prec", [(3, "fff"), (4, "fff")])
+def test_node_kernel_split_form(node_report, N, prec):
+    """The one-block-per-element launch of a task-splitting order is a
+    separate instance; it needs no more registers than the form that takes
+    whole tasks, and no scratch."""
+    base = node_kernel(node_report, N, 1, prec)
+    r = node_kernel(node_report, N, 1, prec, split=True)
+    assert r["vgpr"] <= max(base["vgpr"], 80) and r["scratch"] == 0, (N, PREC[prec], r, base)
