@@ -274,6 +274,9 @@ struct LaunchArgs
   int mode;
   int xcd_remap;
   int nxcd;
+  /* node-parallel stage: one block per task even where the one-element
+   * kernel takes one element per block (an order launched beside others) */
+  int whole_tasks;
   double t_rhs;  /* time for MODE_RHS */
   double g_rhs;  /* ricker(t_rhs) */
   double sw_rhs; /* sin(omega t_rhs) */

@@ -1619,6 +1619,19 @@ static GlueArgs glue_args(const bfamx_batch_t *b)
   return g;
 }
 
+/* the order of a batch with the most volume nodes (the one launched on the
+ * batch's stream when orders launch side by side) */
+static int batch_big_order(const bfamx_batch_t *b)
+{
+  int big = -1;
+  for (int n = 0; n < 9; ++n)
+    if (b->grp_cnt[n] &&
+        (big < 0 || (size_t)b->grp_cnt[n] * (n + 1) * (n + 1) * (n + 1) >
+                        (size_t)b->grp_cnt[big] * (big + 1) * (big + 1) * (big + 1)))
+      big = n;
+  return big;
+}
+
 /* the stage kernels of a batch: one launch (one order, or all orders with
  * the mixed kernel), or one launch per order over its slice of the tasks */
 /* the stage kernels of the orders [n] of a batch with their chosen kernel:
@@ -1632,6 +1645,9 @@ static hipError_t launch_order(bfamx_batch_t *b, const LaunchArgs &a, int n,
   {
     g.tasks = b->d_tasks_node + b->grpn_off[n];
     g.ntasks = b->grpn_cnt[n];
+    /* an order that runs beside the batch's largest keeps whole tasks,
+     * in the autotuner's trials as in the batch stage */
+    g.whole_tasks = b->N < 0 && n != batch_big_order(b);
     const int slot = kernel_slot(b->dev, b->prec, n);
     return launch_elastic_stage(b->prec, n, g, grid_node(b->dev, b->prec, n, g.ntasks), st,
                                 kernel_dyn(b->dev, b->prec, n, slot), slot == 2);
@@ -1662,15 +1678,11 @@ static hipError_t launch_batch_kernels(bfamx_batch_t *b, const LaunchArgs &a,
   /* one launch per order: the largest on the batch's stream, the others on
    * side streams forked after and joined before the next launch, so small
    * orders do not wait for the tail of a large one */
-  int ord[9], no = 0, big = -1;
+  int ord[9], no = 0;
+  const int big = batch_big_order(b);
   for (int n = 0; n < 9; ++n)
     if (b->grp_cnt[n])
-    {
       ord[no++] = n;
-      if (big < 0 || (size_t)b->grp_cnt[n] * (n + 1) * (n + 1) * (n + 1) >
-                         (size_t)b->grp_cnt[big] * (big + 1) * (big + 1) * (big + 1))
-        big = n;
-    }
   bfamx_device_t *d = b->dev;
   hipError_t e = hipSuccess;
   if (no > 1)

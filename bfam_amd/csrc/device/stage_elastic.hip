@@ -653,7 +653,11 @@ template <int N, class P> struct NodeEpbAlt
  * bricks of about 124 M DOF: FP32 N = 3 0.443 -> 0.412-0.414, N = 4
  * 0.548-0.552 -> 0.523-0.530.  FP32 at N = 5 measured slower with
  * one-element tasks (0.455 -> 0.464), as did the resident grids; the FP64
- * N = 4 form spilled 12 bytes per lane and took 0.866 -> 0.946 ms. */
+ * N = 4 form spilled 12 bytes per lane and took 0.866 -> 0.946 ms.  An
+ * order launched beside a larger one of its batch keeps whole tasks
+ * (LaunchArgs::whole_tasks), as the autotuner then times it too: split
+ * there, its many small blocks crowded the larger order's (adapted mesh,
+ * orders 7/5/3: 0.555-0.567 ms per step, 0.550-0.553 with whole tasks). */
 template <int N, int EPB, class P> struct NodeSplit
 {
   static constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
@@ -722,7 +726,7 @@ static hipError_t launch_ne(const LaunchArgs &a, int grid, hipStream_t s, unsign
   constexpr int NS = NodeSplit<N, EPB, P>::value;
   /* one block per task (not a resident grid): one block per element */
   if constexpr (NS > 1)
-    if (grid >= a.ntasks)
+    if (grid >= a.ntasks && !a.whole_tasks)
     {
       const int vt = a.ntasks * NS;
       grid = a.xcd_remap == 1 && a.nxcd > 1 ? a.nxcd * ((vt + a.nxcd - 1) / a.nxcd) : vt;
