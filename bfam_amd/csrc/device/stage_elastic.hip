@@ -646,8 +646,9 @@ template <int N, class P> struct NodeEpbAlt
 };
 
 /* blocks per node task on a one-block-per-task grid: the task's elements
- * (one per block) where one-element blocks measured faster taking one
- * element each than a whole task in passes, 1 elsewhere.  Each XCD's
+ * over EPB (each block taking EPB of them) where that measured faster than
+ * one block taking the whole task in passes, 1 elsewhere.  FP64 at N = 3
+ * (two-element blocks): 0.805 -> 0.789 ms per stage.  Each XCD's
  * blocks then cover 2x2x2 elements of a Morton group instead of 4x4x4
  * (N = 3), and fewer elements are in flight per XCD.  ms per stage on
  * bricks of about 124 M DOF: FP32 N = 3 0.443 -> 0.412-0.414, N = 4
@@ -661,11 +662,14 @@ template <int N, class P> struct NodeEpbAlt
 template <int N, int EPB, class P> struct NodeSplit
 {
   static constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
-  static constexpr int value = EPB == 1 && __is_same(P, P32) && (N == 3 || N == 4) ? TE : 1;
+  static constexpr int value =
+      (EPB == 1 && __is_same(P, P32) && (N == 3 || N == 4)) || (EPB == 2 && __is_same(P, P64) && N == 3)
+          ? TE / EPB
+          : 1;
 };
 
-/* SPLIT: the one-block-per-element launch of a NodeSplit order, block v
- * taking element v % NS of task v / NS */
+/* SPLIT: the launch of a NodeSplit order with NS blocks per task, block v
+ * taking elements EPB (v % NS) .. EPB (v % NS) + EPB - 1 of task v / NS */
 template <int N, int EPB, class P, bool SPLIT = false>
 __global__ __launch_bounds__(EPB *(N + 1) * (N + 1) * (N + 1))
 __attribute__((amdgpu_waves_per_eu(NodeWaves<N, EPB, P>::value))) void
@@ -679,17 +683,17 @@ elastic_stage_kernel(LaunchArgs a)
   if constexpr (SPLIT)
   {
     constexpr int NS = NodeSplit<N, EPB, P>::value;
-    static_assert(EPB == 1 && NS > 1, "split launches take one element per block");
+    static_assert(NS > 1 && NS * EPB == TE, "split launches take EPB elements per block");
     for (int vi = first_task(a, a.ntasks * NS, &stride, &end); vi < end; vi += stride)
     {
-      const int ti = vi / NS, s = vi - ti * NS;
+      const int ti = vi / NS, s0 = (vi - ti * NS) * EPB;
       Task tk = load_task(a.tasks, ti);
-      if (s < tk.ne)
+      if (s0 < tk.ne)
       {
         const auto &sd = *(cglob(a.subs) + tk.sub);
         const StageCtx cx = stage_ctx(sd, a);
-        tk.e0 += s;
-        tk.ne = 1;
+        tk.e0 += s0;
+        tk.ne = tk.ne - s0 < EPB ? tk.ne - s0 : EPB;
         elastic_task<N, EPB, P>(sd, tk, cx, a.mode, smem);
         __syncthreads();
       }

@@ -83,11 +83,11 @@ def test_node_kernel_small_orders(node_report, N, prec):
     assert r["vgpr"] <= max_vgpr and r["scratch"] <= max_scratch, (N, PREC[prec], r)
 
 
-@pytest.mark.parametrize("N,prec", [(3, "fff"), (4, "fff")])
-def test_node_kernel_split_form(node_report, N, prec):
-    """The one-block-per-element launch of a task-splitting order is a
-    separate instance; it needs no more registers than the form that takes
-    whole tasks, and no scratch."""
-    base = node_kernel(node_report, N, 1, prec)
-    r = node_kernel(node_report, N, 1, prec, split=True)
+@pytest.mark.parametrize("N,prec,epb", [(3, "fff", 1), (4, "fff", 1), (3, "ddd", 2)])
+def test_node_kernel_split_form(node_report, N, prec, epb):
+    """The launch of a task-splitting order with one block per EPB elements
+    of a task is a separate instance; it needs no more registers than the
+    form that takes whole tasks, and no scratch."""
+    base = node_kernel(node_report, N, epb, prec)
+    r = node_kernel(node_report, N, epb, prec, split=True)
     assert r["vgpr"] <= max(base["vgpr"], 80) and r["scratch"] == 0, (N, PREC[prec], r, base)
