@@ -627,11 +627,12 @@ template <int N, int EPB, class P> struct NodeWaves
  *   N = 4: FP64 one element 1.28 -> 0.948 (its line kernel: 1.188); FP32
  *   one element at 96 registers 0.590 -> 0.572 (resident grid; at 80
  *   registers 0.607, at 114 0.610; two elements 0.626).
- * Slower, and not used: FP32 rates at N = 3, 4 with one or two elements
- * (0.708 -> 0.759, 0.894 -> 1.007). */
+ * FP32 rates at N = 3, 4 with one element measured slower taking a task
+ * in passes (0.708 -> 0.759, 0.894 -> 1.007) and faster with one block per
+ * element (NodeSplit): 0.705 -> 0.642, 0.868 -> 0.813. */
 template <int N, class P> struct NodeEpb
 {
-  static constexpr bool f32 = __is_same(P, P32), f64 = __is_same(P, P64);
+  static constexpr bool f32 = __is_same(P, P32) || __is_same(P, P32R), f64 = __is_same(P, P64);
   static constexpr int value =
       N == 3 ? (f32 ? 1 : (f64 ? 2 : 8)) : (N == 4 ? ((f64 || f32) ? 1 : 4) : 1);
 };
@@ -663,7 +664,8 @@ template <int N, int EPB, class P> struct NodeSplit
 {
   static constexpr int TE = 512 / ((N + 1) * (N + 1) * (N + 1)) < 1 ? 1 : 512 / ((N + 1) * (N + 1) * (N + 1));
   static constexpr int value =
-      (EPB == 1 && __is_same(P, P32) && (N == 3 || N == 4)) || (EPB == 2 && __is_same(P, P64) && N == 3)
+      (EPB == 1 && (__is_same(P, P32) || __is_same(P, P32R)) && (N == 3 || N == 4)) ||
+              (EPB == 2 && __is_same(P, P64) && N == 3)
           ? TE / EPB
           : 1;
 };

@@ -73,7 +73,7 @@ def test_node_kernel_fits_its_blocks_without_scratch(node_report, N, prec, max_v
 # spills a few bytes per lane there (measured no slower), while the
 # one-block-per-element form that the other grid runs spills none
 NODE = {(3, "fff"): (1, 80, 16), (4, "fff"): (1, 96, 12), (5, "fff"): (1, 96, 0),
-        (3, "dff"): (8, 256, 0), (4, "dff"): (4, 256, 0), (5, "dff"): (1, 128, 0)}
+        (3, "dff"): (1, 128, 0), (4, "dff"): (1, 128, 0), (5, "dff"): (1, 128, 0)}
 
 
 @pytest.mark.parametrize("N,prec", sorted(NODE))
@@ -83,7 +83,8 @@ def test_node_kernel_small_orders(node_report, N, prec):
     assert r["vgpr"] <= max_vgpr and r["scratch"] <= max_scratch, (N, PREC[prec], r)
 
 
-@pytest.mark.parametrize("N,prec,epb", [(3, "fff", 1), (4, "fff", 1), (3, "ddd", 2)])
+@pytest.mark.parametrize("N,prec,epb", [(3, "fff", 1), (4, "fff", 1), (3, "dff", 1),
+                                        (4, "dff", 1), (3, "ddd", 2)])
 def test_node_kernel_split_form(node_report, N, prec, epb):
     """The launch of a task-splitting order with one block per EPB elements
     of a task is a separate instance; it needs no more registers than the
